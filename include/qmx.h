@@ -1,0 +1,152 @@
+/*
+ * qmx -- C-ABI of the MI355X Sequential MPS encoder (libqmx.so).
+ *
+ * The reference (Qualition/qmprs) is pure Python over quimb/numpy and has no
+ * FFI; each entry point below replaces the reference routine named in its
+ * comment, and is bound from Python with ctypes (qmprs_amd/_lib.py).
+ *
+ * Conventions
+ *   - Every pointer marked "device" is device memory owned by the caller
+ *     (torch tensors); the library never allocates device memory.
+ *   - Scratch memory comes from a caller-provided workspace whose size is
+ *     returned by the matching *_workspace() query (256-byte aligned inside).
+ *   - Complex numbers are interleaved (re, im): complex128 = 2 doubles,
+ *     complex64 = 2 floats.  Dense states of N qubits hold 2^N amplitudes,
+ *     site 0 is the most significant bit of the amplitude index, and a batch
+ *     of B states is stored state after state.
+ *   - Gates are 4x4 complex128 matrices (out x in, row-major; a one-qubit gate
+ *     occupies the top-left 2x2) stored at gates[((b*L + layer)*N + site)*16],
+ *     with widths[(b*L + layer)*N + site] in {1, 2}.  Layer index = extraction
+ *     order (the circuit applies layer L_b-1 first, as sequential.py:396).
+ *   - Calls are asynchronous on `stream` (a hipStream_t, NULL = default)
+ *     unless stated; they return a QMX_* status.
+ *   - One handle per host thread.
+ */
+#ifndef QMX_H_
+#define QMX_H_
+
+#include <stddef.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define QMX_OK 0
+#define QMX_ERR_INVALID 1        /* bad argument          -> ValueError        */
+#define QMX_ERR_NOT_UNITARY 2    /* mps.py:837-839        -> ValueError        */
+#define QMX_ERR_NO_CONVERGENCE 3 /* eigensolver           -> RuntimeError      */
+#define QMX_ERR_HIP 4            /* HIP runtime failure   -> RuntimeError      */
+#define QMX_ERR_WORKSPACE 5      /* workspace too small   -> ValueError        */
+#define QMX_ERR_BASIS 6          /* degenerate projector  -> RuntimeError      */
+
+#define QMX_C64 0
+#define QMX_C128 1
+
+typedef struct qmx_handle qmx_handle;
+
+int qmx_version(void);
+int qmx_create(int device, qmx_handle** out);
+int qmx_destroy(qmx_handle* h);
+const char* qmx_last_error(const qmx_handle* h);
+
+/* (a)+(b) statevector -> chi-truncated MPS state.
+ * Replaces MPS.from_statevector (qmprs/primitives/mps.py:218-249): quick Ket
+ * normalisation (base.py:96-97), MatrixProductState.from_dense and
+ * tensor_network_1d_compress(max_bond=chi), evaluated as the right-to-left
+ * truncated projection chain with FP64 Jacobi eigensolves.
+ *   statevectors  device, B x 2^N of sv_dtype (QMX_C64 / QMX_C128)
+ *   psi_out       device, B x 2^N complex128: normalised input (may be NULL)
+ *   target_out    device, B x 2^N complex128: state of the chi-MPS (its norm
+ *                 is below 1 by the discarded weight, as quimb leaves it)
+ *   spectra       device or NULL, B x (N+1) x (2*chi) doubles: eigenvalues
+ *                 (squared singular values) of each truncated bond, decreasing */
+size_t qmx_chi_truncate_workspace(int num_qubits, int batch, int bond_dimension);
+int qmx_chi_truncate(qmx_handle* h, int num_qubits, int batch, int bond_dimension, double cutoff,
+                     const void* statevectors, int sv_dtype, void* psi_out, void* target_out,
+                     double* spectra, void* workspace, size_t workspace_bytes, void* stream);
+
+/* (b)+(c) one Ran layer from a normalised dense state.
+ * Replaces MPS.generate_bond_D_unitary_layer (mps.py:849-891) with
+ * generate_unitary_layer (mps.py:746-847): bond-2 compression in right form,
+ * block split (quick _get_submps_indices), first/two/last-site unitaries with
+ * Householder completion (scipy.linalg.null_space, mps.py:597,657,730) and
+ * the unitarity check.  Writes layer `layer` of gates/widths (L = num_layers).
+ *   psi_work device, B x 2^N complex128 (unit norm)
+ *   bonds    device or NULL, B x L x N ints: bond dims of the bond-2 MPS
+ * Synchronises the stream; returns QMX_ERR_NOT_UNITARY like mps.py:837-839. */
+size_t qmx_layer_workspace(int num_qubits, int batch);
+int qmx_generate_layer(qmx_handle* h, int num_qubits, int batch, int num_layers, int layer,
+                       double cutoff, const void* psi_work, void* gates, int* widths, int* bonds,
+                       void* workspace, size_t workspace_bytes, void* stream);
+
+/* (d) apply layer `layer` (inverse = 1: U^dagger, gates i = end..start as
+ * mps.py:933-971; inverse = 0: U, as mps.py:893-931) to dense states in place.
+ *   psi device, B x 2^N complex128 */
+int qmx_apply_layer(qmx_handle* h, int num_qubits, int batch, int num_layers, int layer,
+                    int inverse, void* psi, const void* gates, const int* widths, void* stream);
+
+/* (b)-(d) the whole layer loop of Sequential._get_unitary_layers
+ * (sequential.py:330-398): per layer, generate + apply the inverse layer and
+ * stop a state once |conj(psi[0]) - 1| <= (1 - fidelity_threshold) + 1e-5
+ * (sequential.py:390).  layers_used[b] = layers kept for state b.
+ *   psi_work  device, B x 2^N complex128, in: normalised chi-state, out:
+ *             disentangled state
+ *   amp0      device or NULL, B x L complex128: conj(psi[0]) after each layer
+ * Synchronises the stream. */
+size_t qmx_extract_layers_workspace(int num_qubits, int batch);
+int qmx_extract_layers(qmx_handle* h, int num_qubits, int batch, int num_layers,
+                       double fidelity_threshold, double cutoff, void* psi_work, void* gates,
+                       int* widths, int* bonds, int* layers_used, void* amp0, void* workspace,
+                       size_t workspace_bytes, void* stream);
+
+/* (e) environment sweeps (sequential.py:400-541, Sequential._optimize_unitary_layers):
+ * num_sweeps passes over the gates in reverse circuit order; per gate
+ * phi <- G^dagger phi, E = <T'|phi>_rest, G' = conj(polar(E)), T' <- G'^dagger T'.
+ *   target      device, B x 2^N of storage_dtype: the chi-MPS state
+ *   gates       device, in/out (updated in place)
+ *   psi_ref     device or NULL, B x 2^N complex128: fidelity reference
+ *   fidelity    device or NULL, B x (num_sweeps+1) complex128: <psi_ref|C|0>
+ *               before sweep s (s < num_sweeps) and after the last sweep
+ *   overlap     device or NULL, B x num_sweeps doubles: <T|C'|0> at the end of
+ *               each sweep (sum of singular values of the last environment)
+ *   goal        device or NULL, B doubles: a state stops sweeping once
+ *               |<psi_ref|C|0>| >= goal[b] (requires psi_ref; synchronises
+ *               once per sweep); sweeps_done (device or NULL, B ints) then
+ *               receives the number of sweeps each state ran.
+ *   circuit_out device or NULL, B x 2^N of storage_dtype: final C|0..0> */
+size_t qmx_sweep_workspace(int num_qubits, int batch, int storage_dtype);
+int qmx_sweep(qmx_handle* h, int num_qubits, int batch, int num_layers, const int* layers_used,
+              int num_sweeps, const void* target, int storage_dtype, void* gates,
+              const int* widths, const void* psi_ref, void* fidelity, double* overlap,
+              const double* goal, int* sweeps_done, void* circuit_out, void* workspace,
+              size_t workspace_bytes, void* stream);
+
+/* Whole Sequential.prepare_state (base.py:58-106 -> sequential.py:543-600)
+ * for a batch of same-size states: chi-truncation, layer extraction, sweeps
+ * and the final fidelity |<psi|C|0..0>|.  Gate emission into a circuit
+ * object stays on the host (sequential.py:155-213).
+ *   fidelity   device, B x (num_sweeps+1) complex128 (see qmx_sweep); when
+ *              num_sweeps == 0 a single entry per state
+ *   goal       device or NULL (see qmx_sweep) */
+typedef struct qmx_encode_config {
+  int num_qubits;
+  int batch;
+  int bond_dimension;
+  int num_layers;
+  int num_sweeps;
+  int sv_dtype;        /* QMX_C64 / QMX_C128 input statevectors */
+  int sweep_dtype;     /* storage of the sweep's dense states */
+  double fidelity_threshold;
+  double cutoff;
+} qmx_encode_config;
+
+size_t qmx_encode_workspace(const qmx_encode_config* cfg);
+int qmx_encode(qmx_handle* h, const qmx_encode_config* cfg, const void* statevectors,
+               void* gates, int* widths, int* layers_used, void* fidelity, const double* goal,
+               int* sweeps_done, void* workspace, size_t workspace_bytes, void* stream);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* QMX_H_ */

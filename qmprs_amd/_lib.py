@@ -1,0 +1,139 @@
+"""ctypes binding of libqmx.so (the C-ABI declared in include/qmx.h).
+
+Device buffers are torch tensors on a ROCm GPU; their ``data_ptr()`` and the
+current stream handle are passed straight through.  There is no CPU path: if
+the library or the GPU is missing every entry point raises.
+"""
+
+from __future__ import annotations
+
+import ctypes
+import threading
+from pathlib import Path
+
+import torch
+
+LIB_PATH = Path(__file__).resolve().with_name("libqmx.so")
+
+QMX_OK = 0
+QMX_ERR_INVALID = 1
+QMX_ERR_NOT_UNITARY = 2
+QMX_ERR_NO_CONVERGENCE = 3
+QMX_ERR_HIP = 4
+QMX_ERR_WORKSPACE = 5
+QMX_ERR_BASIS = 6
+
+QMX_C64 = 0
+QMX_C128 = 1
+
+_VP = ctypes.c_void_p
+_INT = ctypes.c_int
+_DBL = ctypes.c_double
+_SZ = ctypes.c_size_t
+
+
+class EncodeConfig(ctypes.Structure):
+    """Mirror of ``qmx_encode_config``."""
+    _fields_ = [
+        ("num_qubits", _INT),
+        ("batch", _INT),
+        ("bond_dimension", _INT),
+        ("num_layers", _INT),
+        ("num_sweeps", _INT),
+        ("sv_dtype", _INT),
+        ("sweep_dtype", _INT),
+        ("fidelity_threshold", _DBL),
+        ("cutoff", _DBL),
+    ]
+
+
+# name -> (restype, argtypes)
+SIGNATURES = {
+    "qmx_version": (_INT, []),
+    "qmx_create": (_INT, [_INT, ctypes.POINTER(_VP)]),
+    "qmx_destroy": (_INT, [_VP]),
+    "qmx_last_error": (ctypes.c_char_p, [_VP]),
+    "qmx_chi_truncate_workspace": (_SZ, [_INT, _INT, _INT]),
+    "qmx_chi_truncate": (_INT, [_VP, _INT, _INT, _INT, _DBL, _VP, _INT, _VP, _VP, _VP, _VP, _SZ, _VP]),
+    "qmx_layer_workspace": (_SZ, [_INT, _INT]),
+    "qmx_generate_layer": (_INT, [_VP, _INT, _INT, _INT, _INT, _DBL, _VP, _VP, _VP, _VP, _VP, _SZ, _VP]),
+    "qmx_apply_layer": (_INT, [_VP, _INT, _INT, _INT, _INT, _INT, _VP, _VP, _VP, _VP]),
+    "qmx_extract_layers_workspace": (_SZ, [_INT, _INT]),
+    "qmx_extract_layers": (_INT, [_VP, _INT, _INT, _INT, _DBL, _DBL, _VP, _VP, _VP, _VP, _VP, _VP,
+                                  _VP, _SZ, _VP]),
+    "qmx_sweep_workspace": (_SZ, [_INT, _INT, _INT]),
+    "qmx_sweep": (_INT, [_VP, _INT, _INT, _INT, _VP, _INT, _VP, _INT, _VP, _VP, _VP, _VP, _VP, _VP,
+                         _VP, _VP, _VP, _SZ, _VP]),
+    "qmx_encode_workspace": (_SZ, [ctypes.POINTER(EncodeConfig)]),
+    "qmx_encode": (_INT, [_VP, ctypes.POINTER(EncodeConfig), _VP, _VP, _VP, _VP, _VP, _VP, _VP, _VP,
+                          _SZ, _VP]),
+}
+
+_lib = None
+_lock = threading.Lock()
+_local = threading.local()
+
+
+def load() -> ctypes.CDLL:
+    """Load libqmx.so and declare every exported signature."""
+    global _lib
+    with _lock:
+        if _lib is None:
+            if not LIB_PATH.exists():
+                raise RuntimeError(
+                    f"{LIB_PATH} is not built; run `python -m qmprs_amd.build` (hipcc, gfx950)")
+            lib = ctypes.CDLL(str(LIB_PATH))
+            for name, (res, args) in SIGNATURES.items():
+                fn = getattr(lib, name)
+                fn.restype = res
+                fn.argtypes = args
+            _lib = lib
+    return _lib
+
+
+def require_gpu() -> torch.device:
+    if not torch.cuda.is_available():
+        raise RuntimeError("qmprs_amd runs on an AMD GPU (ROCm); no GPU is visible")
+    return torch.device("cuda", torch.cuda.current_device())
+
+
+def handle() -> ctypes.c_void_p:
+    """Per-thread, per-device qmx handle."""
+    dev = require_gpu()
+    lib = load()
+    cache = getattr(_local, "handles", None)
+    if cache is None:
+        cache = _local.handles = {}
+    h = cache.get(dev.index)
+    if h is None:
+        h = _VP()
+        st = lib.qmx_create(dev.index, ctypes.byref(h))
+        if st != QMX_OK:
+            raise RuntimeError(f"qmx_create failed on device {dev.index} (status {st})")
+        cache[dev.index] = h
+    return h
+
+
+def stream_ptr() -> ctypes.c_void_p:
+    return _VP(torch.cuda.current_stream().cuda_stream)
+
+
+def ptr(t) -> ctypes.c_void_p:
+    return _VP(0 if t is None else t.data_ptr())
+
+
+def check(status: int, h) -> None:
+    """Map a QMX_* status to the reference's exception types."""
+    if status == QMX_OK:
+        return
+    msg = load().qmx_last_error(h)
+    msg = msg.decode() if msg else f"qmx status {status}"
+    if status in (QMX_ERR_INVALID, QMX_ERR_NOT_UNITARY, QMX_ERR_WORKSPACE):
+        raise ValueError(msg)
+    raise RuntimeError(msg)
+
+
+def workspace(nbytes: int, device) -> "torch.Tensor":
+    if nbytes == 0:
+        raise ValueError("invalid problem size for the qmx workspace")
+    return torch.empty(int(nbytes), dtype=torch.uint8, device=device)

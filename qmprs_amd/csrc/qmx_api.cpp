@@ -1,0 +1,728 @@
+// C-ABI entry points and host orchestration of libqmx (see include/qmx.h).
+//
+// The host code only enqueues kernels on the caller's stream; every shape that
+// depends on data (bond dimensions after a cutoff, layers kept after the early
+// exit, states that reached their fidelity goal) stays on the device and the
+// kernels read it, so a whole prepare_state runs without host round trips
+// except the error checks at the end of the chain and layer phases.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cstring>
+#include <string>
+#include <vector>
+
+#include "qmx.h"
+#include "qmx_internal.h"
+
+struct qmx_handle {
+  int device;
+  int cus;
+  std::string err;
+};
+
+namespace {
+using namespace qmx;
+
+constexpr int kVersion = 1;
+
+struct Carver {
+  char* base;
+  size_t off = 0;
+  explicit Carver(void* b) : base(static_cast<char*>(b)) {}
+  template <typename T>
+  T* take(size_t count) {
+    off = (off + 255) & ~static_cast<size_t>(255);
+    T* p = base ? reinterpret_cast<T*>(base + off) : nullptr;
+    off += count * sizeof(T);
+    return p;
+  }
+};
+
+int fail(qmx_handle* h, int code, const std::string& msg) {
+  if (h) h->err = msg;
+  return code;
+}
+
+int hip_fail(qmx_handle* h, hipError_t e, const char* what) {
+  return fail(h, QMX_ERR_HIP, std::string(what) + ": " + hipGetErrorString(e));
+}
+
+#define QMX_HIP(h, expr)                                  \
+  do {                                                    \
+    hipError_t e_ = (expr);                               \
+    if (e_ != hipSuccess) return hip_fail(h, e_, #expr);  \
+  } while (0)
+
+#define QMX_LAUNCHED(h, what)                             \
+  do {                                                    \
+    hipError_t e_ = hipGetLastError();                    \
+    if (e_ != hipSuccess) return hip_fail(h, e_, what);   \
+  } while (0)
+
+#define QMX_CALL(expr)            \
+  do {                            \
+    int s_ = (expr);              \
+    if (s_ != QMX_OK) return s_;  \
+  } while (0)
+
+inline size_t state_len(int n) { return static_cast<size_t>(1) << n; }
+
+int blocks_for(long ntiles, int batch, int cus) {
+  long target = 4L * cus;
+  long per = (target + batch - 1) / batch;
+  if (per < 1) per = 1;
+  if (per > ntiles) per = ntiles;
+  if (per > 65535) per = 65535;
+  return static_cast<int>(per);
+}
+
+int check_common(qmx_handle* h, int n, int batch) {
+  if (!h) return QMX_ERR_INVALID;
+  if (n < 2 || n > kMaxQubits) return fail(h, QMX_ERR_INVALID, "num_qubits must be in [2, 30]");
+  if (batch < 1 || batch > 65535) return fail(h, QMX_ERR_INVALID, "batch must be in [1, 65535]");
+  return QMX_OK;
+}
+
+int device_error_status(qmx_handle* h, int* err_dev, hipStream_t st) {
+  int err = 0;
+  QMX_HIP(h, hipMemcpyAsync(&err, err_dev, sizeof(int), hipMemcpyDeviceToHost, st));
+  QMX_HIP(h, hipStreamSynchronize(st));
+  if (err & kErrNotUnitary) return fail(h, QMX_ERR_NOT_UNITARY, "All the generated unitaries must be unitary.");
+  if (err & kErrEigNoConv) return fail(h, QMX_ERR_NO_CONVERGENCE, "Jacobi eigensolver did not converge");
+  if (err & kErrBasis) return fail(h, QMX_ERR_BASIS, "degenerate projector in the canonical gauge");
+  return QMX_OK;
+}
+
+// Worst-case gate bits of site i (width 2 unless it is the last site).
+uint64_t worst_bits(int n, int site) {
+  uint64_t m = 1ull << (n - 1 - site);
+  if (site < n - 1) m |= 1ull << (n - 2 - site);
+  return m;
+}
+
+// Greedy grouping of a gate sequence into passes whose bit union fits a tile.
+std::vector<std::vector<int>> plan_passes(int n, const std::vector<int>& pos,
+                                          const std::vector<int>& sites, int max_union) {
+  std::vector<std::vector<int>> passes;
+  std::vector<int> cur;
+  uint64_t um = 0;
+  for (size_t j = 0; j < pos.size(); ++j) {
+    uint64_t nb = um | worst_bits(n, sites[j]);
+    if (!cur.empty() && (__builtin_popcountll(nb) > max_union ||
+                         static_cast<int>(cur.size()) >= kMaxPassGates)) {
+      passes.push_back(cur);
+      cur.clear();
+      nb = worst_bits(n, sites[j]);
+    }
+    cur.push_back(pos[j]);
+    um = nb;
+  }
+  if (!cur.empty()) passes.push_back(cur);
+  return passes;
+}
+
+// The Jacobi kernel handles matrices up to 512 x 512, i.e. chi <= 256 whenever
+// the chain truncates (2^(N-1) > chi).
+bool chi_supported(int n, int chi) { return chi <= 256 || (1l << (n - 1)) <= chi; }
+
+// ------------------------------------------------------------ layouts -------
+struct ChiLayout {
+  double2* psi;      // only when the caller gives no psi_out
+  double2* xa;
+  double2* xb;
+  double* nrm;
+  double* npart;
+  int* ncnt;
+  double2* a;
+  double2* v;
+  double2* w;        // [i_first + 1][B][nmax^2]
+  double2* part;
+  int* nn;
+  int* kk;
+  int* ncur;
+  int* error;
+  int i_first, nmax, nchunks_max, nblk_norm;
+};
+
+ChiLayout chi_layout(Carver& c, int n, int batch, int chi, bool need_psi) {
+  ChiLayout L{};
+  size_t len = state_len(n);
+  int i_first = 0;
+  for (int i = n - 1; i >= 1; --i)
+    if ((1l << (n - i)) > chi) { i_first = i; break; }
+  L.i_first = i_first;
+  // n_i <= 2 k_{i+1} <= 2 chi, and n_{i_first} = 2^(N - i_first) <= 2 chi
+  L.nmax = i_first ? 2 * chi : 2;
+  L.nchunks_max = 16;
+  L.nblk_norm = 256;
+  L.psi = need_psi ? c.take<double2>(len * batch) : nullptr;
+  L.xa = i_first ? c.take<double2>(len * batch) : nullptr;
+  L.xb = i_first ? c.take<double2>(len * batch) : nullptr;
+  L.nrm = c.take<double>(batch);
+  L.npart = c.take<double>(static_cast<size_t>(batch) * L.nblk_norm * 2);
+  L.ncnt = c.take<int>(batch);
+  size_t nm2 = static_cast<size_t>(L.nmax) * L.nmax;
+  if (i_first) {
+    L.a = c.take<double2>(nm2 * batch);
+    L.v = c.take<double2>(nm2 * batch);
+    L.w = c.take<double2>(nm2 * batch * (i_first + 1));
+    L.part = c.take<double2>(nm2 * batch * L.nchunks_max);
+  }
+  L.nn = c.take<int>(static_cast<size_t>(batch) * (n + 1));
+  L.kk = c.take<int>(static_cast<size_t>(batch) * (n + 1));
+  L.ncur = c.take<int>(batch);
+  L.error = c.take<int>(1);
+  return L;
+}
+
+struct LayerLayout {
+  double2* xa;
+  double2* xb;
+  int* r;
+  double2* w;
+  double* part;
+  int* counter;
+  int* skip;
+  int* error;
+};
+constexpr int kChain2MaxBlocks = 256;
+
+LayerLayout layer_layout(Carver& c, int n, int batch) {
+  LayerLayout L{};
+  size_t len = state_len(n);
+  L.xa = c.take<double2>(len * batch);
+  L.xb = c.take<double2>(len * batch);
+  L.r = c.take<int>(static_cast<size_t>(batch) * (n + 1));
+  L.w = c.take<double2>(static_cast<size_t>(batch) * (n + 1) * 16);
+  L.part = c.take<double>(static_cast<size_t>(batch) * kChain2MaxBlocks * 32);
+  L.counter = c.take<int>(batch);
+  L.skip = c.take<int>(batch);
+  L.error = c.take<int>(1);
+  return L;
+}
+
+template <typename S>
+struct SweepLayout {
+  S* phi;
+  S* tprime;
+  double* part;
+  int* counter;
+  double* dot_part;
+  int* dot_counter;
+  int* skip;
+  int* active;
+  int nblk_max;
+};
+
+template <typename S>
+SweepLayout<S> sweep_layout(Carver& c, int n, int batch) {
+  SweepLayout<S> L{};
+  size_t len = state_len(n);
+  L.nblk_max = 65535;
+  L.phi = c.take<S>(len * batch);
+  L.tprime = c.take<S>(len * batch);
+  // partial buffers sized for the largest grid any pass can use
+  long ntiles = 1l << (n - std::min(n, kSweepTileBits));
+  long ntiles_apply = 1l << (n - std::min(n, kApplyTileBits));
+  long mx = std::max(ntiles, ntiles_apply);
+  L.part = c.take<double>(static_cast<size_t>(batch) * mx * 32);
+  L.counter = c.take<int>(batch);
+  L.dot_part = c.take<double>(static_cast<size_t>(batch) * mx * 2);
+  L.dot_counter = c.take<int>(batch);
+  L.skip = c.take<int>(batch);
+  L.active = c.take<int>(1);
+  return L;
+}
+
+// ------------------------------------------------------------- phases -------
+int run_chi(qmx_handle* h, int n, int batch, int chi, double cutoff, const void* sv, int sv_dtype,
+            double2* psi, double2* target, double* spectra, const ChiLayout& L, hipStream_t st) {
+  const size_t len = state_len(n);
+  launch_fill_int(L.ncnt, 0, batch, st);
+  launch_fill_int(L.error, 0, 1, st);
+  int nb = static_cast<int>(std::min<size_t>(L.nblk_norm, std::max<size_t>(1, len / 1024)));
+  if (sv_dtype == QMX_C64) {
+    launch_norm2(static_cast<const float2*>(sv), n, batch, L.npart, L.ncnt, L.nrm, nb, st);
+    launch_scale_convert(static_cast<const float2*>(sv), psi, n, batch, L.nrm, st);
+  } else {
+    launch_norm2(static_cast<const double2*>(sv), n, batch, L.npart, L.ncnt, L.nrm, nb, st);
+    launch_scale_convert(static_cast<const double2*>(sv), psi, n, batch, L.nrm, st);
+  }
+  QMX_LAUNCHED(h, "normalise");
+  const int i_first = L.i_first;
+  if (i_first == 0) {
+    launch_scale_convert(static_cast<const double2*>(psi), target, n, batch, nullptr, st);
+    QMX_LAUNCHED(h, "copy");
+    return QMX_OK;
+  }
+  const int nmax = L.nmax;
+  const size_t nm2 = static_cast<size_t>(nmax) * nmax;
+  const int kmax = std::min(chi, nmax);
+  auto xbuf = [&](int i) { return ((i_first - i) % 2 == 0) ? L.xa : L.xb; };
+  // forward chain: X_i = Y_i W_i
+  for (int i = i_first; i >= 1; --i) {
+    const double2* y = (i == i_first) ? psi : xbuf(i + 1);
+    launch_chain_dims(L.nn, L.ncur, L.kk, n, i, (i == i_first) ? (1 << (n - i)) : 0, batch, st);
+    int rows = 1 << i;
+    int nchunks = std::max(1, std::min(L.nchunks_max, rows / 256));
+    launch_gram(y, len, L.ncur, i, nmax, L.part, nchunks, batch, st);
+    launch_gram_reduce(L.part, nchunks, L.ncur, nmax, L.a, nm2, batch, st);
+    launch_jacobi(L.a, L.v, nm2, L.ncur, nmax, batch, L.error, st);
+    double2* wi = L.w + static_cast<size_t>(i) * batch * nm2;
+    launch_select(L.a, L.v, wi, nm2, L.ncur, nmax, chi, cutoff, L.kk + i, n + 1,
+                  spectra ? spectra + static_cast<size_t>(i) * 2 * chi : nullptr,
+                  static_cast<size_t>(n + 1) * 2 * chi, batch, st);
+    GemmDims d{i, L.nn + i, L.kk + i, n + 1, nmax, kmax, nmax};
+    launch_gemm(y, len, wi, nm2, xbuf(i), len, d, 0, batch, st);
+    QMX_LAUNCHED(h, "chi chain step");
+  }
+  // contraction back: Z_{i+1} = Z_i W_i^H, Z_1 = X_1
+  const double2* z = xbuf(1);
+  for (int i = 1; i <= i_first; ++i) {
+    double2* out = (i == i_first) ? target : (z == L.xa ? L.xb : L.xa);
+    const double2* wi = L.w + static_cast<size_t>(i) * batch * nm2;
+    GemmDims d{i, L.kk + i, L.nn + i, n + 1, kmax, nmax, nmax};
+    launch_gemm(z, len, wi, nm2, out, len, d, 1, batch, st);
+    QMX_LAUNCHED(h, "chi contraction step");
+    z = out;
+  }
+  return QMX_OK;
+}
+
+int run_generate_layer(qmx_handle* h, int n, int batch, int lmax, int layer, double cutoff,
+                       const double2* psi_w, double2* gates, int* widths, int* bonds,
+                       const LayerLayout& L, const int* skip, hipStream_t st) {
+  for (int i = n - 1; i >= 1; --i) {
+    Chain2Args a{};
+    a.n = n;
+    a.step = i;
+    a.cutoff = cutoff;
+    a.psi = psi_w;
+    a.xa = L.xa;
+    a.xb = L.xb;
+    a.r = L.r;
+    a.w = L.w;
+    a.spectra = nullptr;
+    a.skip = skip;
+    a.part = L.part;
+    a.counter = L.counter;
+    a.error = L.error;
+    long rows = 1l << i;
+    int nblk = static_cast<int>(std::max(1l, std::min<long>(kChain2MaxBlocks, rows / 1024)));
+    launch_chain2_step(a, nblk, batch, st);
+  }
+  RanArgs r{};
+  r.n = n;
+  r.lmax = lmax;
+  r.layer = layer;
+  r.psi = psi_w;
+  r.xa = L.xa;
+  r.xb = L.xb;
+  r.r = L.r;
+  r.w = L.w;
+  r.gates = gates;
+  r.widths = widths;
+  r.bonds = bonds;
+  r.skip = skip;
+  r.error = L.error;
+  launch_ran_gates(r, batch, st);
+  QMX_LAUNCHED(h, "generate layer");
+  return QMX_OK;
+}
+
+template <typename S>
+int run_passes(qmx_handle* h, int n, int batch, int lmax, int mode, int layer,
+               const std::vector<std::vector<int>>& passes, S* psi, const double2* gates,
+               const int* widths, const int* layers, const int* skip, const double2* ref,
+               double* dot_part, int* dot_counter, double2* dot_out, int dot_stride, int cus,
+               hipStream_t st) {
+  const int t = std::min(n, kApplyTileBits);
+  const long ntiles = 1l << (n - t);
+  const int nblk = blocks_for(ntiles, batch, cus);
+  for (size_t p = 0; p < passes.size(); ++p) {
+    PassArgs<S> a{};
+    a.n = n;
+    a.lmax = lmax;
+    a.tbits = kApplyTileBits;
+    a.mode = mode;
+    a.layer = layer;
+    a.count = static_cast<int>(passes[p].size());
+    for (int j = 0; j < a.count; ++j) a.pos[j] = passes[p][j];
+    a.psi = psi;
+    a.gates = gates;
+    a.widths = widths;
+    a.layers = layers;
+    a.skip = skip;
+    bool last = (p + 1 == passes.size());
+    a.ref = last ? ref : nullptr;
+    a.dot_part = dot_part;
+    a.dot_counter = dot_counter;
+    a.dot_out = dot_out;
+    a.dot_stride = dot_stride;
+    launch_apply_pass(a, nblk, batch, st);
+  }
+  QMX_LAUNCHED(h, "apply passes");
+  return QMX_OK;
+}
+
+std::vector<std::vector<int>> inverse_layer_passes(int n) {
+  std::vector<int> pos, sites;
+  for (int i = n - 1; i >= 0; --i) { pos.push_back(i); sites.push_back(i); }
+  return plan_passes(n, pos, sites, kApplyMaxUnion);
+}
+
+std::vector<std::vector<int>> forward_layer_passes(int n) {
+  std::vector<int> pos, sites;
+  for (int i = 0; i < n; ++i) { pos.push_back(i); sites.push_back(i); }
+  return plan_passes(n, pos, sites, kApplyMaxUnion);
+}
+
+std::vector<std::vector<int>> forward_circuit_passes(int n, int lmax) {
+  std::vector<int> pos, sites;
+  for (int f = 0; f < lmax * n; ++f) { pos.push_back(f); sites.push_back(f % n); }
+  return plan_passes(n, pos, sites, kApplyMaxUnion);
+}
+
+int run_extract(qmx_handle* h, int n, int batch, int lmax, double thr, double cutoff,
+                double2* psi_w, double2* gates, int* widths, int* bonds, int* layers_used,
+                double2* amp0, const LayerLayout& L, hipStream_t st) {
+  launch_fill_int(L.skip, 0, batch, st);
+  launch_fill_int(L.counter, 0, batch, st);
+  launch_fill_int(L.error, 0, 1, st);
+  launch_fill_int(layers_used, 0, batch, st);
+  launch_fill_int(widths, 1, batch * lmax * n, st);
+  auto inv = inverse_layer_passes(n);
+  for (int l = 0; l < lmax; ++l) {
+    QMX_CALL(run_generate_layer(h, n, batch, lmax, l, cutoff, psi_w, gates, widths, bonds, L,
+                                L.skip, st));
+    QMX_CALL(run_passes<double2>(h, n, batch, lmax, 0, l, inv, psi_w, gates, widths, nullptr,
+                                 L.skip, nullptr, nullptr, nullptr, nullptr, 1, h->cus, st));
+    Amp0Args a{};
+    a.n = n;
+    a.batch = batch;
+    a.lmax = lmax;
+    a.layer = l;
+    a.atol = 1.0 - thr;
+    a.psi = psi_w;
+    a.skip = L.skip;
+    a.layers = layers_used;
+    a.amp0 = amp0;
+    launch_amp0(a, st);
+    QMX_LAUNCHED(h, "early exit check");
+  }
+  return device_error_status(h, L.error, st);
+}
+
+template <typename S>
+int run_sweep(qmx_handle* h, int n, int batch, int lmax, const int* layers_used, int nsweeps,
+              const S* target, double2* gates, const int* widths, const double2* psi_ref,
+              double2* fidelity, double* overlap, const double* goal, int* sweeps_done,
+              S* circuit_out, const SweepLayout<S>& L, hipStream_t st) {
+  const int kmax = lmax * n;
+  launch_fill_int(L.counter, 0, batch, st);
+  launch_fill_int(L.dot_counter, 0, batch, st);
+  launch_fill_int(L.skip, 0, batch, st);
+  if (fidelity) launch_fill_int(reinterpret_cast<int*>(fidelity), 0, batch * (nsweeps + 1) * 4, st);
+  if (sweeps_done) launch_fill_int(sweeps_done, goal ? 0 : nsweeps, batch, st);
+  const bool use_goal = goal && psi_ref && fidelity;
+  const int* skip = use_goal ? L.skip : nullptr;
+  auto rebuild = forward_circuit_passes(n, lmax);
+  const int ts = std::min(n, kSweepTileBits);
+  const int nblk = blocks_for(1l << (n - ts), batch, h->cus);
+  for (int s = 0; s < nsweeps; ++s) {
+    launch_basis0(L.phi, n, batch, skip, st);
+    QMX_CALL(run_passes<S>(h, n, batch, lmax, 1, 0, rebuild, L.phi, gates, widths, layers_used,
+                           skip, fidelity ? psi_ref : nullptr, L.dot_part, L.dot_counter,
+                           fidelity ? fidelity + s : nullptr, nsweeps + 1, h->cus, st));
+    if (use_goal) {
+      launch_fill_int(L.active, 0, 1, st);
+      launch_goal_check(fidelity + s, nsweeps + 1, goal, L.skip, L.active, sweeps_done, batch, st);
+      int active = 0;
+      QMX_HIP(h, hipMemcpyAsync(&active, L.active, sizeof(int), hipMemcpyDeviceToHost, st));
+      QMX_HIP(h, hipStreamSynchronize(st));
+      if (active == 0) break;
+    }
+    for (int k = 0; k < kmax; ++k) {
+      SweepArgs<S> a{};
+      a.n = n;
+      a.lmax = lmax;
+      a.tbits = kSweepTileBits;
+      a.step = k;
+      a.target = target;
+      a.tprime = L.tprime;
+      a.phi = L.phi;
+      a.gates = gates;
+      a.widths = widths;
+      a.layers = layers_used;
+      a.skip = skip;
+      a.part = L.part;
+      a.counter = L.counter;
+      a.overlap = overlap ? overlap + s : nullptr;
+      a.overlap_stride = nsweeps;
+      launch_sweep_step(a, nblk, batch, st);
+    }
+    QMX_LAUNCHED(h, "sweep");
+  }
+  // final circuit state and fidelity
+  launch_basis0(L.phi, n, batch, nullptr, st);
+  QMX_CALL(run_passes<S>(h, n, batch, lmax, 1, 0, rebuild, L.phi, gates, widths, layers_used,
+                         nullptr, fidelity ? psi_ref : nullptr, L.dot_part, L.dot_counter,
+                         fidelity ? fidelity + nsweeps : nullptr, nsweeps + 1, h->cus, st));
+  if (circuit_out) launch_scale_convert<S, S>(L.phi, circuit_out, n, batch, nullptr, st);
+  QMX_LAUNCHED(h, "final circuit state");
+  return QMX_OK;
+}
+
+}  // namespace
+
+// =================================================================== C-ABI ==
+extern "C" {
+
+int qmx_version(void) { return kVersion; }
+
+int qmx_create(int device, qmx_handle** out) {
+  if (!out) return QMX_ERR_INVALID;
+  int count = 0;
+  if (hipGetDeviceCount(&count) != hipSuccess || device < 0 || device >= count) {
+    *out = nullptr;
+    return QMX_ERR_HIP;
+  }
+  hipDeviceProp_t prop;
+  if (hipGetDeviceProperties(&prop, device) != hipSuccess) return QMX_ERR_HIP;
+  qmx_handle* h = new qmx_handle();
+  h->device = device;
+  h->cus = prop.multiProcessorCount > 0 ? prop.multiProcessorCount : 256;
+  *out = h;
+  return QMX_OK;
+}
+
+int qmx_destroy(qmx_handle* h) {
+  delete h;
+  return QMX_OK;
+}
+
+const char* qmx_last_error(const qmx_handle* h) { return h ? h->err.c_str() : "null handle"; }
+
+size_t qmx_chi_truncate_workspace(int num_qubits, int batch, int bond_dimension) {
+  if (num_qubits < 2 || num_qubits > kMaxQubits || batch < 1 || bond_dimension < 1) return 0;
+  Carver c(nullptr);
+  chi_layout(c, num_qubits, batch, bond_dimension, true);
+  return c.off + 256;
+}
+
+int qmx_chi_truncate(qmx_handle* h, int num_qubits, int batch, int bond_dimension, double cutoff,
+                     const void* statevectors, int sv_dtype, void* psi_out, void* target_out,
+                     double* spectra, void* workspace, size_t workspace_bytes, void* stream) {
+  QMX_CALL(check_common(h, num_qubits, batch));
+  if (bond_dimension < 1) return fail(h, QMX_ERR_INVALID, "`bond_dimension` must be an integer greater than 0.");
+  if (!statevectors || !target_out || !workspace) return fail(h, QMX_ERR_INVALID, "null pointer");
+  if (sv_dtype != QMX_C64 && sv_dtype != QMX_C128) return fail(h, QMX_ERR_INVALID, "sv_dtype");
+  if (!chi_supported(num_qubits, bond_dimension))
+    return fail(h, QMX_ERR_INVALID, "bond_dimension above 256 with truncation is not supported");
+  if (workspace_bytes < qmx_chi_truncate_workspace(num_qubits, batch, bond_dimension))
+    return fail(h, QMX_ERR_WORKSPACE, "workspace too small");
+  QMX_HIP(h, hipSetDevice(h->device));
+  hipStream_t st = static_cast<hipStream_t>(stream);
+  Carver c(workspace);
+  ChiLayout L = chi_layout(c, num_qubits, batch, bond_dimension, true);
+  double2* psi = psi_out ? static_cast<double2*>(psi_out) : L.psi;
+  QMX_CALL(run_chi(h, num_qubits, batch, bond_dimension, cutoff, statevectors, sv_dtype, psi,
+                   static_cast<double2*>(target_out), spectra, L, st));
+  return device_error_status(h, L.error, st);
+}
+
+size_t qmx_layer_workspace(int num_qubits, int batch) {
+  if (num_qubits < 2 || num_qubits > kMaxQubits || batch < 1) return 0;
+  Carver c(nullptr);
+  layer_layout(c, num_qubits, batch);
+  return c.off + 256;
+}
+
+int qmx_generate_layer(qmx_handle* h, int num_qubits, int batch, int num_layers, int layer,
+                       double cutoff, const void* psi_work, void* gates, int* widths, int* bonds,
+                       void* workspace, size_t workspace_bytes, void* stream) {
+  QMX_CALL(check_common(h, num_qubits, batch));
+  if (num_layers < 1 || layer < 0 || layer >= num_layers) return fail(h, QMX_ERR_INVALID, "layer index");
+  if (!psi_work || !gates || !widths || !workspace) return fail(h, QMX_ERR_INVALID, "null pointer");
+  if (workspace_bytes < qmx_layer_workspace(num_qubits, batch))
+    return fail(h, QMX_ERR_WORKSPACE, "workspace too small");
+  QMX_HIP(h, hipSetDevice(h->device));
+  hipStream_t st = static_cast<hipStream_t>(stream);
+  Carver c(workspace);
+  LayerLayout L = layer_layout(c, num_qubits, batch);
+  launch_fill_int(L.counter, 0, batch, st);
+  launch_fill_int(L.error, 0, 1, st);
+  QMX_CALL(run_generate_layer(h, num_qubits, batch, num_layers, layer, cutoff,
+                              static_cast<const double2*>(psi_work), static_cast<double2*>(gates),
+                              widths, bonds, L, nullptr, st));
+  return device_error_status(h, L.error, st);
+}
+
+int qmx_apply_layer(qmx_handle* h, int num_qubits, int batch, int num_layers, int layer,
+                    int inverse, void* psi, const void* gates, const int* widths, void* stream) {
+  QMX_CALL(check_common(h, num_qubits, batch));
+  if (num_layers < 1 || layer < 0 || layer >= num_layers) return fail(h, QMX_ERR_INVALID, "layer index");
+  if (!psi || !gates || !widths) return fail(h, QMX_ERR_INVALID, "null pointer");
+  QMX_HIP(h, hipSetDevice(h->device));
+  hipStream_t st = static_cast<hipStream_t>(stream);
+  const int n = num_qubits;
+  if (inverse) {
+    return run_passes<double2>(h, n, batch, num_layers, 0, layer, inverse_layer_passes(n),
+                               static_cast<double2*>(psi), static_cast<const double2*>(gates),
+                               widths, nullptr, nullptr, nullptr, nullptr, nullptr, nullptr, 1,
+                               h->cus, st);
+  }
+  // forward: positions f = (L-1-layer)*n + i with layers = nullptr -> L
+  std::vector<int> pos, sites;
+  for (int i = 0; i < n; ++i) {
+    pos.push_back((num_layers - 1 - layer) * n + i);
+    sites.push_back(i);
+  }
+  return run_passes<double2>(h, n, batch, num_layers, 1, 0, plan_passes(n, pos, sites, kApplyMaxUnion),
+                             static_cast<double2*>(psi), static_cast<const double2*>(gates), widths,
+                             nullptr, nullptr, nullptr, nullptr, nullptr, nullptr, 1, h->cus, st);
+}
+
+size_t qmx_extract_layers_workspace(int num_qubits, int batch) {
+  return qmx_layer_workspace(num_qubits, batch);
+}
+
+int qmx_extract_layers(qmx_handle* h, int num_qubits, int batch, int num_layers,
+                       double fidelity_threshold, double cutoff, void* psi_work, void* gates,
+                       int* widths, int* bonds, int* layers_used, void* amp0, void* workspace,
+                       size_t workspace_bytes, void* stream) {
+  QMX_CALL(check_common(h, num_qubits, batch));
+  if (num_layers < 1) return fail(h, QMX_ERR_INVALID, "The number of layers must be a positive integer.");
+  if (!psi_work || !gates || !widths || !layers_used || !workspace)
+    return fail(h, QMX_ERR_INVALID, "null pointer");
+  if (workspace_bytes < qmx_extract_layers_workspace(num_qubits, batch))
+    return fail(h, QMX_ERR_WORKSPACE, "workspace too small");
+  QMX_HIP(h, hipSetDevice(h->device));
+  hipStream_t st = static_cast<hipStream_t>(stream);
+  Carver c(workspace);
+  LayerLayout L = layer_layout(c, num_qubits, batch);
+  return run_extract(h, num_qubits, batch, num_layers, fidelity_threshold, cutoff,
+                     static_cast<double2*>(psi_work), static_cast<double2*>(gates), widths, bonds,
+                     layers_used, static_cast<double2*>(amp0), L, st);
+}
+
+size_t qmx_sweep_workspace(int num_qubits, int batch, int storage_dtype) {
+  if (num_qubits < 2 || num_qubits > kMaxQubits || batch < 1) return 0;
+  Carver c(nullptr);
+  if (storage_dtype == QMX_C64) sweep_layout<float2>(c, num_qubits, batch);
+  else sweep_layout<double2>(c, num_qubits, batch);
+  return c.off + 256;
+}
+
+int qmx_sweep(qmx_handle* h, int num_qubits, int batch, int num_layers, const int* layers_used,
+              int num_sweeps, const void* target, int storage_dtype, void* gates,
+              const int* widths, const void* psi_ref, void* fidelity, double* overlap,
+              const double* goal, int* sweeps_done, void* circuit_out, void* workspace,
+              size_t workspace_bytes, void* stream) {
+  QMX_CALL(check_common(h, num_qubits, batch));
+  if (num_layers < 1 || num_sweeps < 0) return fail(h, QMX_ERR_INVALID, "num_layers / num_sweeps");
+  if (!layers_used || !target || !gates || !widths || !workspace)
+    return fail(h, QMX_ERR_INVALID, "null pointer");
+  if (storage_dtype != QMX_C64 && storage_dtype != QMX_C128) return fail(h, QMX_ERR_INVALID, "storage_dtype");
+  if (workspace_bytes < qmx_sweep_workspace(num_qubits, batch, storage_dtype))
+    return fail(h, QMX_ERR_WORKSPACE, "workspace too small");
+  QMX_HIP(h, hipSetDevice(h->device));
+  hipStream_t st = static_cast<hipStream_t>(stream);
+  Carver c(workspace);
+  if (storage_dtype == QMX_C64) {
+    SweepLayout<float2> L = sweep_layout<float2>(c, num_qubits, batch);
+    return run_sweep<float2>(h, num_qubits, batch, num_layers, layers_used, num_sweeps,
+                             static_cast<const float2*>(target), static_cast<double2*>(gates),
+                             widths, static_cast<const double2*>(psi_ref),
+                             static_cast<double2*>(fidelity), overlap, goal, sweeps_done,
+                             static_cast<float2*>(circuit_out), L, st);
+  }
+  SweepLayout<double2> L = sweep_layout<double2>(c, num_qubits, batch);
+  return run_sweep<double2>(h, num_qubits, batch, num_layers, layers_used, num_sweeps,
+                            static_cast<const double2*>(target), static_cast<double2*>(gates),
+                            widths, static_cast<const double2*>(psi_ref),
+                            static_cast<double2*>(fidelity), overlap, goal, sweeps_done,
+                            static_cast<double2*>(circuit_out), L, st);
+}
+
+namespace {
+struct EncodeLayout {
+  ChiLayout chi;
+  LayerLayout layer;
+  double2* psi;
+  double2* target;
+  double2* psi_w;
+  void* target_s;
+  void* sweep_ws;
+  size_t sweep_bytes;
+};
+
+EncodeLayout encode_layout(Carver& c, const qmx_encode_config& g) {
+  EncodeLayout E{};
+  size_t len = state_len(g.num_qubits);
+  E.chi = chi_layout(c, g.num_qubits, g.batch, g.bond_dimension, false);
+  E.layer = layer_layout(c, g.num_qubits, g.batch);
+  E.psi = c.take<double2>(len * g.batch);
+  E.target = c.take<double2>(len * g.batch);
+  E.psi_w = c.take<double2>(len * g.batch);
+  E.target_s = (g.sweep_dtype == QMX_C64) ? static_cast<void*>(c.take<float2>(len * g.batch))
+                                          : static_cast<void*>(E.target);
+  E.sweep_bytes = qmx_sweep_workspace(g.num_qubits, g.batch, g.sweep_dtype);
+  E.sweep_ws = c.take<char>(E.sweep_bytes);
+  return E;
+}
+}  // namespace
+
+size_t qmx_encode_workspace(const qmx_encode_config* cfg) {
+  if (!cfg || cfg->num_qubits < 2 || cfg->num_qubits > kMaxQubits || cfg->batch < 1 ||
+      cfg->bond_dimension < 1)
+    return 0;
+  Carver c(nullptr);
+  encode_layout(c, *cfg);
+  return c.off + 256;
+}
+
+int qmx_encode(qmx_handle* h, const qmx_encode_config* cfg, const void* statevectors,
+               void* gates, int* widths, int* layers_used, void* fidelity, const double* goal,
+               int* sweeps_done, void* workspace, size_t workspace_bytes, void* stream) {
+  if (!h || !cfg) return QMX_ERR_INVALID;
+  const qmx_encode_config g = *cfg;
+  QMX_CALL(check_common(h, g.num_qubits, g.batch));
+  if (g.bond_dimension < 1) return fail(h, QMX_ERR_INVALID, "`bond_dimension` must be an integer greater than 0.");
+  if (g.num_layers < 1) return fail(h, QMX_ERR_INVALID, "The number of layers must be a positive integer.");
+  if (!chi_supported(g.num_qubits, g.bond_dimension))
+    return fail(h, QMX_ERR_INVALID, "bond_dimension above 256 with truncation is not supported");
+  if (g.num_sweeps < 0) return fail(h, QMX_ERR_INVALID, "num_sweeps must be >= 0");
+  if ((g.sv_dtype != QMX_C64 && g.sv_dtype != QMX_C128) ||
+      (g.sweep_dtype != QMX_C64 && g.sweep_dtype != QMX_C128))
+    return fail(h, QMX_ERR_INVALID, "dtype");
+  if (!statevectors || !gates || !widths || !layers_used || !fidelity || !workspace)
+    return fail(h, QMX_ERR_INVALID, "null pointer");
+  if (workspace_bytes < qmx_encode_workspace(cfg)) return fail(h, QMX_ERR_WORKSPACE, "workspace too small");
+  QMX_HIP(h, hipSetDevice(h->device));
+  hipStream_t st = static_cast<hipStream_t>(stream);
+  Carver c(workspace);
+  EncodeLayout E = encode_layout(c, g);
+  const int n = g.num_qubits, B = g.batch;
+  QMX_CALL(run_chi(h, n, B, g.bond_dimension, g.cutoff, statevectors, g.sv_dtype, E.psi, E.target,
+                   nullptr, E.chi, st));
+  QMX_CALL(device_error_status(h, E.chi.error, st));
+  // psi_w = target / |target|
+  int nb = static_cast<int>(std::min<size_t>(E.chi.nblk_norm, std::max<size_t>(1, state_len(n) / 1024)));
+  launch_fill_int(E.chi.ncnt, 0, B, st);
+  launch_norm2(static_cast<const double2*>(E.target), n, B, E.chi.npart, E.chi.ncnt, E.chi.nrm, nb, st);
+  launch_scale_convert(static_cast<const double2*>(E.target), E.psi_w, n, B, E.chi.nrm, st);
+  QMX_LAUNCHED(h, "normalise target");
+  QMX_CALL(run_extract(h, n, B, g.num_layers, g.fidelity_threshold, g.cutoff, E.psi_w,
+                       static_cast<double2*>(gates), widths, nullptr, layers_used, nullptr,
+                       E.layer, st));
+  if (g.sweep_dtype == QMX_C64)
+    launch_scale_convert(static_cast<const double2*>(E.target), static_cast<float2*>(E.target_s), n,
+                         B, nullptr, st);
+  return qmx_sweep(h, n, B, g.num_layers, layers_used, g.num_sweeps, E.target_s, g.sweep_dtype,
+                   gates, widths, E.psi, fidelity, nullptr, goal, sweeps_done, nullptr, E.sweep_ws,
+                   E.sweep_bytes, stream);
+}
+
+}  // extern "C"

@@ -1,0 +1,142 @@
+// Internal (host + device) declarations of the qmx library: kernel argument
+// blocks, limits and launchers.  Not part of the C-ABI.
+#pragma once
+#include <hip/hip_runtime.h>
+#include <stddef.h>
+#include <stdint.h>
+
+namespace qmx {
+
+constexpr int kMaxQubits = 30;
+constexpr int kMaxPassGates = 16;
+constexpr int kMaxHighBits = 8;
+constexpr int kApplyTileBits = 11;   // 2^11 complex128 = 32 KiB LDS
+constexpr int kSweepTileBits = 10;   // two tiles of 2^10 complex128 = 32 KiB
+constexpr int kApplyMaxUnion = 7;    // gate bits per pass (>= 4 coalescing bits)
+constexpr double kGsTol = 1e-7;      // canonical-gauge Gram-Schmidt threshold
+constexpr double kPhaseTol = 1e-12;  // Ran two-site phase fix skipped below this
+
+// error bits accumulated on the device
+constexpr int kErrNotUnitary = 1;
+constexpr int kErrBasis = 2;
+constexpr int kErrEigNoConv = 4;
+
+template <typename S>
+struct PassArgs {
+  int n, lmax, tbits;
+  int mode;            // 0: inverse layer `layer` (pos = sites); 1: forward positions
+  int layer;
+  int count;
+  int pos[kMaxPassGates];
+  S* psi;
+  const double2* gates;
+  const int* widths;
+  const int* layers;   // per-state layer count (mode 1); nullptr -> lmax
+  const int* skip;     // per-state skip flag, may be nullptr
+  const double2* ref;  // optional overlap reference (complex128)
+  double* dot_part;
+  int* dot_counter;
+  double2* dot_out;    // dot_out[b * dot_stride]
+  int dot_stride;
+};
+
+template <typename S>
+struct SweepArgs {
+  int n, lmax, tbits, step;
+  const S* target;
+  S* tprime;
+  S* phi;
+  double2* gates;
+  const int* widths;
+  const int* layers;
+  const int* skip;
+  double* part;
+  int* counter;
+  double* overlap;     // overlap[b * overlap_stride], may be nullptr
+  int overlap_stride;
+};
+
+struct Chain2Args {
+  int n, step;
+  double cutoff;
+  const double2* psi;  // X_N (normalised working state)
+  double2* xa;
+  double2* xb;
+  int* r;              // [B][n+1]
+  double2* w;          // [B][n+1][16]
+  double* spectra;     // [B][n+1][4] or nullptr
+  const int* skip;
+  double* part;
+  int* counter;
+  int* error;
+};
+
+struct RanArgs {
+  int n, lmax, layer;
+  const double2* psi;
+  const double2* xa;
+  const double2* xb;
+  const int* r;
+  const double2* w;
+  double2* gates;
+  int* widths;
+  int* bonds;          // optional [B][lmax][n]
+  const int* skip;
+  int* error;
+};
+
+struct Amp0Args {
+  int n, batch, lmax, layer;
+  double atol;
+  const double2* psi;
+  int* skip;
+  int* layers;
+  double2* amp0;       // optional [B][lmax]
+};
+
+// Buffer holding X_j (2^j x r_j) in the bond-2 chain (j <= N-1).
+inline __host__ __device__ int chain_buf_is_a(int n, int j) { return ((n - j) & 1) == 1; }
+
+// launchers (qmx_kernels.hip)
+template <typename S> void launch_apply_pass(const PassArgs<S>&, int nblk, int batch, hipStream_t);
+template <typename S> void launch_sweep_step(const SweepArgs<S>&, int nblk, int batch, hipStream_t);
+template <typename S>
+void launch_norm2(const S* x, int n, int batch, double* part, int* counter, double* out, int nblk,
+                  hipStream_t);
+template <typename S, typename D>
+void launch_scale_convert(const S* x, D* y, int n, int batch, const double* norm2, hipStream_t);
+void launch_fill_int(int* p, int v, int count, hipStream_t);
+template <typename S> void launch_basis0(S* psi, int n, int batch, const int* skip, hipStream_t);
+void launch_chain2_step(const Chain2Args&, int nblk, int batch, hipStream_t);
+void launch_ran_gates(const RanArgs&, int batch, hipStream_t);
+void launch_amp0(const Amp0Args&, hipStream_t);
+
+// chi-truncation chain (qmx_linalg.hip)
+void launch_gram(const double2* y, size_t y_stride, const int* ncols_dev, int rows_log2,
+                 int ncols_max, double2* part, int nchunks, int batch, hipStream_t);
+void launch_gram_reduce(const double2* part, int nchunks, const int* ncols_dev, int ncols_max,
+                        double2* a, size_t a_stride, int batch, hipStream_t);
+void launch_jacobi(double2* a, double2* v, size_t stride, const int* ncols_dev, int ncols_max,
+                   int batch, int* error, hipStream_t);
+void launch_select(const double2* a, const double2* v, double2* w, size_t stride,
+                   const int* ncols_dev, int ncols_max, int chi, double cutoff, int* k_out,
+                   int kstride, double* spectra, size_t spec_stride, int batch, hipStream_t);
+void launch_chain_dims(int* nn, int* ncur, const int* kk, int n, int step, int first_cols,
+                       int batch, hipStream_t);
+void launch_goal_check(const double2* dot, int dot_stride, const double* goal, int* skip,
+                       int* active, int* sweeps_done, int batch, hipStream_t);
+// C[b] (m x ncols) = A[b] (m x kdim) * op(B[b]); conj_b: op(B)[j][c] = conj(B[c][j])
+// with leading dimension ldb; dims read from device arrays (per state) when given.
+struct GemmDims {
+  int rows_log2;          // m = 2^rows_log2
+  const int* kdim;        // inner dimension per state (kdim[b * dim_stride]) or nullptr
+  const int* ncols;       // output columns per state or nullptr
+  int dim_stride;
+  int kmax, nmax;         // host upper bounds (used when the pointers are null)
+  int ldb;                // leading dimension of B
+};
+void launch_gemm(const double2* a, size_t a_stride, const double2* bm, size_t b_stride,
+                 double2* c, size_t c_stride, const GemmDims& d, int conj_b, int batch,
+                 hipStream_t);
+
+}  // namespace qmx

@@ -1,0 +1,774 @@
+// Dense-state kernels of the MI355X Sequential encoder.
+//
+// Layout.  A batch of B states of N qubits is stored state-major: state b
+// occupies elements [b * 2^N, (b + 1) * 2^N).  Amplitude index x has site 0 as
+// its most significant bit (site s <-> bit N-1-s).  Gates are 4x4 complex128
+// (out x in, row-major, a 1-qubit gate in the top-left 2x2) stored at
+// gates[((b * lmax + layer) * N + site) * 16], their width in widths[...].
+//
+// Tiles.  Every kernel that applies gates stages a tile of 2^t amplitudes in
+// LDS.  The tile bit set P is the union U of the gate bits plus the lowest
+// bits outside U, so a tile is a few long contiguous runs in HBM (coalesced
+// loads) and every gate of the pass acts inside it.  Tiles partition the state.
+#include "qmx_device.h"
+#include "qmx_internal.h"
+
+namespace qmx {
+
+// ------------------------------------------------------------ tile geometry --
+struct Geom {
+  int t;             // tile bits
+  int c0;            // contiguous low bits 0..c0-1 of P
+  int nh;            // high tile bits (P above c0)
+  uint64_t qmask;    // global bits outside the tile
+  int hoffs[1 << kMaxHighBits];
+};
+
+QMX_DEV uint64_t deposit(uint64_t x, uint64_t mask) {
+  uint64_t r = 0;
+  while (mask) {
+    uint64_t low = mask & (~mask + 1ull);
+    if (x & 1ull) r |= low;
+    x >>= 1;
+    mask &= mask - 1ull;
+  }
+  return r;
+}
+
+// Build the tile geometry for the union of gate bits `umask` (thread 0 only).
+QMX_DEV void build_geom(Geom& g, int n, int tbits, uint64_t umask) {
+  int t = tbits < n ? tbits : n;
+  uint64_t full = (n >= 64) ? ~0ull : ((1ull << n) - 1ull);
+  uint64_t p = umask & full;
+  for (int bit = 0; bit < n && __popcll(p) < t; ++bit) p |= (1ull << bit);
+  g.t = t;
+  int c0 = 0;
+  while (c0 < n && ((p >> c0) & 1ull)) ++c0;
+  g.c0 = c0;
+  uint64_t high = p & ~((c0 >= 64) ? ~0ull : ((1ull << c0) - 1ull));
+  int nh = __popcll(high);
+  if (nh > kMaxHighBits) nh = kMaxHighBits;   // callers keep gate unions below this
+  g.nh = nh;
+  g.qmask = full & ~p;
+  for (int h = 0; h < (1 << nh); ++h) g.hoffs[h] = (int)deposit((uint64_t)h, high);
+  // local positions are computed with local_bit()
+}
+
+// Position of global bit `gb` inside the tile index (P sorted ascending).
+QMX_DEV int local_bit(const Geom& g, int gb) {
+  if (gb < g.c0) return gb;
+  // high bits: rank among P's bits below gb
+  uint64_t pm = (~g.qmask) & ((1ull << gb) - 1ull);
+  return __popcll(pm);
+}
+
+QMX_DEV size_t tile_addr(const Geom& g, uint64_t base, int l) {
+  return (size_t)(base + (uint64_t)(l & ((1 << g.c0) - 1)) + (uint64_t)g.hoffs[l >> g.c0]);
+}
+
+// Apply a gate (or its adjoint) to LDS tile s in place; gate on local bits
+// m1 (site s) and m0 (site s+1) for width 2, m1 for width 1.
+template <bool ADJ>
+QMX_DEV void tile_gate(double2* s, int t, int width, int m1, int m0, const double2* gm) {
+  if (width == 2) {
+    const int ng = 1 << (t - 2);
+    const int s0 = 1 << m0, s1 = 1 << m1;
+    for (int q = threadIdx.x; q < ng; q += blockDim.x) {
+      int l0 = (int)insert_zero(insert_zero((uint64_t)q, m0), m1);
+      double2 x0 = s[l0], x1 = s[l0 + s0], x2 = s[l0 + s1], x3 = s[l0 + s0 + s1];
+      double2 y[4];
+#pragma unroll
+      for (int o = 0; o < 4; ++o) {
+        double2 acc = cz();
+        if (ADJ) {
+          acc = cfmac(gm[0 * 4 + o], x0, acc);
+          acc = cfmac(gm[1 * 4 + o], x1, acc);
+          acc = cfmac(gm[2 * 4 + o], x2, acc);
+          acc = cfmac(gm[3 * 4 + o], x3, acc);
+        } else {
+          acc = cfma(gm[o * 4 + 0], x0, acc);
+          acc = cfma(gm[o * 4 + 1], x1, acc);
+          acc = cfma(gm[o * 4 + 2], x2, acc);
+          acc = cfma(gm[o * 4 + 3], x3, acc);
+        }
+        y[o] = acc;
+      }
+      s[l0] = y[0]; s[l0 + s0] = y[1]; s[l0 + s1] = y[2]; s[l0 + s0 + s1] = y[3];
+    }
+  } else {
+    const int ng = 1 << (t - 1);
+    const int s1 = 1 << m1;
+    for (int q = threadIdx.x; q < ng; q += blockDim.x) {
+      int l0 = (int)insert_zero((uint64_t)q, m1);
+      double2 x0 = s[l0], x1 = s[l0 + s1];
+      double2 y0, y1;
+      if (ADJ) {
+        y0 = cfmac(gm[1 * 4 + 0], x1, cmulc(gm[0 * 4 + 0], x0));
+        y1 = cfmac(gm[1 * 4 + 1], x1, cmulc(gm[0 * 4 + 1], x0));
+      } else {
+        y0 = cfma(gm[0 * 4 + 1], x1, cmul(gm[0 * 4 + 0], x0));
+        y1 = cfma(gm[1 * 4 + 1], x1, cmul(gm[1 * 4 + 0], x0));
+      }
+      s[l0] = y0; s[l0 + s1] = y1;
+    }
+  }
+}
+
+// A two-qubit gate needs a right neighbour; anything else acts as one qubit.
+QMX_DEV int sane_width(int n, int site, int width) { return (width == 2 && site < n - 1) ? 2 : 1; }
+
+QMX_DEV uint64_t gate_bits(int n, int site, int width) {
+  uint64_t m = 1ull << (n - 1 - site);
+  if (width == 2) m |= 1ull << (n - 2 - site);
+  return m;
+}
+
+// Block-wide sum of NV doubles held per thread; result valid in red[0..NV-1]
+// for all threads after return.  red must hold NV * (blockDim/64) doubles.
+template <int NV>
+QMX_DEV void block_sum(double (&v)[NV], double* red) {
+  const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6, nw = blockDim.x >> 6;
+#pragma unroll
+  for (int j = 0; j < NV; ++j) {
+    double s = wave_sum(v[j]);
+    if (lane == 0) red[wid * NV + j] = s;
+  }
+  __syncthreads();
+  if (threadIdx.x < NV) {
+    double s = 0.0;
+    for (int w = 0; w < nw; ++w) s += red[w * NV + threadIdx.x];
+    red[threadIdx.x] = s;   // row 0 is only overwritten by its own column owner
+  }
+  __syncthreads();
+}
+
+// Publish this block's NV partial values and detect the last block of state b.
+// Returns true (uniformly) in the block that arrives last; that block then
+// sees every block's partials.
+template <int NV>
+QMX_DEV bool publish_partial(const double* red, double* part, int* counter, int nblk,
+                             int* s_flag) {
+  if (threadIdx.x < NV) part[(size_t)blockIdx.x * NV + threadIdx.x] = red[threadIdx.x];
+  __threadfence();
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    int prev = atomicAdd(counter, 1);
+    *s_flag = (prev == nblk - 1) ? 1 : 0;
+  }
+  __syncthreads();
+  bool last = *s_flag != 0;
+  if (last) __threadfence();
+  return last;
+}
+
+// ---------------------------------------------------------------- fill ------
+__global__ void fill_int_kernel(int* p, int v, int count) {
+  int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < count) p[i] = v;
+}
+
+// |0..0> in every (non-skipped) state
+template <typename S>
+__global__ __launch_bounds__(256) void basis0_kernel(S* psi, int n, const int* skip) {
+  const int b = blockIdx.y;
+  if (skip && skip[b]) return;
+  const size_t len = (size_t)1 << n;
+  S* pb = psi + (size_t)b * len;
+  for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < len;
+       i += (size_t)gridDim.x * blockDim.x)
+    pb[i] = from_c<S>(make_double2(i == 0 ? 1.0 : 0.0, 0.0));
+}
+
+// ----------------------------------------------------- norm / conversion ----
+// Sum of |x|^2 per state (last block writes out[b]) -- one pass.
+template <typename S>
+__global__ __launch_bounds__(256) void norm2_kernel(const S* __restrict__ x, int n,
+                                                    double* part, int* counter,
+                                                    double* out) {
+  __shared__ double red[4 * 1];
+  __shared__ int s_flag;
+  const int b = blockIdx.y;
+  const size_t len = (size_t)1 << n;
+  const S* xb = x + (size_t)b * len;
+  double v[1] = {0.0};
+  for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < len;
+       i += (size_t)gridDim.x * blockDim.x)
+    v[0] += cabs2(to_c(xb[i]));
+  block_sum<1>(v, red);
+  double* pb = part + (size_t)b * gridDim.x;
+  if (publish_partial<1>(red, pb, counter + b, gridDim.x, &s_flag)) {
+    if (threadIdx.x == 0) {
+      double s = 0.0;
+      for (int k = 0; k < (int)gridDim.x; ++k) s += pb[k];
+      out[b] = s;
+      counter[b] = 0;
+    }
+  }
+}
+
+// y[b] = x[b] * scale(b), scale = 1/sqrt(norm2[b]) (or 1 when norm2 == nullptr)
+template <typename S, typename D>
+__global__ __launch_bounds__(256) void scale_convert_kernel(const S* __restrict__ x, D* y, int n,
+                                                            const double* norm2) {
+  const int b = blockIdx.y;
+  const size_t len = (size_t)1 << n;
+  double sc = norm2 ? 1.0 / sqrt(norm2[b]) : 1.0;
+  const S* xb = x + (size_t)b * len;
+  D* yb = y + (size_t)b * len;
+  for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < len;
+       i += (size_t)gridDim.x * blockDim.x)
+    yb[i] = from_c<D>(cscale(to_c(xb[i]), sc));
+}
+
+// ------------------------------------------------------ gate-sequence apply --
+// Applies up to kMaxPassGates gates to every state of the batch, staged
+// through one LDS tile.  mode 0: inverse layer `layer`, pos[] are sites, each
+// gate applied as G^dagger; mode 1: forward circuit positions f (pos[]), gate
+// (layer L_b-1-f/N, site f%N) applied as G.  When ref != nullptr the overlap
+// sum conj(ref) * psi_new is accumulated and written to dot_out[b].
+template <typename S>
+__global__ __launch_bounds__(256) void apply_pass_kernel(PassArgs<S> a) {
+  extern __shared__ double2 smem[];
+  __shared__ Geom g;
+  __shared__ double2 gm[kMaxPassGates][16];
+  __shared__ int gw[kMaxPassGates], gm1[kMaxPassGates], gm0[kMaxPassGates], gcount;
+  __shared__ double red[4 * 2];
+  __shared__ int s_flag;
+  const int b = blockIdx.y;
+  const int n = a.n;
+  if (a.skip && a.skip[b]) return;
+  const int lb = a.layers ? a.layers[b] : a.lmax;
+  if (threadIdx.x == 0) {
+    uint64_t um = 0;
+    int c = 0;
+    for (int j = 0; j < a.count; ++j) {
+      int layer, site;
+      if (a.mode == 0) {
+        layer = a.layer;
+        site = a.pos[j];
+        if (layer >= lb) continue;
+      } else {
+        int f = a.pos[j];
+        if (f >= lb * n) continue;
+        layer = lb - 1 - f / n;
+        site = f % n;
+      }
+      size_t gi = ((size_t)b * a.lmax + layer) * n + site;
+      int w = sane_width(n, site, a.widths[gi]);
+      gw[c] = w;
+      gm1[c] = site;     // temporarily the site
+      for (int e = 0; e < 16; ++e) gm[c][e] = a.gates[gi * 16 + e];
+      um |= gate_bits(n, site, w);
+      ++c;
+    }
+    gcount = c;
+    build_geom(g, n, a.tbits, um);
+    for (int j = 0; j < c; ++j) {
+      int site = gm1[j];
+      gm1[j] = local_bit(g, n - 1 - site);
+      gm0[j] = gw[j] == 2 ? local_bit(g, n - 2 - site) : 0;
+    }
+  }
+  __syncthreads();
+  if (gcount == 0 && a.ref == nullptr) return;
+  const int t = g.t;
+  const int tl = 1 << t;
+  const size_t len = (size_t)1 << n;
+  S* psi = a.psi + (size_t)b * len;
+  const double2* ref = a.ref ? a.ref + (size_t)b * len : nullptr;
+  const long ntiles = 1l << (n - t);
+  double acc[2] = {0.0, 0.0};
+  for (long tile = blockIdx.x; tile < ntiles; tile += gridDim.x) {
+    uint64_t base = deposit((uint64_t)tile, g.qmask);
+    for (int l = threadIdx.x; l < tl; l += blockDim.x) smem[l] = to_c(psi[tile_addr(g, base, l)]);
+    __syncthreads();
+    for (int j = 0; j < gcount; ++j) {
+      if (a.mode == 0) tile_gate<true>(smem, t, gw[j], gm1[j], gm0[j], gm[j]);
+      else tile_gate<false>(smem, t, gw[j], gm1[j], gm0[j], gm[j]);
+      __syncthreads();
+    }
+    for (int l = threadIdx.x; l < tl; l += blockDim.x) {
+      size_t ad = tile_addr(g, base, l);
+      double2 v = smem[l];
+      if (gcount > 0) psi[ad] = from_c<S>(v);
+      if (ref) {
+        double2 r = ref[ad];
+        v = to_c(from_c<S>(v));
+        acc[0] = fma(r.x, v.x, fma(r.y, v.y, acc[0]));
+        acc[1] = fma(r.x, v.y, fma(-r.y, v.x, acc[1]));
+      }
+    }
+    __syncthreads();
+  }
+  if (ref) {
+    block_sum<2>(acc, red);
+    double* pb = a.dot_part + (size_t)b * gridDim.x * 2;
+    if (publish_partial<2>(red, pb, a.dot_counter + b, gridDim.x, &s_flag)) {
+      if (threadIdx.x == 0) {
+        double sr = 0.0, si = 0.0;
+        for (int k = 0; k < (int)gridDim.x; ++k) { sr += pb[2 * k]; si += pb[2 * k + 1]; }
+        a.dot_out[(size_t)b * a.dot_stride] = make_double2(sr, si);
+        a.dot_counter[b] = 0;
+      }
+    }
+  }
+}
+
+// ------------------------------------------------------------ sweep step ----
+// One gate step of the environment sweep (sequential.py:452-505) for every
+// state of the batch, in one pass over the tile of (T', phi):
+//   T'  <- G'_{k-1}^dagger T'     (pull the target through the updated gate)
+//   phi <- G_k^dagger phi          (remove the old gate)
+//   E_k[o, i] += conj(T'[o]) phi[i]  (FP64 partial sums)
+// The last block of each state reduces the partials in a fixed order, takes
+// the polar factor of E_k (one-sided Jacobi, FP64) and stores
+// G'_k = conj(U V^H) in place of G_k; overlap[b] receives tr(S) = <T|C'|0>.
+template <typename S>
+__global__ __launch_bounds__(256) void sweep_step_kernel(SweepArgs<S> a) {
+  extern __shared__ double2 smem[];
+  __shared__ Geom g;
+  __shared__ double2 gc[16], gp[16];
+  __shared__ int s_info[8];
+  __shared__ double red[4 * 32];
+  __shared__ double tot[32];
+  __shared__ int s_flag;
+  const int b = blockIdx.y;
+  const int n = a.n;
+  if (a.skip && a.skip[b]) return;
+  const int kb = a.layers[b] * n;
+  const int k = a.step;
+  if (k >= kb) return;
+  const int lc = k / n, ic = n - 1 - k % n;
+  const size_t gic = ((size_t)b * a.lmax + lc) * n + ic;
+  if (threadIdx.x == 0) {
+    int wc = sane_width(n, ic, a.widths[gic]);
+    uint64_t um = gate_bits(n, ic, wc);
+    int wp = 0, ip = 0;
+    if (k > 0) {
+      int lp = (k - 1) / n;
+      ip = n - 1 - (k - 1) % n;
+      size_t gip = ((size_t)b * a.lmax + lp) * n + ip;
+      wp = sane_width(n, ip, a.widths[gip]);
+      um |= gate_bits(n, ip, wp);
+      for (int e = 0; e < 16; ++e) gp[e] = a.gates[gip * 16 + e];
+    }
+    for (int e = 0; e < 16; ++e) gc[e] = a.gates[gic * 16 + e];
+    build_geom(g, n, a.tbits, um);
+    s_info[0] = wc;
+    s_info[1] = local_bit(g, n - 1 - ic);
+    s_info[2] = wc == 2 ? local_bit(g, n - 2 - ic) : 0;
+    s_info[3] = wp;
+    s_info[4] = wp ? local_bit(g, n - 1 - ip) : 0;
+    s_info[5] = wp == 2 ? local_bit(g, n - 2 - ip) : 0;
+  }
+  __syncthreads();
+  const int wc = s_info[0], mc1 = s_info[1], mc0 = s_info[2];
+  const int wp = s_info[3], mp1 = s_info[4], mp0 = s_info[5];
+  const int t = g.t, tl = 1 << t;
+  const size_t len = (size_t)1 << n;
+  double2* sT = smem;
+  double2* sP = smem + tl;
+  const S* tsrc = (k <= 1 ? a.target : a.tprime) + (size_t)b * len;
+  S* tdst = a.tprime + (size_t)b * len;
+  S* phi = a.phi + (size_t)b * len;
+  const long ntiles = 1l << (n - t);
+  double e[32];
+#pragma unroll
+  for (int j = 0; j < 32; ++j) e[j] = 0.0;
+  for (long tile = blockIdx.x; tile < ntiles; tile += gridDim.x) {
+    uint64_t base = deposit((uint64_t)tile, g.qmask);
+    for (int l = threadIdx.x; l < tl; l += blockDim.x) {
+      size_t ad = tile_addr(g, base, l);
+      sT[l] = to_c(tsrc[ad]);
+      sP[l] = to_c(phi[ad]);
+    }
+    __syncthreads();
+    if (wp) {
+      tile_gate<true>(sT, t, wp, mp1, mp0, gp);
+      __syncthreads();
+    }
+    if (wc == 2) {
+      const int ng = 1 << (t - 2);
+      const int s0 = 1 << mc0, s1 = 1 << mc1;
+      for (int q = threadIdx.x; q < ng; q += blockDim.x) {
+        int l0 = (int)insert_zero(insert_zero((uint64_t)q, mc0), mc1);
+        int idx[4] = {l0, l0 + s0, l0 + s1, l0 + s0 + s1};
+        double2 x[4], y[4], tt[4];
+#pragma unroll
+        for (int j = 0; j < 4; ++j) { x[j] = sP[idx[j]]; tt[j] = sT[idx[j]]; }
+#pragma unroll
+        for (int o = 0; o < 4; ++o) {
+          double2 acc = cz();
+#pragma unroll
+          for (int i = 0; i < 4; ++i) acc = cfmac(gc[i * 4 + o], x[i], acc);
+          y[o] = to_c(from_c<S>(acc));   // the value phi will hold in HBM
+        }
+#pragma unroll
+        for (int j = 0; j < 4; ++j) sP[idx[j]] = y[j];
+#pragma unroll
+        for (int o = 0; o < 4; ++o)
+#pragma unroll
+          for (int i = 0; i < 4; ++i) {
+            double2 c = cmulc(tt[o], y[i]);
+            e[2 * (o * 4 + i)] += c.x;
+            e[2 * (o * 4 + i) + 1] += c.y;
+          }
+      }
+    } else {
+      const int ng = 1 << (t - 1);
+      const int s1 = 1 << mc1;
+      for (int q = threadIdx.x; q < ng; q += blockDim.x) {
+        int l0 = (int)insert_zero((uint64_t)q, mc1);
+        double2 x0 = sP[l0], x1 = sP[l0 + s1];
+        double2 t0 = sT[l0], t1 = sT[l0 + s1];
+        double2 y0 = to_c(from_c<S>(cfmac(gc[1 * 4 + 0], x1, cmulc(gc[0 * 4 + 0], x0))));
+        double2 y1 = to_c(from_c<S>(cfmac(gc[1 * 4 + 1], x1, cmulc(gc[0 * 4 + 1], x0))));
+        sP[l0] = y0; sP[l0 + s1] = y1;
+        double2 c;
+        c = cmulc(t0, y0); e[0] += c.x; e[1] += c.y;
+        c = cmulc(t0, y1); e[2] += c.x; e[3] += c.y;
+        c = cmulc(t1, y0); e[8] += c.x; e[9] += c.y;
+        c = cmulc(t1, y1); e[10] += c.x; e[11] += c.y;
+      }
+    }
+    __syncthreads();
+    for (int l = threadIdx.x; l < tl; l += blockDim.x) {
+      size_t ad = tile_addr(g, base, l);
+      if (k >= 1) tdst[ad] = from_c<S>(sT[l]);
+      phi[ad] = from_c<S>(sP[l]);
+    }
+    __syncthreads();
+  }
+  // E entries for width 1 were packed as (o*4+i) with o, i < 2: indices 0,1,4,5
+  block_sum<32>(e, red);
+  double* pb = a.part + (size_t)b * gridDim.x * 32;
+  if (publish_partial<32>(red, pb, a.counter + b, gridDim.x, &s_flag)) {
+    if (threadIdx.x < 32) {
+      double s = 0.0;
+      for (int q = 0; q < (int)gridDim.x; ++q) s += pb[(size_t)q * 32 + threadIdx.x];
+      tot[threadIdx.x] = s;
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+      double2 em[16], u[16];
+      for (int j = 0; j < 16; ++j) em[j] = make_double2(tot[2 * j], tot[2 * j + 1]);
+      int d = wc == 2 ? 4 : 2;
+      double nuc = 0.0;
+      polar4(d, em, u, &nuc);
+      for (int j = 0; j < 16; ++j) {
+        int o = j / 4, i = j % 4;
+        a.gates[gic * 16 + j] = (o < d && i < d) ? cconj(u[j]) : cz();
+      }
+      if (a.overlap) a.overlap[(size_t)b * a.overlap_stride] = nuc;
+      a.counter[b] = 0;
+    }
+  }
+}
+
+// ------------------------------------------------------- bond-2 chain step --
+// Step i of the right-to-left truncated projection chain with max bond 2
+// (compress(form="right", max_bond=2), mps.py:876-887), dense form:
+//   X_{i+1} = Y_{i+1} W_{i+1}   (Y_{i+1} = X_{i+2} viewed as 2^{i+1} x 2 r_{i+2})
+//   rho_i   = Y_i^H Y_i         (Y_i = X_{i+1} viewed as 2^i x 2 r_{i+1})
+// The last block of each state finds the kept subspace of rho_i (FP64 Jacobi,
+// rsum2 cutoff, max bond 2) and stores its canonical basis W_i and r_i.
+__global__ __launch_bounds__(256) void chain2_step_kernel(Chain2Args a) {
+  __shared__ double red[4 * 32];
+  __shared__ double tot[32];
+  __shared__ int s_flag;
+  const int b = blockIdx.y;
+  if (a.skip && a.skip[b]) return;
+  const int n = a.n, i = a.step;
+  const size_t len = (size_t)1 << n;
+  int* rb = a.r + (size_t)b * (n + 1);          // r[b][j] = bond dim left of site j
+  double2* wb = a.w + (size_t)b * (n + 1) * 16; // W_j (n_j x k_j, stride 4)
+  const double2* xin;
+  double2* xout = nullptr;
+  int rin, rcur;
+  const double2* wproj = nullptr;
+  if (i == n - 1) {
+    xin = a.psi + (size_t)b * len;   // X_N: 2^N x 1
+    rin = 1;
+    rcur = 1;
+  } else {
+    xin = (i + 2 == n ? a.psi : (chain_buf_is_a(n, i + 2) ? a.xa : a.xb)) + (size_t)b * len;  // X_{i+2}
+    xout = (chain_buf_is_a(n, i + 1) ? a.xa : a.xb) + (size_t)b * len;                       // X_{i+1}
+    rin = (i + 2 == n) ? 1 : min(max(rb[i + 2], 1), 2);
+    rcur = min(max(rb[i + 1], 1), 2);
+    wproj = wb + (size_t)(i + 1) * 16;
+  }
+  const int ncols = 2 * rcur;                   // columns of Y_i
+  const size_t rows = (size_t)1 << i;
+  double e[32];
+#pragma unroll
+  for (int j = 0; j < 32; ++j) e[j] = 0.0;
+  for (size_t row = (size_t)blockIdx.x * blockDim.x + threadIdx.x; row < rows;
+       row += (size_t)gridDim.x * blockDim.x) {
+    double2 y[4];
+    if (i == n - 1) {
+      y[0] = xin[2 * row];
+      y[1] = xin[2 * row + 1];
+      y[2] = cz(); y[3] = cz();
+    } else {
+      // rows 2row, 2row+1 of X_{i+1}; each from 2 rows of X_{i+1}'s parent
+      const int nin = 2 * rin;   // columns of Y_{i+1}
+      for (int h = 0; h < 2; ++h) {
+        size_t xr = 2 * row + h;            // row of X_{i+1}
+        double2 yin[4];
+        for (int c = 0; c < nin; ++c) yin[c] = xin[xr * nin + c];  // Y_{i+1} row xr
+        for (int c = 0; c < rcur; ++c) {
+          double2 s = cz();
+          for (int j = 0; j < nin; ++j) s = cfma(yin[j], wproj[j * 4 + c], s);
+          y[h * rcur + c] = s;
+          xout[xr * rcur + c] = s;
+        }
+      }
+      for (int c = ncols; c < 4; ++c) y[c] = cz();
+    }
+#pragma unroll
+    for (int p = 0; p < 4; ++p)
+#pragma unroll
+      for (int q = 0; q < 4; ++q) {
+        double2 c = cmulc(y[p], y[q]);
+        e[2 * (p * 4 + q)] += c.x;
+        e[2 * (p * 4 + q) + 1] += c.y;
+      }
+  }
+  block_sum<32>(e, red);
+  double* pb = a.part + (size_t)b * gridDim.x * 32;
+  if (publish_partial<32>(red, pb, a.counter + b, gridDim.x, &s_flag)) {
+    if (threadIdx.x < 32) {
+      double s = 0.0;
+      for (int q = 0; q < (int)gridDim.x; ++q) s += pb[(size_t)q * 32 + threadIdx.x];
+      tot[threadIdx.x] = s;
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+      double2 rho[16], v[16], w[16];
+      double lam[4];
+      for (int j = 0; j < 16; ++j) rho[j] = make_double2(tot[2 * j], tot[2 * j + 1]);
+      for (int p = 0; p < 4; ++p)   // Hermitian part
+        for (int q = p; q < 4; ++q) {
+          double2 x = rho[p * 4 + q], y = rho[q * 4 + p];
+          double2 h = make_double2(0.5 * (x.x + y.x), 0.5 * (x.y - y.y));
+          rho[p * 4 + q] = h;
+          rho[q * 4 + p] = cconj(h);
+        }
+      herm_eig4(ncols, rho, v, lam);
+      int k = keep_count(lam, ncols, 2, a.cutoff);
+      for (int j = 0; j < 16; ++j) w[j] = cz();
+      int found = canonical_basis4(ncols, k, v, w, kGsTol);
+      if (found < k) {
+        atomicOr(a.error, kErrBasis);
+        k = found > 0 ? found : 1;
+      }
+      for (int j = 0; j < 16; ++j) wb[(size_t)i * 16 + j] = w[j];
+      rb[i] = k;
+      if (a.spectra) {
+        for (int j = 0; j < 4; ++j)
+          a.spectra[((size_t)b * (n + 1) + i) * 4 + j] = j < ncols ? lam[j] : 0.0;
+      }
+      a.counter[b] = 0;
+    }
+  }
+}
+
+// ---------------------------------------------------------- Ran gates -------
+// generate_unitary_layer (mps.py:746-847) on the bond-2 MPS held as
+// (W_1 .. W_{N-1}, X_2): one lane per site builds the site tensor, the gate
+// (first / two / last site unitary, Ran eqs. 9, 7, 6) and checks unitarity.
+__global__ __launch_bounds__(64) void ran_gates_kernel(RanArgs a) {
+  const int b = blockIdx.x;
+  const int n = a.n;
+  if (a.skip && a.skip[b]) return;
+  const int site = threadIdx.x;
+  const int* rb = a.r + (size_t)b * (n + 1);
+  const double2* wb = a.w + (size_t)b * (n + 1) * 16;
+  __shared__ double2 a0[4];
+  __shared__ double a0n;
+  if (threadIdx.x == 0) {
+    // X_1 = Y_1 W_1 with Y_1 = X_2 viewed as 2 x 2 r_2
+    const int r2 = (n == 2) ? 1 : min(max(rb[2], 1), 2);
+    const int r1 = min(max(rb[1], 1), 2);
+    const double2* x2 = ((n == 2) ? a.psi : (chain_buf_is_a(n, 2) ? a.xa : a.xb)) +
+                        (size_t)b * ((size_t)1 << n);
+    const double2* w1 = wb + 16;
+    double nn = 0.0;
+    for (int p = 0; p < 2; ++p)
+      for (int c = 0; c < 2; ++c) {
+        double2 s = cz();
+        if (c < r1)
+          for (int j = 0; j < 2 * r2; ++j) s = cfma(x2[p * 2 * r2 + j], w1[j * 4 + c], s);
+        a0[p * 2 + c] = s;
+        nn += cabs2(s);
+      }
+    a0n = sqrt(nn);
+  }
+  __syncthreads();
+  if (site >= n) return;
+  const int rl = site == 0 ? 1 : min(max(rb[site], 1), 2);          // left bond
+  const int rr = site == n - 1 ? 1 : min(max(rb[site + 1], 1), 2);  // right bond
+  const bool is_end = (site == n - 1) || rr < 2;
+  const bool is_start = (site == 0) || rl < 2;
+  // site tensor T[l][p][c] (l < rl, p < 2, c < rr), flattened as l*2*rr + p*rr + c
+  double2 tens[8];
+  for (int j = 0; j < 8; ++j) tens[j] = cz();
+  if (site == 0) {
+    for (int p = 0; p < 2; ++p)
+      for (int c = 0; c < rr; ++c) tens[p * rr + c] = cscale(a0[p * 2 + c], 1.0 / a0n);
+  } else {
+    const double2* ws = wb + (size_t)site * 16;   // (2 rr) x rl, stride 4
+    for (int l = 0; l < rl; ++l)
+      for (int p = 0; p < 2; ++p)
+        for (int c = 0; c < rr; ++c) tens[l * 2 * rr + p * rr + c] = cconj(ws[(p * rr + c) * 4 + l]);
+  }
+  double2 m[16];
+  for (int j = 0; j < 16; ++j) m[j] = cz();
+  int width;
+  if (is_end && is_start) {
+    // single site: rows [a; null(conj a)] transposed
+    double2 rows[16], q[16];
+    for (int j = 0; j < 16; ++j) rows[j] = cz();
+    rows[0] = tens[0]; rows[1] = tens[1];
+    householder_q(1, 2, rows, q);
+    m[0 * 4 + 0] = tens[0]; m[1 * 4 + 0] = tens[1];
+    m[0 * 4 + 1] = q[0 * 4 + 1]; m[1 * 4 + 1] = q[1 * 4 + 1];
+    width = 1;
+  } else if (is_end) {
+    // M[p][l] = A[l][p]
+    for (int p = 0; p < 2; ++p)
+      for (int l = 0; l < 2; ++l) m[p * 4 + l] = tens[l * 2 + p];
+    width = 1;
+  } else if (is_start) {
+    double2 rows[16], q[16];
+    for (int j = 0; j < 16; ++j) rows[j] = cz();
+    for (int j = 0; j < 4; ++j) rows[j] = tens[j];
+    householder_q(1, 4, rows, q);
+    // M = [a | n1 | n0 | n2], n_k = q[:, 1 + k]
+    for (int r = 0; r < 4; ++r) {
+      m[r * 4 + 0] = tens[r];
+      m[r * 4 + 1] = q[r * 4 + 2];
+      m[r * 4 + 2] = q[r * 4 + 1];
+      m[r * 4 + 3] = q[r * 4 + 3];
+    }
+    width = 2;
+  } else {
+    double2 rows[16], q[16];
+    for (int j = 0; j < 16; ++j) rows[j] = cz();
+    for (int l = 0; l < 2; ++l)
+      for (int j = 0; j < 4; ++j) rows[l * 4 + j] = tens[l * 4 + j];
+    householder_q(2, 4, rows, q);
+    double2 nb[2][4];
+    for (int c = 0; c < 2; ++c) {
+      double2 f = q[0 * 4 + 2 + c];
+      double af = cabs_(f);
+      // the phase of a rounding-level entry is meaningless: leave the column
+      double2 ph = af > kPhaseTol ? make_double2(f.x / af, f.y / af) : make_double2(1.0, 0.0);
+      for (int r = 0; r < 4; ++r) nb[c][r] = cmulc(ph, q[r * 4 + 2 + c]);  // / e^{i arg}
+    }
+    for (int r = 0; r < 4; ++r) {
+      m[r * 4 + 0] = tens[0 * 4 + r];
+      m[r * 4 + 1] = nb[0][r];
+      m[r * 4 + 2] = tens[1 * 4 + r];
+      m[r * 4 + 3] = nb[1][r];
+    }
+    width = 2;
+  }
+  // unitarity: allclose(M M^H, I, rtol=1e-5, atol=1e-8)
+  const int d = width == 2 ? 4 : 2;
+  bool ok = true;
+  for (int i = 0; i < d; ++i)
+    for (int j = 0; j < d; ++j) {
+      double2 s = cz();
+      for (int kk = 0; kk < d; ++kk) s = cfma(m[i * 4 + kk], cconj(m[j * 4 + kk]), s);
+      double ref = (i == j) ? 1.0 : 0.0;
+      double diff = sqrt((s.x - ref) * (s.x - ref) + s.y * s.y);
+      if (!(diff <= 1e-8 + 1e-5 * fabs(ref))) ok = false;
+    }
+  if (!ok) atomicOr(a.error, kErrNotUnitary);
+  size_t gi = ((size_t)b * a.lmax + a.layer) * n + site;
+  for (int j = 0; j < 16; ++j) a.gates[gi * 16 + j] = m[j];
+  a.widths[gi] = width;
+  if (a.bonds && site < n - 1) a.bonds[((size_t)b * a.lmax + a.layer) * n + site] = rr;
+}
+
+// ------------------------------------------------------- early exit check ---
+// fidelity_with_zero_state (mps.py:1020-1039) and the break test of
+// sequential.py:390: |conj(psi[0]) - 1| <= atol + 1e-5.
+__global__ void amp0_kernel(Amp0Args a) {
+  int b = blockIdx.x * blockDim.x + threadIdx.x;
+  if (b >= a.batch) return;
+  if (a.skip[b]) return;
+  double2 v = a.psi[(size_t)b << a.n];
+  double2 c = cconj(v);
+  if (a.amp0) a.amp0[(size_t)b * a.lmax + a.layer] = c;
+  a.layers[b] = a.layer + 1;
+  double dx = c.x - 1.0, dy = c.y;
+  if (sqrt(dx * dx + dy * dy) <= a.atol + 1e-5) a.skip[b] = 1;
+}
+
+// ------------------------------------------------------------- launchers ----
+template <typename S>
+void launch_apply_pass(const PassArgs<S>& a, int nblk, int batch, hipStream_t st) {
+  size_t shm = sizeof(double2) << (a.tbits < a.n ? a.tbits : a.n);
+  hipLaunchKernelGGL(apply_pass_kernel<S>, dim3(nblk, batch), dim3(256), shm, st, a);
+}
+template void launch_apply_pass<float2>(const PassArgs<float2>&, int, int, hipStream_t);
+template void launch_apply_pass<double2>(const PassArgs<double2>&, int, int, hipStream_t);
+
+template <typename S>
+void launch_sweep_step(const SweepArgs<S>& a, int nblk, int batch, hipStream_t st) {
+  size_t shm = 2 * (sizeof(double2) << (a.tbits < a.n ? a.tbits : a.n));
+  hipLaunchKernelGGL(sweep_step_kernel<S>, dim3(nblk, batch), dim3(256), shm, st, a);
+}
+template void launch_sweep_step<float2>(const SweepArgs<float2>&, int, int, hipStream_t);
+template void launch_sweep_step<double2>(const SweepArgs<double2>&, int, int, hipStream_t);
+
+template <typename S>
+void launch_norm2(const S* x, int n, int batch, double* part, int* counter, double* out,
+                  int nblk, hipStream_t st) {
+  hipLaunchKernelGGL(norm2_kernel<S>, dim3(nblk, batch), dim3(256), 0, st, x, n, part, counter, out);
+}
+template void launch_norm2<float2>(const float2*, int, int, double*, int*, double*, int, hipStream_t);
+template void launch_norm2<double2>(const double2*, int, int, double*, int*, double*, int, hipStream_t);
+
+template <typename S, typename D>
+void launch_scale_convert(const S* x, D* y, int n, int batch, const double* norm2, hipStream_t st) {
+  size_t len = (size_t)1 << n;
+  int nblk = (int)((len + 255) / 256);
+  if (nblk > 1024) nblk = 1024;
+  hipLaunchKernelGGL((scale_convert_kernel<S, D>), dim3(nblk, batch), dim3(256), 0, st, x, y, n, norm2);
+}
+template void launch_scale_convert<float2, double2>(const float2*, double2*, int, int, const double*, hipStream_t);
+template void launch_scale_convert<double2, double2>(const double2*, double2*, int, int, const double*, hipStream_t);
+template void launch_scale_convert<double2, float2>(const double2*, float2*, int, int, const double*, hipStream_t);
+template void launch_scale_convert<float2, float2>(const float2*, float2*, int, int, const double*, hipStream_t);
+
+template <typename S>
+void launch_basis0(S* psi, int n, int batch, const int* skip, hipStream_t st) {
+  size_t len = (size_t)1 << n;
+  int nblk = (int)((len + 255) / 256);
+  if (nblk > 1024) nblk = 1024;
+  hipLaunchKernelGGL(basis0_kernel<S>, dim3(nblk, batch), dim3(256), 0, st, psi, n, skip);
+}
+template void launch_basis0<float2>(float2*, int, int, const int*, hipStream_t);
+template void launch_basis0<double2>(double2*, int, int, const int*, hipStream_t);
+
+void launch_fill_int(int* p, int v, int count, hipStream_t st) {
+  if (count <= 0) return;
+  hipLaunchKernelGGL(fill_int_kernel, dim3((count + 255) / 256), dim3(256), 0, st, p, v, count);
+}
+
+void launch_chain2_step(const Chain2Args& a, int nblk, int batch, hipStream_t st) {
+  hipLaunchKernelGGL(chain2_step_kernel, dim3(nblk, batch), dim3(256), 0, st, a);
+}
+
+void launch_ran_gates(const RanArgs& a, int batch, hipStream_t st) {
+  hipLaunchKernelGGL(ran_gates_kernel, dim3(batch), dim3(64), 0, st, a);
+}
+
+void launch_amp0(const Amp0Args& a, hipStream_t st) {
+  hipLaunchKernelGGL(amp0_kernel, dim3((a.batch + 63) / 64), dim3(64), 0, st, a);
+}
+
+}  // namespace qmx

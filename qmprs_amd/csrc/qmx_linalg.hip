@@ -1,0 +1,377 @@
+// FP64 linear algebra of the chi-truncation chain (statevector -> chi-MPS).
+//
+// Step i of the right-to-left truncated projection chain works on
+// Y_i = X_{i+1} viewed as a (2^i x n_i) row-major complex128 matrix:
+//   rho_i = Y_i^H Y_i             gram + gram_reduce   (n_i x n_i, ld = nmax)
+//   rho_i = V diag(lam) V^H       jacobi               (one block per state)
+//   W_i   = V[:, top k_i]         select               (rsum2 cutoff, max bond chi)
+//   X_i   = Y_i W_i               gemm
+// and the state is contracted back with Z_{i+1} = Z_i W_i^H (gemm, conj_b).
+// Shapes that depend on the cutoff live on the device (k_i, n_i per state);
+// launches are sized by host upper bounds and idle blocks exit early.
+#include "qmx_device.h"
+#include "qmx_internal.h"
+
+namespace qmx {
+
+// ------------------------------------------------------------------ gram ----
+// part[b][chunk] (nmax x nmax) = sum over the chunk's rows of conj(Y)[:, p] Y[:, q]
+__global__ __launch_bounds__(256) void gram_kernel(const double2* __restrict__ y, size_t y_stride,
+                                                   const int* ncols_dev, int rows_log2, int nmax,
+                                                   double2* part, int nchunks) {
+  __shared__ double2 ya[32][33];
+  __shared__ double2 yb[32][33];
+  const int b = blockIdx.z;
+  const int nc = ncols_dev[b];
+  const int ntile = (nmax + 31) / 32;
+  const int ti = blockIdx.x / ntile, tj = blockIdx.x % ntile;
+  if (ti * 32 >= nc || tj * 32 >= nc) return;
+  const size_t rows = (size_t)1 << rows_log2;
+  const size_t per = (rows + nchunks - 1) / nchunks;
+  const size_t r0 = per * blockIdx.y;
+  const size_t r1 = (r0 + per < rows) ? r0 + per : rows;
+  const double2* yb0 = y + (size_t)b * y_stride;
+  const int tx = threadIdx.x & 31, ty = threadIdx.x >> 5;
+  double2 acc[4] = {cz(), cz(), cz(), cz()};
+  for (size_t rb = r0; rb < r1; rb += 32) {
+    for (int e = threadIdx.x; e < 32 * 32; e += 256) {
+      int rr = e >> 5, cc = e & 31;
+      size_t row = rb + rr;
+      int ca = ti * 32 + cc, cb = tj * 32 + cc;
+      ya[rr][cc] = (row < r1 && ca < nc) ? yb0[row * nc + ca] : cz();
+      yb[rr][cc] = (row < r1 && cb < nc) ? yb0[row * nc + cb] : cz();
+    }
+    __syncthreads();
+#pragma unroll 4
+    for (int rr = 0; rr < 32; ++rr) {
+      double2 q = yb[rr][tx];
+#pragma unroll
+      for (int j = 0; j < 4; ++j) acc[j] = cfmac(ya[rr][ty * 4 + j], q, acc[j]);
+    }
+    __syncthreads();
+  }
+  double2* out = part + ((size_t)b * nchunks + blockIdx.y) * nmax * nmax;
+  for (int j = 0; j < 4; ++j) {
+    int p = ti * 32 + ty * 4 + j, q = tj * 32 + tx;
+    if (p < nc && q < nc) out[(size_t)p * nmax + q] = acc[j];
+  }
+}
+
+__global__ void gram_reduce_kernel(const double2* part, int nchunks, const int* ncols_dev, int nmax,
+                                   double2* a, size_t a_stride) {
+  const int b = blockIdx.y;
+  const int nc = ncols_dev[b];
+  int e = blockIdx.x * blockDim.x + threadIdx.x;
+  if (e >= nc * nc) return;
+  int p = e / nc, q = e % nc;
+  double2 s = cz();
+  for (int c = 0; c < nchunks; ++c)
+    s = cadd(s, part[((size_t)b * nchunks + c) * nmax * nmax + (size_t)p * nmax + q]);
+  a[(size_t)b * a_stride + (size_t)p * nmax + q] = s;
+}
+
+// ---------------------------------------------------------------- jacobi ----
+// Parallel cyclic two-sided Jacobi for a Hermitian n x n matrix (n even,
+// n <= 2 * 256), one block per state.  Round r pairs the indices by the circle
+// method; all n/2 rotations of a round are disjoint, so the update
+// A <- J^H A J is done blockwise: each 2x2 block (P, Q) of A becomes
+// J_P^H A_PQ J_Q, independently of every other block.  V accumulates J.
+constexpr int kJacobiMaxPairs = 256;
+
+__global__ __launch_bounds__(512) void jacobi_kernel(double2* a_all, double2* v_all, size_t stride,
+                                                     const int* ncols_dev, int ld, int* error) {
+  __shared__ int pp[kJacobiMaxPairs], qq[kJacobiMaxPairs];
+  __shared__ double jc[kJacobiMaxPairs], js[kJacobiMaxPairs];
+  __shared__ double2 je[kJacobiMaxPairs];
+  __shared__ int any_rot;
+  __shared__ double s_scale;
+  const int b = blockIdx.x;
+  const int n = ncols_dev[b];
+  double2* a = a_all + (size_t)b * stride;
+  double2* v = v_all + (size_t)b * stride;
+  const int tid = threadIdx.x, nt = blockDim.x;
+  for (int e = tid; e < n * n; e += nt) {
+    int i = e / n, j = e % n;
+    v[(size_t)i * ld + j] = make_double2(i == j ? 1.0 : 0.0, 0.0);
+  }
+  if (tid == 0) {
+    double tr = 0.0;
+    for (int i = 0; i < n; ++i) tr += fabs(a[(size_t)i * ld + i].x);
+    s_scale = tr;
+  }
+  __syncthreads();
+  if (n < 2) return;
+  const int np = n / 2;
+  const double abs_tol = 1e-17 * s_scale + 1e-300;
+  bool converged = false;
+  for (int sweep = 0; sweep < 40; ++sweep) {
+    __syncthreads();
+    if (tid == 0) any_rot = 0;
+    __syncthreads();
+    for (int r = 0; r < n - 1; ++r) {
+      if (tid < np) {
+        int m = tid;
+        int p = (m == 0) ? n - 1 : (r + m) % (n - 1);
+        int q = (r + n - 1 - m) % (n - 1);
+        if (m == 0) q = r % (n - 1);
+        if (p > q) { int t = p; p = q; q = t; }
+        double al = a[(size_t)p * ld + p].x, be = a[(size_t)q * ld + q].x;
+        double2 g = a[(size_t)p * ld + q];
+        double ag = cabs_(g);
+        double c = 1.0, s = 0.0;
+        double2 em = make_double2(1.0, 0.0);
+        if (ag > abs_tol && ag > 1e-15 * sqrt(fabs(al * be))) {
+          double tau = (be - al) / (2.0 * ag);
+          double t = (tau >= 0 ? 1.0 : -1.0) / (fabs(tau) + sqrt(1.0 + tau * tau));
+          c = 1.0 / sqrt(1.0 + t * t);
+          s = t * c;
+          em = make_double2(g.x / ag, -g.y / ag);   // e^{-i phi}
+          any_rot = 1;
+        }
+        pp[m] = p; qq[m] = q; jc[m] = c; js[m] = s; je[m] = em;
+      }
+      __syncthreads();
+      for (int e = tid; e < np * np; e += nt) {
+        int P = e / np, Q = e % np;
+        int p0 = pp[P], p1 = qq[P], q0 = pp[Q], q1 = qq[Q];
+        // J = [[c, s], [-s em, c em]]
+        double2 J00 = make_double2(jc[Q], 0.0), J01 = make_double2(js[Q], 0.0);
+        double2 J10 = cscale(je[Q], -js[Q]), J11 = cscale(je[Q], jc[Q]);
+        double2 K00 = make_double2(jc[P], 0.0), K01 = make_double2(js[P], 0.0);
+        double2 K10 = cscale(je[P], -js[P]), K11 = cscale(je[P], jc[P]);
+        double2 a00 = a[(size_t)p0 * ld + q0], a01 = a[(size_t)p0 * ld + q1];
+        double2 a10 = a[(size_t)p1 * ld + q0], a11 = a[(size_t)p1 * ld + q1];
+        double2 b00 = cadd(cmul(a00, J00), cmul(a01, J10));
+        double2 b01 = cadd(cmul(a00, J01), cmul(a01, J11));
+        double2 b10 = cadd(cmul(a10, J00), cmul(a11, J10));
+        double2 b11 = cadd(cmul(a10, J01), cmul(a11, J11));
+        double2 c00 = cadd(cmulc(K00, b00), cmulc(K10, b10));
+        double2 c01 = cadd(cmulc(K00, b01), cmulc(K10, b11));
+        double2 c10 = cadd(cmulc(K01, b00), cmulc(K11, b10));
+        double2 c11 = cadd(cmulc(K01, b01), cmulc(K11, b11));
+        if (P == Q) {
+          c00.y = 0.0; c11.y = 0.0;
+          c01 = cz(); c10 = cz();
+        }
+        a[(size_t)p0 * ld + q0] = c00; a[(size_t)p0 * ld + q1] = c01;
+        a[(size_t)p1 * ld + q0] = c10; a[(size_t)p1 * ld + q1] = c11;
+      }
+      for (int e = tid; e < n * np; e += nt) {
+        int i = e / np, Q = e % np;
+        int q0 = pp[Q], q1 = qq[Q];
+        double2 J00 = make_double2(jc[Q], 0.0), J01 = make_double2(js[Q], 0.0);
+        double2 J10 = cscale(je[Q], -js[Q]), J11 = cscale(je[Q], jc[Q]);
+        double2 x = v[(size_t)i * ld + q0], y = v[(size_t)i * ld + q1];
+        v[(size_t)i * ld + q0] = cadd(cmul(x, J00), cmul(y, J10));
+        v[(size_t)i * ld + q1] = cadd(cmul(x, J01), cmul(y, J11));
+      }
+      __syncthreads();
+    }
+    __syncthreads();
+    if (!any_rot) { converged = true; break; }
+  }
+  if (!converged && tid == 0 && error) atomicOr(error, kErrEigNoConv);
+}
+
+// ---------------------------------------------------------------- select ----
+// Sort eigenvalues (diag of A) in decreasing order, pick k (rsum2 cutoff and
+// max bond chi; k = n when n <= chi), and write the sorted eigenvectors into
+// the columns of w (same layout as v).  k_out[b] = k; spectra (optional).
+__global__ __launch_bounds__(512) void select_kernel(const double2* a_all, const double2* v_all,
+                                                     double2* w_all, size_t stride,
+                                                     const int* ncols_dev, int ld, int chi,
+                                                     double cutoff, int* k_out, int kstride,
+                                                     double* spectra, size_t spec_stride) {
+  __shared__ double lam[2 * kJacobiMaxPairs];
+  __shared__ int perm[2 * kJacobiMaxPairs];
+  __shared__ int s_k;
+  const int b = blockIdx.x;
+  const int n = ncols_dev[b];
+  const double2* a = a_all + (size_t)b * stride;
+  const double2* v = v_all + (size_t)b * stride;
+  double2* w = w_all + (size_t)b * stride;
+  for (int j = threadIdx.x; j < n; j += blockDim.x) {
+    double lj = a[(size_t)j * ld + j].x;
+    int rank = 0;
+    for (int i = 0; i < n; ++i) {
+      double li = a[(size_t)i * ld + i].x;
+      if (li > lj || (li == lj && i < j)) ++rank;
+    }
+    lam[rank] = lj;
+    perm[rank] = j;
+  }
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    int k;
+    if (n <= chi) {
+      k = n;
+    } else {
+      double total = 0.0;
+      for (int i = 0; i < n; ++i) total += fmax(lam[i], 0.0);
+      k = n;
+      if (total > 0.0 && cutoff > 0.0) {
+        double tail = 0.0;
+        for (int j = n - 1; j > 0; --j) {
+          tail += fmax(lam[j], 0.0);
+          if (tail <= cutoff * total) k = j; else break;
+        }
+      }
+      if (k > chi) k = chi;
+      if (k < 1) k = 1;
+    }
+    s_k = k;
+    k_out[(size_t)b * kstride] = k;
+  }
+  __syncthreads();
+  for (int e = threadIdx.x; e < n * n; e += blockDim.x) {
+    int i = e / n, c = e % n;
+    w[(size_t)i * ld + c] = v[(size_t)i * ld + perm[c]];
+  }
+  if (spectra)
+    for (int j = threadIdx.x; j < n; j += blockDim.x) spectra[(size_t)b * spec_stride + j] = lam[j];
+}
+
+// ------------------------------------------------------------------ gemm ----
+// C (m x nc, ld nc) = A (m x kd, ld kd) * op(B); B has leading dimension ldb.
+// 64 x 64 output tile per block, 4 x 4 per thread, K in chunks of 16.
+__global__ __launch_bounds__(256) void gemm_kernel(const double2* __restrict__ a_all, size_t a_stride,
+                                                   const double2* __restrict__ b_all, size_t b_stride,
+                                                   double2* c_all, size_t c_stride, int rows_log2,
+                                                   const int* kdim, const int* ncols, int dim_stride,
+                                                   int kmax, int nmax, int ldb, int conj_b) {
+  __shared__ double2 as[64][17];
+  __shared__ double2 bs[16][65];
+  const int b = blockIdx.z;
+  const int kd = kdim ? kdim[(size_t)b * dim_stride] : kmax;
+  const int nc = ncols ? ncols[(size_t)b * dim_stride] : nmax;
+  const size_t m = (size_t)1 << rows_log2;
+  const size_t row0 = (size_t)blockIdx.y * 64;
+  const int col0 = blockIdx.x * 64;
+  if (col0 >= nc || row0 >= m) return;
+  const double2* A = a_all + (size_t)b * a_stride;
+  const double2* B = b_all + (size_t)b * b_stride;
+  double2* C = c_all + (size_t)b * c_stride;
+  const int tx = threadIdx.x & 15, ty = threadIdx.x >> 4;
+  double2 acc[4][4];
+#pragma unroll
+  for (int i = 0; i < 4; ++i)
+#pragma unroll
+    for (int j = 0; j < 4; ++j) acc[i][j] = cz();
+  for (int k0 = 0; k0 < kd; k0 += 16) {
+    for (int e = threadIdx.x; e < 64 * 16; e += 256) {
+      int rr = e >> 4, kk = e & 15;
+      size_t row = row0 + rr;
+      int kc = k0 + kk;
+      as[rr][kk] = (row < m && kc < kd) ? A[row * kd + kc] : cz();
+    }
+    for (int e = threadIdx.x; e < 16 * 64; e += 256) {
+      int kk = e >> 6, cc = e & 63;
+      int kc = k0 + kk, col = col0 + cc;
+      double2 val = cz();
+      if (kc < kd && col < nc) val = conj_b ? cconj(B[(size_t)col * ldb + kc]) : B[(size_t)kc * ldb + col];
+      bs[kk][cc] = val;
+    }
+    __syncthreads();
+#pragma unroll 4
+    for (int kk = 0; kk < 16; ++kk) {
+      double2 av[4], bv[4];
+#pragma unroll
+      for (int i = 0; i < 4; ++i) av[i] = as[ty * 4 + i][kk];
+#pragma unroll
+      for (int j = 0; j < 4; ++j) bv[j] = bs[kk][tx * 4 + j];
+#pragma unroll
+      for (int i = 0; i < 4; ++i)
+#pragma unroll
+        for (int j = 0; j < 4; ++j) acc[i][j] = cfma(av[i], bv[j], acc[i][j]);
+    }
+    __syncthreads();
+  }
+#pragma unroll
+  for (int i = 0; i < 4; ++i) {
+    size_t row = row0 + ty * 4 + i;
+    if (row >= m) continue;
+#pragma unroll
+    for (int j = 0; j < 4; ++j) {
+      int col = col0 + tx * 4 + j;
+      if (col < nc) C[row * nc + col] = acc[i][j];
+    }
+  }
+}
+
+// ------------------------------------------------------------- launchers ----
+void launch_gram(const double2* y, size_t y_stride, const int* ncols_dev, int rows_log2,
+                 int ncols_max, double2* part, int nchunks, int batch, hipStream_t st) {
+  int nt = (ncols_max + 31) / 32;
+  hipLaunchKernelGGL(gram_kernel, dim3(nt * nt, nchunks, batch), dim3(256), 0, st, y, y_stride,
+                     ncols_dev, rows_log2, ncols_max, part, nchunks);
+}
+
+void launch_gram_reduce(const double2* part, int nchunks, const int* ncols_dev, int ncols_max,
+                        double2* a, size_t a_stride, int batch, hipStream_t st) {
+  int tot = ncols_max * ncols_max;
+  hipLaunchKernelGGL(gram_reduce_kernel, dim3((tot + 255) / 256, batch), dim3(256), 0, st, part,
+                     nchunks, ncols_dev, ncols_max, a, a_stride);
+}
+
+void launch_jacobi(double2* a, double2* v, size_t stride, const int* ncols_dev, int ncols_max,
+                   int batch, int* error, hipStream_t st) {
+  hipLaunchKernelGGL(jacobi_kernel, dim3(batch), dim3(512), 0, st, a, v, stride, ncols_dev,
+                     ncols_max, error);
+}
+
+void launch_select(const double2* a, const double2* v, double2* w, size_t stride,
+                   const int* ncols_dev, int ncols_max, int chi, double cutoff, int* k_out,
+                   int kstride, double* spectra, size_t spec_stride, int batch, hipStream_t st) {
+  hipLaunchKernelGGL(select_kernel, dim3(batch), dim3(512), 0, st, a, v, w, stride, ncols_dev,
+                     ncols_max, chi, cutoff, k_out, kstride, spectra, spec_stride);
+}
+
+// nn[b][step] = ncur[b] = first_cols (> 0) or 2 * kk[b][step + 1]
+__global__ void chain_dims_kernel(int* nn, int* ncur, const int* kk, int n, int step,
+                                  int first_cols, int batch) {
+  int b = blockIdx.x * blockDim.x + threadIdx.x;
+  if (b >= batch) return;
+  int v = first_cols > 0 ? first_cols : 2 * kk[(size_t)b * (n + 1) + step + 1];
+  nn[(size_t)b * (n + 1) + step] = v;
+  ncur[b] = v;
+}
+
+void launch_chain_dims(int* nn, int* ncur, const int* kk, int n, int step, int first_cols,
+                       int batch, hipStream_t st) {
+  hipLaunchKernelGGL(chain_dims_kernel, dim3((batch + 63) / 64), dim3(64), 0, st, nn, ncur, kk, n,
+                     step, first_cols, batch);
+}
+
+// Sweep early stop: skip[b] = 1 once |<psi|C|0>| >= goal[b]; active[0] counts
+// the states still running, whose sweeps_done[b] is incremented.
+__global__ void goal_check_kernel(const double2* dot, int dot_stride, const double* goal, int* skip,
+                                  int* active, int* sweeps_done, int batch) {
+  int b = blockIdx.x * blockDim.x + threadIdx.x;
+  if (b >= batch) return;
+  if (!skip[b]) {
+    double2 d = dot[(size_t)b * dot_stride];
+    double f = sqrt(d.x * d.x + d.y * d.y);
+    if (f >= goal[b]) {
+      skip[b] = 1;
+    } else {
+      atomicAdd(active, 1);
+      if (sweeps_done) sweeps_done[b] += 1;
+    }
+  }
+}
+
+void launch_goal_check(const double2* dot, int dot_stride, const double* goal, int* skip,
+                       int* active, int* sweeps_done, int batch, hipStream_t st) {
+  hipLaunchKernelGGL(goal_check_kernel, dim3((batch + 63) / 64), dim3(64), 0, st, dot, dot_stride,
+                     goal, skip, active, sweeps_done, batch);
+}
+
+void launch_gemm(const double2* a, size_t a_stride, const double2* bm, size_t b_stride, double2* c,
+                 size_t c_stride, const GemmDims& d, int conj_b, int batch, hipStream_t st) {
+  size_t m = (size_t)1 << d.rows_log2;
+  dim3 grid((d.nmax + 63) / 64, (unsigned)((m + 63) / 64), batch);
+  hipLaunchKernelGGL(gemm_kernel, grid, dim3(256), 0, st, a, a_stride, bm, b_stride, c, c_stride,
+                     d.rows_log2, d.kdim, d.ncols, d.dim_stride, d.kmax, d.nmax, d.ldb, conj_b);
+}
+
+}  // namespace qmx

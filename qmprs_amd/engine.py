@@ -1,0 +1,228 @@
+"""Batched GPU engine: thin Python wrappers over the libqmx C-ABI.
+
+Every function takes and returns torch tensors on the current ROCm device and
+enqueues its kernels on the current stream.  A batch holds states of equal
+qubit count, one per row.
+"""
+
+from __future__ import annotations
+
+import ctypes
+from dataclasses import dataclass
+
+import numpy as np
+import torch
+
+from qmprs_amd import _lib
+
+CUTOFF = 1e-10
+DEFAULT_THRESHOLD = 1 - 1e-6
+
+_DTYPES = {torch.complex64: _lib.QMX_C64, torch.complex128: _lib.QMX_C128}
+
+
+def num_qubits_of(length: int) -> int:
+    n = int(length).bit_length() - 1
+    if length < 2 or (1 << n) != length:
+        raise ValueError("The statevector length must be a power of two.")
+    return n
+
+
+def as_device_states(statevectors, dtype=None) -> torch.Tensor:
+    """(B, 2^N) complex tensor on the GPU (accepts numpy / torch / 1-D)."""
+    dev = _lib.require_gpu()
+    if isinstance(statevectors, torch.Tensor):
+        t = statevectors
+    else:
+        arr = np.asarray(statevectors)
+        if not np.iscomplexobj(arr):
+            arr = arr.astype(np.complex128)
+        t = torch.from_numpy(np.ascontiguousarray(arr))
+    if t.dim() == 1:
+        t = t.unsqueeze(0)
+    if not t.is_complex():
+        t = t.to(torch.complex128)
+    if dtype is not None:
+        t = t.to(dtype)
+    if t.dtype not in _DTYPES:
+        t = t.to(torch.complex128)
+    return t.to(dev).contiguous()
+
+
+def chi_truncate(states: torch.Tensor, bond_dimension: int, cutoff: float = CUTOFF,
+                 spectra: bool = False):
+    """(a)+(b): normalised inputs and the chi-MPS states (complex128)."""
+    b, length = states.shape
+    n = num_qubits_of(length)
+    lib, h = _lib.load(), _lib.handle()
+    psi = torch.empty((b, length), dtype=torch.complex128, device=states.device)
+    target = torch.empty_like(psi)
+    spec = (torch.zeros((b, n + 1, 2 * bond_dimension), dtype=torch.float64, device=states.device)
+            if spectra else None)
+    ws = _lib.workspace(lib.qmx_chi_truncate_workspace(n, b, bond_dimension), states.device)
+    st = lib.qmx_chi_truncate(h, n, b, bond_dimension, cutoff, _lib.ptr(states), _DTYPES[states.dtype],
+                              _lib.ptr(psi), _lib.ptr(target), _lib.ptr(spec), _lib.ptr(ws), ws.numel(),
+                              _lib.stream_ptr())
+    _lib.check(st, h)
+    return psi, target, spec
+
+
+def extract_layers(psi_work: torch.Tensor, num_layers: int,
+                   fidelity_threshold: float = DEFAULT_THRESHOLD, cutoff: float = CUTOFF,
+                   with_bonds: bool = False):
+    """(b)-(d): layer loop of _get_unitary_layers on normalised states (in place)."""
+    b, length = psi_work.shape
+    n = num_qubits_of(length)
+    lib, h = _lib.load(), _lib.handle()
+    dev = psi_work.device
+    gates = torch.zeros((b, num_layers, n, 4, 4), dtype=torch.complex128, device=dev)
+    widths = torch.ones((b, num_layers, n), dtype=torch.int32, device=dev)
+    bonds = torch.zeros((b, num_layers, n), dtype=torch.int32, device=dev) if with_bonds else None
+    used = torch.zeros(b, dtype=torch.int32, device=dev)
+    amp0 = torch.zeros((b, num_layers), dtype=torch.complex128, device=dev)
+    ws = _lib.workspace(lib.qmx_extract_layers_workspace(n, b), dev)
+    st = lib.qmx_extract_layers(h, n, b, num_layers, float(fidelity_threshold), cutoff,
+                                _lib.ptr(psi_work), _lib.ptr(gates), _lib.ptr(widths), _lib.ptr(bonds),
+                                _lib.ptr(used), _lib.ptr(amp0), _lib.ptr(ws), ws.numel(),
+                                _lib.stream_ptr())
+    _lib.check(st, h)
+    return gates, widths, used, amp0, bonds
+
+
+def generate_layer(psi_work: torch.Tensor, cutoff: float = CUTOFF):
+    """(b)+(c): one Ran layer (bond-2 compression + unitary completion)."""
+    b, length = psi_work.shape
+    n = num_qubits_of(length)
+    lib, h = _lib.load(), _lib.handle()
+    dev = psi_work.device
+    gates = torch.zeros((b, 1, n, 4, 4), dtype=torch.complex128, device=dev)
+    widths = torch.ones((b, 1, n), dtype=torch.int32, device=dev)
+    bonds = torch.zeros((b, 1, n), dtype=torch.int32, device=dev)
+    ws = _lib.workspace(lib.qmx_layer_workspace(n, b), dev)
+    st = lib.qmx_generate_layer(h, n, b, 1, 0, cutoff, _lib.ptr(psi_work), _lib.ptr(gates),
+                                _lib.ptr(widths), _lib.ptr(bonds), _lib.ptr(ws), ws.numel(),
+                                _lib.stream_ptr())
+    _lib.check(st, h)
+    return gates[:, 0], widths[:, 0], bonds[:, 0]
+
+
+def apply_layer(psi: torch.Tensor, gates: torch.Tensor, widths: torch.Tensor, inverse: bool):
+    """(d): apply one layer (gates (B, N, 4, 4), widths (B, N)) in place."""
+    b, length = psi.shape
+    n = num_qubits_of(length)
+    if psi.dtype != torch.complex128:
+        raise ValueError("apply_layer works on complex128 states")
+    lib, h = _lib.load(), _lib.handle()
+    g = gates.reshape(b, 1, n, 4, 4).to(torch.complex128).contiguous()
+    w = widths.reshape(b, 1, n).to(torch.int32).contiguous()
+    st = lib.qmx_apply_layer(h, n, b, 1, 0, 1 if inverse else 0, _lib.ptr(psi), _lib.ptr(g), _lib.ptr(w),
+                             _lib.stream_ptr())
+    _lib.check(st, h)
+    return psi
+
+
+@dataclass
+class SweepResult:
+    fidelity: torch.Tensor       # (B, S+1) complex128: <psi|C|0> before each sweep and at the end
+    overlap: torch.Tensor        # (B, S) float64: <T|C'|0> after each sweep
+    sweeps_done: torch.Tensor    # (B,) int32
+    circuit: torch.Tensor | None
+
+
+def sweep(target: torch.Tensor, gates: torch.Tensor, widths: torch.Tensor, layers_used: torch.Tensor,
+          num_sweeps: int, psi_ref: torch.Tensor | None = None, goal: torch.Tensor | None = None,
+          circuit_out: bool = False) -> SweepResult:
+    """(e): environment sweeps, gates updated in place."""
+    b, length = target.shape
+    n = num_qubits_of(length)
+    num_layers = gates.shape[1]
+    lib, h = _lib.load(), _lib.handle()
+    dev = target.device
+    dt = _DTYPES[target.dtype]
+    fid = torch.zeros((b, num_sweeps + 1), dtype=torch.complex128, device=dev) if psi_ref is not None else None
+    ov = torch.zeros((b, max(num_sweeps, 1)), dtype=torch.float64, device=dev)
+    done = torch.zeros(b, dtype=torch.int32, device=dev)
+    circ = torch.empty_like(target) if circuit_out else None
+    ws = _lib.workspace(lib.qmx_sweep_workspace(n, b, dt), dev)
+    st = lib.qmx_sweep(h, n, b, num_layers, _lib.ptr(layers_used), num_sweeps, _lib.ptr(target), dt,
+                       _lib.ptr(gates), _lib.ptr(widths), _lib.ptr(psi_ref), _lib.ptr(fid),
+                       _lib.ptr(ov if num_sweeps > 0 else None), _lib.ptr(goal), _lib.ptr(done),
+                       _lib.ptr(circ), _lib.ptr(ws), ws.numel(), _lib.stream_ptr())
+    _lib.check(st, h)
+    return SweepResult(fid, ov[:, :num_sweeps], done, circ)
+
+
+@dataclass
+class EncodeResult:
+    gates: torch.Tensor          # (B, L, N, 4, 4) complex128, layer = extraction order
+    widths: torch.Tensor         # (B, L, N) int32
+    layers_used: torch.Tensor    # (B,) int32
+    fidelity: torch.Tensor       # (B, S+1) complex128
+    sweeps_done: torch.Tensor    # (B,) int32
+
+
+def encode(statevectors, bond_dimension: int, num_layers: int = 1, num_sweeps: int = 0,
+           fidelity_threshold: float = DEFAULT_THRESHOLD, cutoff: float = CUTOFF,
+           sweep_dtype=torch.complex64, goal: torch.Tensor | None = None,
+           workspace: torch.Tensor | None = None) -> EncodeResult:
+    """Whole prepare_state for a batch of equal-size states in one C-ABI call."""
+    sv = as_device_states(statevectors)
+    b, length = sv.shape
+    n = num_qubits_of(length)
+    lib, h = _lib.load(), _lib.handle()
+    cfg = _lib.EncodeConfig(n, b, int(bond_dimension), int(num_layers), int(num_sweeps),
+                            _DTYPES[sv.dtype], _DTYPES[sweep_dtype], float(fidelity_threshold),
+                            float(cutoff))
+    need = lib.qmx_encode_workspace(ctypes.byref(cfg))
+    if workspace is None or workspace.numel() < need:
+        workspace = _lib.workspace(need, sv.device)
+    gates = torch.zeros((b, num_layers, n, 4, 4), dtype=torch.complex128, device=sv.device)
+    widths = torch.ones((b, num_layers, n), dtype=torch.int32, device=sv.device)
+    used = torch.zeros(b, dtype=torch.int32, device=sv.device)
+    fid = torch.zeros((b, num_sweeps + 1), dtype=torch.complex128, device=sv.device)
+    done = torch.zeros(b, dtype=torch.int32, device=sv.device)
+    st = lib.qmx_encode(h, ctypes.byref(cfg), _lib.ptr(sv), _lib.ptr(gates), _lib.ptr(widths),
+                        _lib.ptr(used), _lib.ptr(fid), _lib.ptr(goal), _lib.ptr(done),
+                        _lib.ptr(workspace), workspace.numel(), _lib.stream_ptr())
+    _lib.check(st, h)
+    return EncodeResult(gates, widths, used, fid, done)
+
+
+def gates_to_layers(gates: np.ndarray, widths: np.ndarray, layers_used: int):
+    """Device gate arrays of one state -> list[UnitaryLayer] in circuit order.
+
+    Blocks follow the widths: a block ends at every one-qubit gate
+    (mps.py:816-834); the list is reversed as sequential.py:396 does.
+    """
+    from qmprs_amd.primitives.mps import Tensor
+    n = gates.shape[1]
+    layers = []
+    for l in range(int(layers_used)):
+        blocks = []
+        start = 0
+        for i in range(n):
+            if int(widths[l, i]) == 1:
+                mats = []
+                for j in range(start, i + 1):
+                    d = 4 if j < i else 2
+                    mats.append(Tensor(np.array(gates[l, j, :d, :d], dtype=np.complex128)))
+                blocks.append((start, i, mats))
+                start = i + 1
+        layers.append(blocks)
+    layers.reverse()
+    return layers
+
+
+def layers_to_arrays(layers, num_sites: int):
+    """list[UnitaryLayer] (circuit order) -> (gates (L,N,4,4), widths (L,N)) in extraction order."""
+    nl = len(layers)
+    gates = np.zeros((nl, num_sites, 4, 4), dtype=np.complex128)
+    widths = np.ones((nl, num_sites), dtype=np.int32)
+    for l, layer in enumerate(reversed(layers)):
+        for start, end, mats in layer:
+            for j in range(start, end + 1):
+                m = np.asarray(getattr(mats[j - start], "data", mats[j - start]), dtype=np.complex128)
+                d = m.shape[0]
+                gates[l, j, :d, :d] = m
+                widths[l, j] = 2 if d == 4 else 1
+    return gates, widths

@@ -1,0 +1,177 @@
+"""GPU-resident MPS primitive (mirror of qmprs/primitives/mps.py).
+
+The reference wraps a quimb ``MatrixProductState``.  Here the MPS is kept as
+the dense state it represents (complex128 on the GPU): every operation of the
+encoding path either acts on that state (normalisation, layer application,
+overlap with |0..0>) or rebuilds the site tensors it needs from it with the
+right-to-left truncated projection chain (chi-truncation, bond-2 compression
+for Ran layers).
+"""
+
+from __future__ import annotations
+
+from typing import Literal, TypeAlias
+
+import numpy as np
+import torch
+
+from qmprs_amd import engine
+
+
+class Tensor:
+    """Gate container with the ``.data`` / ``.inds`` / ``.tags`` the reference's
+    ``qtn.Tensor`` gates expose (mps.py:617-621)."""
+
+    __slots__ = ("data", "inds", "tags")
+
+    def __init__(self, data, inds=("L", "R"), tags=("G",)) -> None:
+        self.data = np.asarray(data, dtype=np.complex128)
+        self.inds = tuple(inds)
+        self.tags = set(tags)
+
+    def __repr__(self) -> str:
+        return f"Tensor(shape={self.data.shape})"
+
+
+UnitaryBlock: TypeAlias = tuple[int, int, list[Tensor]]
+UnitaryLayer: TypeAlias = list[UnitaryBlock]
+
+
+def _normalised(statevector) -> np.ndarray:
+    """quick ``Ket``: flat complex128 unit vector (base.py:96-97)."""
+    sv = np.asarray(getattr(statevector, "data", statevector), dtype=np.complex128).reshape(-1)
+    engine.num_qubits_of(sv.size)
+    nrm = np.linalg.norm(sv)
+    if nrm == 0:
+        raise ValueError("The statevector must be non-zero.")
+    return sv / nrm
+
+
+class MPS:
+    """``qmprs.primitives.MPS`` (mps.py:50-216) on the GPU.
+
+    Parameters
+    ----------
+    statevector : array-like or Ket, optional
+    mps : sequence of 'lpr' site tensors (numpy, shapes (1,2,r), (l,2,r), (l,2,1)), optional
+    bond_dimension : int, default 64
+    """
+
+    def __init__(self, statevector=None, mps=None, bond_dimension: int = 64) -> None:
+        if not isinstance(bond_dimension, int) or isinstance(bond_dimension, bool) or bond_dimension < 1:
+            raise ValueError(
+                "`bond_dimension` must be an integer greater than 0. "
+                f"Received {bond_dimension}.")
+        if statevector is not None and mps is None:
+            sv = _normalised(statevector)
+            if sv.size == 2:
+                raise ValueError(f"The statevector must have at least 2 qubits. Received 1.")
+            self.statevector = sv
+            states = engine.as_device_states(sv)
+            psi, target, _ = engine.chi_truncate(states, bond_dimension)
+            self._psi = psi
+            self._state = target
+        elif mps is not None and statevector is None:
+            if isinstance(mps, (str, bytes)) or not hasattr(mps, "__len__"):
+                raise TypeError(
+                    "`mps` must be a sequence of MPS site tensors. "
+                    f"Received {type(mps)}.")
+            if len(mps) == 1:
+                raise ValueError(f"The MPS must have at least 2 tensors. Received {len(mps)}.")
+            tensors = [np.asarray(a, dtype=np.complex128) for a in mps]
+            if any(a.ndim != 3 or a.shape[1] != 2 for a in tensors):
+                raise ValueError("Only supports MPS with physical dimension of 2.")
+            # contract on the GPU (MPS.to_statevector, mps.py:183-197)
+            dev = engine.as_device_states(np.zeros(2)).device
+            dense = torch.from_numpy(tensors[0]).to(dev).reshape(-1, tensors[0].shape[-1])
+            for a in tensors[1:]:
+                at = torch.from_numpy(a).to(dev)
+                dense = (dense @ at.reshape(a.shape[0], -1)).reshape(-1, a.shape[-1])
+            dense = dense.reshape(1, -1).contiguous()
+            nrm = torch.linalg.vector_norm(dense)
+            if float(nrm) == 0.0:
+                raise ValueError("The MPS must represent a non-zero state.")
+            self._state = dense
+            self._psi = (dense / nrm).contiguous()
+            self.statevector = self._psi[0].cpu().numpy()
+        else:
+            raise ValueError("Must provide either `statevector` or `mps` not both.")
+        self.bond_dimension = bond_dimension
+        self.num_sites = engine.num_qubits_of(self.statevector.size)
+        self.physical_dimension = 2
+
+    # -- state access --------------------------------------------------------
+    @property
+    def state(self) -> torch.Tensor:
+        """Dense state of the MPS on the GPU, shape (1, 2^N), complex128."""
+        return self._state
+
+    @property
+    def norm(self) -> float:
+        return float(torch.linalg.vector_norm(self._state).item())
+
+    @property
+    def is_normalized(self) -> bool:
+        return bool(np.isclose(self.norm, 1))
+
+    def normalize(self) -> None:
+        """mps.py:302-310."""
+        if not self.is_normalized:
+            self._state = self._state / self.norm
+
+    def to_statevector(self) -> np.ndarray:
+        """mps.py:251-270: dense statevector (host copy)."""
+        return self._state[0].cpu().numpy()
+
+    # -- canonical forms / compression (mps.py:348-459) ----------------------
+    def canonicalize(self, mode: Literal["left", "right"], normalize: bool = False) -> None:
+        """Canonical forms change only the gauge; the represented state is kept
+        (and normalised when asked)."""
+        if mode not in ("left", "right"):
+            raise ValueError("`mode` must be either 'left' or 'right'.")
+        if normalize:
+            self._state = self._state / self.norm
+
+    def compress(self, max_bond_dimension: int | None = None,
+                 mode: Literal["left", "right"] | None = None) -> None:
+        if mode is not None and mode not in ("left", "right"):
+            raise ValueError(f"`mode` must be either 'left', or 'right'. Received {mode}.")
+        if max_bond_dimension:
+            _, self._state, _ = engine.chi_truncate(self._state, int(max_bond_dimension))
+            self.bond_dimension = int(max_bond_dimension)
+
+    def permute(self, shape: Literal["lrp", "lpr"]) -> None:
+        if shape not in ("lrp", "lpr"):
+            raise ValueError(f"`shape` must be either 'lrp' or 'lpr'. Received {shape}.")
+
+    # -- layers (mps.py:849-1039) --------------------------------------------
+    def generate_bond_D_unitary_layer(self) -> UnitaryLayer:
+        """Bond-2 compression + Ran unitaries of the current state."""
+        work = (self._state / self.norm).contiguous()
+        gates, widths, _ = engine.generate_layer(work)
+        return engine.gates_to_layers(gates.cpu().numpy()[0][None], widths.cpu().numpy()[0][None], 1)[0]
+
+    def apply_unitary_layer(self, unitary_layer: UnitaryLayer, inverse: bool = False) -> None:
+        g, w = engine.layers_to_arrays([unitary_layer], self.num_sites)
+        gt = torch.from_numpy(g[0]).to(self._state.device).unsqueeze(0)
+        wt = torch.from_numpy(w[0]).to(self._state.device).unsqueeze(0)
+        state = self._state.clone()
+        engine.apply_layer(state, gt, wt, inverse)
+        self._state = state
+
+    def apply_unitary_layers(self, unitary_layers: list[UnitaryLayer], inverse: bool = False) -> None:
+        for layer in reversed(unitary_layers):
+            self.apply_unitary_layer(layer, inverse=inverse)
+
+    def fidelity_with_zero_state(self) -> complex:
+        """conj(<0..0|psi>) (mps.py:1020-1039), O(1) on the dense state."""
+        return complex(np.conj(self._state[0, 0].item()))
+
+    def __len__(self) -> int:
+        return self.num_sites
+
+    def __str__(self) -> str:
+        return (f"{self.__class__.__name__}(num_sites={self.num_sites}, "
+                f"bond_dimension={self.bond_dimension}, normalized={self.is_normalized})")
+
+    __repr__ = __str__

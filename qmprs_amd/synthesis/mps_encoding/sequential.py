@@ -1,0 +1,116 @@
+"""Sequential encoder (mirror of qmprs/synthesis/mps_encoding/sequential.py).
+
+Ran's analytic disentangling layers followed by Rudolph et al.'s environment
+sweeps (DallOall), with every arithmetic step on the GPU through libqmx:
+
+* ``_get_unitary_layers``      sequential.py:330-398 -> qmx_extract_layers
+* ``_optimize_unitary_layers`` sequential.py:509-541 -> qmx_sweep
+* ``_circuit_from_unitary_layers`` sequential.py:189-213 (host: gate emission)
+"""
+
+from __future__ import annotations
+
+__all__ = ["Sequential"]
+
+import torch
+
+from qmprs_amd import engine
+from qmprs_amd.primitives.mps import MPS, UnitaryLayer
+from qmprs_amd.synthesis.mps_encoding.base import MPSEncoder
+
+
+class Sequential(MPSEncoder):
+    """``Sequential(circuit_framework)`` (sequential.py:31-153).
+
+    Extra keyword arguments of ``prepare_mps`` / ``prepare_state`` beyond the
+    reference's ``num_layers`` and ``num_sweeps``:
+    ``sweep_dtype`` (torch.complex64 default, or torch.complex128) -- storage
+    of the sweep's dense states.
+    """
+
+    def __init__(self, circuit_framework) -> None:
+        super().__init__(circuit_framework)
+        self._fidelity_threshold = 1 - 1e-6
+        self.last_info: dict = {}
+
+    @property
+    def fidelity_threshold(self) -> float:
+        return self._fidelity_threshold
+
+    @fidelity_threshold.setter
+    def fidelity_threshold(self, threshold: float) -> None:
+        if not isinstance(threshold, (int, float)) or threshold < 0 or threshold > 1:
+            raise ValueError("The fidelity threshold must be a float between 0 and 1.")
+        self._fidelity_threshold = threshold
+
+    # -- emission (sequential.py:155-213) -----------------------------------
+    @staticmethod
+    def _apply_unitary_layer_to_circuit(circuit, unitary_layer: UnitaryLayer) -> None:
+        """Gates go to qubit N-1-site (LSB convention of the circuit backend):
+        a two-site gate on sites (i, i+1) to qubits [N-2-i, N-1-i]."""
+        n = circuit.num_qubits
+        for start, end, block in unitary_layer:
+            for site in range(start, end + 1):
+                matrix = block[site - start].data
+                if site == end:
+                    circuit.unitary(matrix, n - 1 - site)
+                else:
+                    circuit.unitary(matrix, [n - 2 - site, n - 1 - site])
+
+    def _circuit_from_unitary_layers(self, mps: MPS, unitary_layers: list[UnitaryLayer]):
+        circuit = self.circuit_framework(mps.num_sites)
+        for layer in unitary_layers:
+            Sequential._apply_unitary_layer_to_circuit(circuit, layer)
+        return circuit
+
+    # -- GPU phases ---------------------------------------------------------
+    def _device_layers(self, mps: MPS, num_layers: int):
+        work = (mps.state / mps.norm).contiguous()
+        gates, widths, used, amp0, _ = engine.extract_layers(
+            work, num_layers, self.fidelity_threshold)
+        return gates, widths, used, amp0
+
+    def _get_unitary_layers(self, mps: MPS, num_layers: int) -> list[UnitaryLayer]:
+        gates, widths, used, _ = self._device_layers(mps, num_layers)
+        return engine.gates_to_layers(gates[0].cpu().numpy(), widths[0].cpu().numpy(),
+                                      int(used[0].item()))
+
+    def _optimize_unitary_layers(self, mps: MPS, unitary_layers: list[UnitaryLayer],
+                                 num_sweeps: int, sweep_dtype=torch.complex64) -> list[UnitaryLayer]:
+        g, w = engine.layers_to_arrays(unitary_layers, mps.num_sites)
+        dev = mps.state.device
+        gates = torch.from_numpy(g).to(dev).unsqueeze(0).contiguous()
+        widths = torch.from_numpy(w).to(dev).unsqueeze(0).contiguous()
+        used = torch.tensor([g.shape[0]], dtype=torch.int32, device=dev)
+        target = mps.state.to(sweep_dtype).contiguous()
+        res = engine.sweep(target, gates, widths, used, num_sweeps, psi_ref=mps._psi)
+        self.last_info = {"fidelity_trace": res.fidelity[0].abs().cpu().numpy(),
+                          "overlap_trace": res.overlap[0].cpu().numpy()}
+        return engine.gates_to_layers(gates[0].cpu().numpy(), widths[0].cpu().numpy(), g.shape[0])
+
+    def _sequential_unitary_circuit(self, mps: MPS, num_layers: int, num_sweeps: int = 0,
+                                    sweep_dtype=torch.complex64):
+        """sequential.py:543-586 (DallOall) with the gates kept on the GPU
+        between the layer and sweep phases."""
+        gates, widths, used, amp0 = self._device_layers(mps, num_layers)
+        target = mps.state.to(sweep_dtype).contiguous()
+        res = engine.sweep(target, gates, widths, used, num_sweeps, psi_ref=mps._psi)
+        fid = res.fidelity[0].abs().cpu().numpy()
+        self.last_info = {"fidelity": float(fid[-1]), "fidelity_trace": fid,
+                          "overlap_trace": res.overlap[0].cpu().numpy(),
+                          "layers_used": int(used[0].item()),
+                          "amp0": amp0[0].cpu().numpy()}
+        layers = engine.gates_to_layers(gates[0].cpu().numpy(), widths[0].cpu().numpy(),
+                                        int(used[0].item()))
+        return self._circuit_from_unitary_layers(mps, layers)
+
+    def prepare_mps(self, mps: MPS, **kwargs):
+        """sequential.py:588-600."""
+        num_layers = kwargs.get("num_layers", 1)
+        num_sweeps = kwargs.get("num_sweeps", 0)
+        if not isinstance(num_layers, int) or num_layers < 1:
+            raise ValueError("The number of layers must be a positive integer.")
+        if not isinstance(num_sweeps, int) or num_sweeps < 0:
+            raise ValueError("The number of sweeps must be a non-negative integer.")
+        return self._sequential_unitary_circuit(mps, num_layers, num_sweeps,
+                                                kwargs.get("sweep_dtype", torch.complex64))

@@ -1,0 +1,251 @@
+"""GPU parity: libqmx (through the C-ABI) against the FP64 oracle.
+
+Every comparison runs the HIP path and the oracle on the same seeded input.
+Tolerances (stated per test):
+  complex128 paths  - gates and states to 1e-8 .. 1e-10 (rounding only)
+  complex64 sweeps  - fidelity within 1e-6 and C|0..0> within 1e-5 (L2) of the
+                      oracle, the bar of the encoder's precision split.
+"""
+
+import numpy as np
+import pytest
+import torch
+
+from oracle import qmprs_oracle as O
+
+pytestmark = pytest.mark.gpu
+
+
+def _engine():
+    from qmprs_amd import engine
+    return engine
+
+
+def _states(kind, n, seed, bond=8):
+    if kind == "uniform":
+        return O.random_uniform_state(n, seed)
+    if kind == "mps":
+        return O.random_mps_state(n, bond, seed)
+    raise ValueError(kind)
+
+
+def _prefix_error(n, gl_a, gl_b):
+    """max_j || G_j..G_1|0> - G'_j..G'_1|0> ||: the circuit's action on every
+    input it ever sees.  A gate is unique only there -- where its environment
+    is rank deficient (qubits still in |0>) the polar factor is free on the
+    unpopulated inputs, for LAPACK as for any other SVD."""
+    a = np.zeros(2 ** n, dtype=np.complex128)
+    a[0] = 1
+    b = a.copy()
+    worst = 0.0
+    for (sa, wa, ga), (sb, wb, gb) in zip(gl_a, gl_b):
+        assert (sa, wa) == (sb, wb)
+        a = O.apply_gate(a, n, ga, sa, wa)
+        b = O.apply_gate(b, n, gb, sb, wb)
+        worst = max(worst, np.linalg.norm(a - b))
+    return worst
+
+
+def _gate_list_from_device(gates, widths, used):
+    """Device arrays of one state -> oracle gate list (circuit order)."""
+    eng = _engine()
+    layers = eng.gates_to_layers(gates, widths, used)
+    out = []
+    for layer in layers:
+        for start, end, mats in layer:
+            for j in range(start, end + 1):
+                out.append((j, 1 if j == end else 2, mats[j - start].data))
+    return out
+
+
+@pytest.mark.parametrize("n,chi,kind,seed", [
+    (4, 2, "uniform", 1), (6, 4, "uniform", 2), (8, 8, "mps", 3), (10, 16, "mps", 4),
+    (10, 512, "uniform", 5), (12, 24, "mps", 6), (14, 32, "mps", 7)])
+def test_chi_truncate_matches_oracle(gpu, n, chi, kind, seed):
+    eng = _engine()
+    sv = _states(kind, n, seed, bond=2 * chi)
+    psi, target, spec = eng.chi_truncate(eng.as_device_states(sv), chi, spectra=True)
+    torch.cuda.synchronize()
+    ref, ref_spec = O.chi_truncate(O.ket(sv), n, chi)
+    got = target[0].cpu().numpy()
+    err = np.linalg.norm(got - ref) / np.linalg.norm(ref)
+    print(f"n={n} chi={chi} rel.err={err:.3e}")
+    assert err < 1e-10
+    np.testing.assert_allclose(psi[0].cpu().numpy(), O.ket(sv), atol=1e-14)
+    sp = spec[0].cpu().numpy()
+    for i, lam in enumerate(ref_spec):
+        if lam is None:
+            continue
+        k = min(len(lam), chi)
+        # retained squared singular values, relative to the largest
+        np.testing.assert_allclose(sp[i, :k], lam[:k], rtol=1e-9, atol=1e-13 * lam[0])
+
+
+@pytest.mark.parametrize("n,kind,seed", [(2, "uniform", 0), (3, "uniform", 1), (5, "uniform", 2),
+                                         (8, "mps", 3), (11, "uniform", 4)])
+def test_generate_layer_matches_oracle(gpu, n, kind, seed):
+    eng = _engine()
+    sv = O.ket(_states(kind, n, seed))
+    g, w, bonds = eng.generate_layer(eng.as_device_states(sv, torch.complex128))
+    torch.cuda.synchronize()
+    ref_layer = O.generate_layer(O.bond2_mps(sv, n))
+    got = eng.gates_to_layers(g.cpu().numpy(), w.cpu().numpy(), 1)[0]
+    assert [(s, e) for s, e, _ in got] == [(s, e) for s, e, _ in ref_layer]
+    worst = 0.0
+    for (_, _, gm), (_, _, rm) in zip(got, ref_layer):
+        for a, b in zip(gm, rm):
+            worst = max(worst, np.abs(a.data - b).max())
+    print(f"n={n} max gate diff {worst:.3e}")
+    assert worst < 1e-10
+
+
+def test_generate_layer_partial_entanglement(gpu):
+    """Blocks follow the bond-2 bond dims (quick _get_submps_indices)."""
+    eng = _engine()
+    from qmprs_amd.circuit import GateCircuit
+    c = GateCircuit(8)
+    c.H(0); c.CX(0, 3); c.H(4); c.H(5); c.CX(5, 7)
+    sv = c.get_statevector()
+    g, w, _ = eng.generate_layer(eng.as_device_states(sv, torch.complex128))
+    ref = O.generate_layer(O.bond2_mps(sv, 8))
+    got = eng.gates_to_layers(g.cpu().numpy(), w.cpu().numpy(), 1)[0]
+    assert [(s, e) for s, e, _ in got] == [(s, e) for s, e, _ in ref]
+    for (_, _, gm), (_, _, rm) in zip(got, ref):
+        for a, b in zip(gm, rm):
+            assert np.abs(a.data - b).max() < 1e-10
+
+
+@pytest.mark.parametrize("n,seed", [(3, 0), (7, 1), (12, 2), (13, 3)])
+def test_apply_layer_matches_oracle(gpu, n, seed):
+    eng = _engine()
+    sv = O.ket(O.random_uniform_state(n, seed))
+    layer = O.generate_layer(O.bond2_mps(sv, n))
+    g, w = eng.layers_to_arrays([layer], n)
+    dev_g = torch.from_numpy(g).to(gpu)
+    dev_w = torch.from_numpy(w).to(gpu)
+    for inverse in (True, False):
+        st = eng.as_device_states(sv, torch.complex128)
+        eng.apply_layer(st, dev_g, dev_w, inverse)
+        got = st[0].cpu().numpy()
+        if inverse:
+            ref = O.apply_inverse_layer_dense(sv, n, layer)
+        else:
+            ref = sv.copy()
+            for s, e, mats in layer:
+                for j in range(s, e + 1):
+                    ref = O.apply_gate(ref, n, mats[j - s], j, 1 if j == e else 2)
+        err = np.linalg.norm(got - ref)
+        print(f"n={n} inverse={inverse} err={err:.3e}")
+        assert err < 1e-12
+
+
+@pytest.mark.parametrize("n,chi,layers,kind,seed", [
+    (4, 4, 3, "uniform", 0), (6, 8, 5, "uniform", 1), (8, 16, 6, "mps", 2), (10, 32, 4, "uniform", 3)])
+def test_extract_layers_matches_oracle(gpu, n, chi, layers, kind, seed):
+    eng = _engine()
+    sv = _states(kind, n, seed)
+    _, target, _ = eng.chi_truncate(eng.as_device_states(sv), chi)
+    work = (target / torch.linalg.vector_norm(target)).contiguous()
+    gates, widths, used, amp0, _ = eng.extract_layers(work, layers)
+    torch.cuda.synchronize()
+    ref_t, _ = O.chi_truncate(O.ket(sv), n, chi)
+    ref_layers, info = O.get_unitary_layers(ref_t, n, layers, return_info=True)
+    assert int(used[0]) == len(ref_layers)
+    got = _gate_list_from_device(gates[0].cpu().numpy(), widths[0].cpu().numpy(), int(used[0]))
+    ref = O.layer_gate_list(ref_layers)
+    worst = max(np.abs(a[2] - b[2]).max() for a, b in zip(got, ref))
+    print(f"n={n} L={layers} gates max diff {worst:.3e}")
+    assert [(a[0], a[1]) for a in got] == [(b[0], b[1]) for b in ref]
+    assert worst < 1e-8
+    np.testing.assert_allclose(amp0[0, :len(info["amp0"])].cpu().numpy(), info["amp0"], atol=1e-9)
+
+
+def test_extract_layers_early_exit(gpu):
+    """A product state is disentangled by one layer: the loop stops at once
+    (sequential.py:390)."""
+    eng = _engine()
+    n = 6
+    sv = np.zeros(2 ** n, dtype=np.complex128)
+    sv[0b101100] = 1.0
+    work = eng.as_device_states(sv, torch.complex128)
+    gates, widths, used, amp0, _ = eng.extract_layers(work, 5)
+    assert int(used[0]) == 1
+    assert abs(amp0[0, 0].item() - 1) < 1e-12
+
+
+@pytest.mark.parametrize("dtype,n,layers,sweeps,seed", [
+    (torch.complex128, 4, 2, 3, 0), (torch.complex128, 6, 3, 4, 1), (torch.complex128, 9, 3, 2, 2),
+    (torch.complex64, 8, 4, 3, 3), (torch.complex64, 12, 3, 2, 4)])
+def test_sweep_matches_oracle(gpu, dtype, n, layers, sweeps, seed):
+    eng = _engine()
+    sv = O.ket(O.random_uniform_state(n, seed))
+    ref_t, _ = O.chi_truncate(sv, n, 2 ** n)
+    ref_layers = O.get_unitary_layers(ref_t, n, layers)
+    gl0 = O.layer_gate_list(ref_layers)
+    trace = []
+    gl = O.optimize_gate_list(ref_t, n, gl0, sweeps, trace)
+    gl_j = O.optimize_gate_list(ref_t, n, gl0, sweeps, None, polar="jacobi")
+    ref_state = O.circuit_state(n, gl)
+    ref_fid = abs(np.vdot(sv, ref_state))
+    g, w = eng.layers_to_arrays(ref_layers, n)
+    gates = torch.from_numpy(g).to(gpu).unsqueeze(0).contiguous()
+    widths = torch.from_numpy(w).to(gpu).unsqueeze(0).contiguous()
+    used = torch.tensor([g.shape[0]], dtype=torch.int32, device=gpu)
+    target = torch.from_numpy(ref_t).to(gpu).to(dtype).unsqueeze(0).contiguous()
+    psi_ref = torch.from_numpy(sv).to(gpu).unsqueeze(0).contiguous()
+    res = eng.sweep(target, gates, widths, used, sweeps, psi_ref=psi_ref, circuit_out=True)
+    torch.cuda.synchronize()
+    fid = abs(res.fidelity[0, -1].item())
+    circ = res.circuit[0].cpu().numpy().astype(np.complex128)
+    got = _gate_list_from_device(gates[0].cpu().numpy(), widths[0].cpu().numpy(), g.shape[0])
+    # gate by gate against the oracle with the kernel's polar completion rule
+    gdiff = max(np.linalg.norm(a[2] - b[2]) for a, b in zip(got, gl_j))
+    serr = np.linalg.norm(circ - ref_state)
+    print(f"{dtype} n={n} fid {fid:.12f} ref {ref_fid:.12f} gate err {gdiff:.3e} state err {serr:.3e}")
+    for _, _, gm in got:   # every updated gate is unitary
+        d = gm.shape[0]
+        assert np.abs(gm @ gm.conj().T - np.eye(d)).max() < 1e-12
+    np.testing.assert_allclose(res.overlap[0].cpu().numpy(), np.abs(trace), rtol=1e-6 if dtype == torch.complex64 else 1e-10)
+    if dtype == torch.complex128:
+        assert abs(fid - ref_fid) < 1e-10
+        assert gdiff < 1e-8
+        assert serr < 1e-9
+    else:
+        assert abs(fid - ref_fid) < 1e-6
+        assert serr < 1e-5
+
+
+@pytest.mark.parametrize("n,chi,layers,sweeps,seed,sdtype", [
+    (6, 64, 4, 3, 0, torch.complex128), (8, 8, 5, 4, 1, torch.complex64),
+    (10, 512, 5, 3, 2, torch.complex64), (12, 16, 4, 2, 3, torch.complex64)])
+def test_encode_end_to_end(gpu, n, chi, layers, sweeps, seed, sdtype):
+    """Whole prepare_state through qmx_encode against the oracle's."""
+    eng = _engine()
+    sv = O.random_uniform_state(n, seed) if n != 12 else O.random_mps_state(n, 2 * chi, seed)
+    res = eng.encode(sv, chi, layers, sweeps, sweep_dtype=sdtype)
+    torch.cuda.synchronize()
+    ref = O.prepare_state(sv, chi, num_layers=layers, num_sweeps=sweeps)
+    fid = abs(res.fidelity[0, -1].item())
+    gl = _gate_list_from_device(res.gates[0].cpu().numpy(), res.widths[0].cpu().numpy(),
+                                int(res.layers_used[0]))
+    state = O.circuit_state(n, gl)
+    serr = np.linalg.norm(state - ref["state"])
+    print(f"n={n} chi={chi} fid {fid:.10f} ref {ref['fidelity']:.10f} state err {serr:.3e}")
+    assert int(res.layers_used[0]) == len(ref["layers"])
+    assert abs(fid - ref["fidelity"]) < 1e-6
+    assert serr < 1e-5
+    # the reported fidelity is the overlap of the emitted circuit with the input
+    assert abs(abs(np.vdot(O.ket(sv), state)) - fid) < 1e-6
+
+
+def test_encode_batch_equals_single(gpu):
+    """A batch gives the same gates as each state alone (no cross-talk)."""
+    eng = _engine()
+    n, chi = 9, 8
+    svs = np.stack([O.random_mps_state(n, 16, s) for s in range(5)])
+    batch = eng.encode(svs, chi, 3, 2, sweep_dtype=torch.complex128)
+    for b in (0, 3):
+        one = eng.encode(svs[b], chi, 3, 2, sweep_dtype=torch.complex128)
+        assert torch.equal(batch.layers_used[b:b + 1], one.layers_used)
+        assert (batch.gates[b] - one.gates[0]).abs().max().item() < 1e-12
+        assert abs(batch.fidelity[b, -1].item() - one.fidelity[0, -1].item()) < 1e-12
