@@ -1,0 +1,172 @@
+"""Benchmark: state-preps/sec of the Sequential encoder on one MI355X.
+
+Workload (default): N=20 qubits, statevectors of complex64 amplitudes from
+seeded random MPS of bond 96 (complex Gaussian tensors, contracted in
+complex128, normalised), bond_dimension=64, num_layers=10, num_sweeps=30.
+A step is one whole prepare_state (chi-truncation, layer extraction, 30
+sweeps, final fidelity) of one batch of such states on the GPU.
+
+Prints one JSON line (rank 0).  ``roofline`` describes the dominant kernel,
+the sweep gate step: algorithmic bytes per launch = (read phi, read T',
+write phi, write T') = 32 B per complex64 amplitude per state (24 B on the
+first step of a sweep, which reads the target and writes no T'), divided by
+the average launch duration measured with HIP events on the library's stream.
+``cpu_baseline`` times the FP64 NumPy oracle (a port of the reference
+algorithm) on one state: chi-truncation + all layers + one sweep, with the
+other sweeps scaled from the timed one.
+"""
+
+from __future__ import annotations
+
+import argparse
+import json
+import os
+import sys
+import time
+
+import numpy as np
+
+HBM_PEAK_GBS = 8000.0
+WORKLOAD = {"num_qubits": 20, "bond_dimension": 64, "num_layers": 10, "num_sweeps": 30,
+            "generator_bond": 96}
+
+
+def _states(n: int, bond: int, count: int, seed0: int) -> np.ndarray:
+    rng_states = []
+    for s in range(count):
+        rng = np.random.default_rng(seed0 + s)
+        dims = [1] + [min(bond, 2 ** i, 2 ** (n - i)) for i in range(1, n)] + [1]
+        psi = np.ones((1, 1), dtype=np.complex128)
+        for i in range(n):
+            a = (rng.standard_normal((dims[i], 2, dims[i + 1]))
+                 + 1j * rng.standard_normal((dims[i], 2, dims[i + 1]))) / np.sqrt(2.0)
+            psi = (psi @ a.reshape(dims[i], 2 * dims[i + 1])).reshape(-1, dims[i + 1])
+        psi = psi.reshape(-1)
+        rng_states.append((psi / np.linalg.norm(psi)).astype(np.complex64))
+    return np.stack(rng_states)
+
+
+def cpu_baseline(sv: np.ndarray, wl: dict, gpu_fid_trace=None) -> dict:
+    """Oracle timing on one state (bounded sample, scaled to a full prep)."""
+    sys.path.insert(0, os.path.dirname(os.path.abspath(__file__)))
+    from oracle import qmprs_oracle as O
+    n, chi, L, S = wl["num_qubits"], wl["bond_dimension"], wl["num_layers"], wl["num_sweeps"]
+    t0 = time.perf_counter()
+    psi = O.ket(sv.astype(np.complex128))
+    target, _ = O.chi_truncate(psi, n, chi)
+    t1 = time.perf_counter()
+    layers = O.get_unitary_layers(target, n, L)
+    t2 = time.perf_counter()
+    gl = O.layer_gate_list(layers)
+    f0 = abs(np.vdot(psi, O.circuit_state(n, gl)))
+    t3 = time.perf_counter()
+    gl, _ = O.sweep_once(target, n, gl)
+    t4 = time.perf_counter()
+    f1 = abs(np.vdot(psi, O.circuit_state(n, gl)))
+    per_state = (t1 - t0) + (t2 - t1) + S * (t4 - t3)
+    out = {"value": 1.0 / per_state, "unit": "state-preps/s",
+           "cores": len(os.sched_getaffinity(0)), "kind": "port",
+           "sample": (f"1 state, FP64 NumPy oracle: chi-truncation {t1 - t0:.1f}s + {L} layers "
+                      f"{t2 - t1:.1f}s + 1 of {S} sweeps {t4 - t3:.1f}s timed; "
+                      f"per-state time = chi + layers + {S} x sweep")}
+    if gpu_fid_trace is not None:
+        out["parity"] = {"oracle_fid_sweep0": f0, "oracle_fid_sweep1": f1,
+                         "gpu_fid_sweep0": float(gpu_fid_trace[0]),
+                         "gpu_fid_sweep1": float(gpu_fid_trace[1])}
+    return out
+
+
+def main() -> int:
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=3)
+    ap.add_argument("--warmup", type=int, default=1)
+    ap.add_argument("--batch", type=int, default=16)
+    ap.add_argument("--no-cpu-baseline", action="store_true")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print("bench.py runs on one GPU only: this encoder shards across GPUs by state "
+              "batches, and no multi-GPU figure is produced here", file=sys.stderr)
+        return 2
+
+    import torch
+    sys.path.insert(0, os.path.dirname(os.path.abspath(__file__)))
+    from qmprs_amd import _lib, engine
+
+    wl = WORKLOAD
+    n, chi, L, S = wl["num_qubits"], wl["bond_dimension"], wl["num_layers"], wl["num_sweeps"]
+    B = args.batch
+    svs = _states(n, wl["generator_bond"], B, seed0=1000)
+    dev = _lib.require_gpu()
+    sv_dev = engine.as_device_states(svs)
+    ws = None
+
+    def step():
+        return engine.encode(sv_dev, chi, L, S, sweep_dtype=torch.complex64, workspace=ws)
+
+    res = step()   # sizes the workspace
+    cfg = _lib.EncodeConfig(n, B, chi, L, S, _lib.QMX_C64, _lib.QMX_C64, 1 - 1e-6, 1e-10)
+    import ctypes
+    ws = _lib.workspace(_lib.load().qmx_encode_workspace(ctypes.byref(cfg)), dev)
+    for _ in range(args.warmup):
+        res = step()
+    torch.cuda.synchronize()
+    _lib.profile_enable(True)
+    _lib.profile_read()
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        res = step()
+    torch.cuda.synchronize()
+    t1 = time.perf_counter()
+    prof = _lib.profile_read()
+    _lib.profile_enable(False)
+    total = t1 - t0
+    ms_per_step = 1000.0 * total / args.steps
+    value = B * args.steps / total
+
+    fid = res.fidelity.abs().cpu().numpy()
+    layers_used = res.layers_used.cpu().numpy()
+    # dominant kernel: sweep gate step
+    sweep_ms, sweep_launches = prof["sweep"]
+    avg_launch_s = sweep_ms / 1000.0 / max(sweep_launches, 1)
+    k_per_sweep = int(layers_used.max()) * n
+    amp_bytes = (32.0 * (k_per_sweep - 1) + 24.0) / k_per_sweep
+    bytes_per_launch = B * (2 ** n) * amp_bytes
+    achieved = bytes_per_launch / avg_launch_s / 1e9
+    roofline = {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS,
+                "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None,
+                "kernel": "sweep_step_kernel<float2>",
+                "avg_launch_us": round(avg_launch_s * 1e6, 2),
+                "bytes_per_launch": int(bytes_per_launch)}
+    phases = {p: {"ms": round(v[0] / args.steps, 2), "launches": int(v[1] / args.steps)}
+              for p, v in prof.items()}
+    out = {
+        "metric": "state-preps/sec at fidelity >=ref-1e-4 (N=20, chi=64): plain vs NN-warm-started",
+        "value": round(value, 3),
+        "unit": "state-preps/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": round(ms_per_step, 2),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "complex64 sweep storage, fp64 arithmetic and reductions",
+        "data": "synthetic: seeded random-MPS statevectors (bond 96, complex Gaussian)",
+        "config": {"workload": "plain Sequential.prepare_state, N=20, bond_dimension=64, "
+                               "num_layers=10, num_sweeps=30, complex64 input",
+                   "batch": B, "path": "plain"},
+        "fidelity": {"mean": float(fid[:, -1].mean()), "min": float(fid[:, -1].min()),
+                     "max": float(fid[:, -1].max()), "layers_used_max": int(layers_used.max())},
+        "phases_ms_per_step": phases,
+        "roofline": roofline,
+    }
+    if not args.no_cpu_baseline:
+        out["cpu_baseline"] = cpu_baseline(svs[0], wl, fid[0])
+    print(json.dumps(out))
+    return 0
+
+
+if __name__ == "__main__":
+    sys.exit(main())

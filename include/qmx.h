@@ -49,6 +49,28 @@ int qmx_create(int device, qmx_handle** out);
 int qmx_destroy(qmx_handle* h);
 const char* qmx_last_error(const qmx_handle* h);
 
+/* Tuning options (no reference counterpart).
+ * QMX_OPT_SWEEP_MODE: QMX_SWEEP_AUTO (windowed passes for N >= 8, else one
+ * pass per gate), QMX_SWEEP_STEP (one pass per gate) or QMX_SWEEP_WINDOW. */
+#define QMX_OPT_SWEEP_MODE 1
+#define QMX_SWEEP_AUTO 0
+#define QMX_SWEEP_STEP 1
+#define QMX_SWEEP_WINDOW 2
+int qmx_set_option(qmx_handle* h, int option, int value);
+
+/* Phase timers (no reference counterpart: instrumentation for benchmarks).
+ * When enabled, every later call records HIP event pairs on its stream around
+ * each phase; qmx_profile_read synchronises on them and returns, per phase,
+ * the summed milliseconds and the number of kernel launches inside, then
+ * clears the records.  ms and launches hold QMX_PHASE_COUNT entries. */
+#define QMX_PHASE_CHI 0      /* statevector -> chi-MPS chain            */
+#define QMX_PHASE_LAYERS 1   /* layer extraction loop                    */
+#define QMX_PHASE_REBUILD 2  /* circuit state C|0..0> before each sweep  */
+#define QMX_PHASE_SWEEP 3    /* sweep gate steps (sweep_step kernels)    */
+#define QMX_PHASE_COUNT 4
+int qmx_profile_enable(qmx_handle* h, int enable);
+int qmx_profile_read(qmx_handle* h, double* ms, long long* launches);
+
 /* (a)+(b) statevector -> chi-truncated MPS state.
  * Replaces MPS.from_statevector (qmprs/primitives/mps.py:218-249): quick Ket
  * normalisation (base.py:96-97), MatrixProductState.from_dense and

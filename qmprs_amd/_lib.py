@@ -26,6 +26,11 @@ QMX_ERR_BASIS = 6
 QMX_C64 = 0
 QMX_C128 = 1
 
+PHASES = ("chi", "layers", "rebuild", "sweep")
+
+QMX_OPT_SWEEP_MODE = 1
+SWEEP_MODES = {"auto": 0, "step": 1, "window": 2}
+
 _VP = ctypes.c_void_p
 _INT = ctypes.c_int
 _DBL = ctypes.c_double
@@ -53,6 +58,9 @@ SIGNATURES = {
     "qmx_create": (_INT, [_INT, ctypes.POINTER(_VP)]),
     "qmx_destroy": (_INT, [_VP]),
     "qmx_last_error": (ctypes.c_char_p, [_VP]),
+    "qmx_set_option": (_INT, [_VP, _INT, _INT]),
+    "qmx_profile_enable": (_INT, [_VP, _INT]),
+    "qmx_profile_read": (_INT, [_VP, ctypes.POINTER(_DBL), ctypes.POINTER(ctypes.c_longlong)]),
     "qmx_chi_truncate_workspace": (_SZ, [_INT, _INT, _INT]),
     "qmx_chi_truncate": (_INT, [_VP, _INT, _INT, _INT, _DBL, _VP, _INT, _VP, _VP, _VP, _VP, _SZ, _VP]),
     "qmx_layer_workspace": (_SZ, [_INT, _INT]),
@@ -137,3 +145,23 @@ def workspace(nbytes: int, device) -> "torch.Tensor":
     if nbytes == 0:
         raise ValueError("invalid problem size for the qmx workspace")
     return torch.empty(int(nbytes), dtype=torch.uint8, device=device)
+
+
+def profile_enable(enable: bool = True) -> None:
+    h = handle()
+    check(load().qmx_profile_enable(h, 1 if enable else 0), h)
+
+
+def profile_read() -> dict:
+    """{phase: (milliseconds, launches)} since the last read (synchronises)."""
+    h = handle()
+    ms = (ctypes.c_double * len(PHASES))()
+    ln = (ctypes.c_longlong * len(PHASES))()
+    check(load().qmx_profile_read(h, ms, ln), h)
+    return {p: (ms[i], ln[i]) for i, p in enumerate(PHASES)}
+
+
+def set_sweep_mode(mode: str) -> None:
+    """'auto' (windowed passes for N >= 8), 'step' (one pass per gate) or 'window'."""
+    h = handle()
+    check(load().qmx_set_option(h, QMX_OPT_SWEEP_MODE, SWEEP_MODES[mode]), h)

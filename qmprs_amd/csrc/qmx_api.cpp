@@ -15,10 +15,22 @@
 #include "qmx.h"
 #include "qmx_internal.h"
 
+// Phase timers (qmx_profile_*): event pairs recorded on the caller's stream
+// around each phase when profiling is enabled.
+struct ProfRec {
+  int phase;
+  hipEvent_t start, end;
+  long long launches;
+};
+
 struct qmx_handle {
   int device;
   int cus;
   std::string err;
+  bool profiling = false;
+  int sweep_mode = QMX_SWEEP_AUTO;
+  std::vector<ProfRec> recs;
+  std::vector<hipEvent_t> pool;
 };
 
 namespace {
@@ -67,6 +79,39 @@ int hip_fail(qmx_handle* h, hipError_t e, const char* what) {
   } while (0)
 
 inline size_t state_len(int n) { return static_cast<size_t>(1) << n; }
+
+hipEvent_t pool_event(qmx_handle* h) {
+  if (!h->pool.empty()) {
+    hipEvent_t e = h->pool.back();
+    h->pool.pop_back();
+    return e;
+  }
+  hipEvent_t e = nullptr;
+  if (hipEventCreate(&e) != hipSuccess) return nullptr;
+  return e;
+}
+
+// Scoped phase timer: records a start event now and an end event on stop().
+struct PhaseTimer {
+  qmx_handle* h;
+  hipStream_t st;
+  int phase;
+  hipEvent_t start = nullptr;
+  PhaseTimer(qmx_handle* h_, hipStream_t st_, int phase_) : h(h_), st(st_), phase(phase_) {
+    if (h->profiling) {
+      start = pool_event(h);
+      if (start) hipEventRecord(start, st);
+    }
+  }
+  void stop(long long launches) {
+    if (!start) return;
+    hipEvent_t end = pool_event(h);
+    if (!end) return;
+    hipEventRecord(end, st);
+    h->recs.push_back({phase, start, end, launches});
+    start = nullptr;
+  }
+};
 
 int blocks_for(long ntiles, int batch, int cus) {
   long target = 4L * cus;
@@ -187,6 +232,34 @@ struct LayerLayout {
   int* error;
 };
 constexpr int kChain2MaxBlocks = 256;
+constexpr int kWinMaxBlocks = 256;
+
+struct Window {
+  int s0, s1;
+  uint64_t umask;
+};
+
+// Sweep windows: maximal runs of consecutive steps (reverse circuit order)
+// whose worst-case qubits fit in 4 bits; each bit set is padded to exactly 4
+// bits with the lowest free bits.
+std::vector<Window> plan_windows(int n, int kmax) {
+  std::vector<Window> out;
+  int s = 0;
+  while (s < kmax) {
+    Window w{s, s, 0};
+    while (w.s1 < kmax && w.s1 - w.s0 < kMaxWindow) {
+      int site = n - 1 - w.s1 % n;
+      uint64_t nb = w.umask | worst_bits(n, site);
+      if (__builtin_popcountll(nb) > 4) break;
+      w.umask = nb;
+      ++w.s1;
+    }
+    for (int bit = 0; bit < n && __builtin_popcountll(w.umask) < 4; ++bit) w.umask |= 1ull << bit;
+    out.push_back(w);
+    s = w.s1;
+  }
+  return out;
+}
 
 LayerLayout layer_layout(Carver& c, int n, int batch) {
   LayerLayout L{};
@@ -226,7 +299,7 @@ SweepLayout<S> sweep_layout(Carver& c, int n, int batch) {
   long ntiles = 1l << (n - std::min(n, kSweepTileBits));
   long ntiles_apply = 1l << (n - std::min(n, kApplyTileBits));
   long mx = std::max(ntiles, ntiles_apply);
-  L.part = c.take<double>(static_cast<size_t>(batch) * mx * 32);
+  L.part = c.take<double>(static_cast<size_t>(batch) * std::max<long>(mx * 32, kWinMaxBlocks * 512));
   L.counter = c.take<int>(batch);
   L.dot_part = c.take<double>(static_cast<size_t>(batch) * mx * 2);
   L.dot_counter = c.take<int>(batch);
@@ -250,10 +323,12 @@ int run_chi(qmx_handle* h, int n, int batch, int chi, double cutoff, const void*
     launch_scale_convert(static_cast<const double2*>(sv), psi, n, batch, L.nrm, st);
   }
   QMX_LAUNCHED(h, "normalise");
+  PhaseTimer timer(h, st, QMX_PHASE_CHI);
   const int i_first = L.i_first;
   if (i_first == 0) {
     launch_scale_convert(static_cast<const double2*>(psi), target, n, batch, nullptr, st);
     QMX_LAUNCHED(h, "copy");
+    timer.stop(1);
     return QMX_OK;
   }
   const int nmax = L.nmax;
@@ -287,6 +362,7 @@ int run_chi(qmx_handle* h, int n, int batch, int chi, double cutoff, const void*
     QMX_LAUNCHED(h, "chi contraction step");
     z = out;
   }
+  timer.stop(7L * i_first);
   return QMX_OK;
 }
 
@@ -393,6 +469,7 @@ int run_extract(qmx_handle* h, int n, int batch, int lmax, double thr, double cu
   launch_fill_int(layers_used, 0, batch, st);
   launch_fill_int(widths, 1, batch * lmax * n, st);
   auto inv = inverse_layer_passes(n);
+  PhaseTimer timer(h, st, QMX_PHASE_LAYERS);
   for (int l = 0; l < lmax; ++l) {
     QMX_CALL(run_generate_layer(h, n, batch, lmax, l, cutoff, psi_w, gates, widths, bonds, L,
                                 L.skip, st));
@@ -411,6 +488,7 @@ int run_extract(qmx_handle* h, int n, int batch, int lmax, double thr, double cu
     launch_amp0(a, st);
     QMX_LAUNCHED(h, "early exit check");
   }
+  timer.stop(static_cast<long long>(lmax) * (n + 1 + inv.size()));
   return device_error_status(h, L.error, st);
 }
 
@@ -430,11 +508,18 @@ int run_sweep(qmx_handle* h, int n, int batch, int lmax, const int* layers_used,
   auto rebuild = forward_circuit_passes(n, lmax);
   const int ts = std::min(n, kSweepTileBits);
   const int nblk = blocks_for(1l << (n - ts), batch, h->cus);
+  const bool windowed = (h->sweep_mode == QMX_SWEEP_WINDOW ||
+                         (h->sweep_mode == QMX_SWEEP_AUTO)) && n >= kWinMinQubits;
+  const auto wins = plan_windows(n, kmax);
+  const int wblk = std::min(kWinMaxBlocks,
+                            blocks_for(1l << (n - std::min(n, kWinTileBits)), batch, h->cus));
   for (int s = 0; s < nsweeps; ++s) {
+    PhaseTimer rt(h, st, QMX_PHASE_REBUILD);
     launch_basis0(L.phi, n, batch, skip, st);
     QMX_CALL(run_passes<S>(h, n, batch, lmax, 1, 0, rebuild, L.phi, gates, widths, layers_used,
                            skip, fidelity ? psi_ref : nullptr, L.dot_part, L.dot_counter,
                            fidelity ? fidelity + s : nullptr, nsweeps + 1, h->cus, st));
+    rt.stop(static_cast<long long>(rebuild.size()));
     if (use_goal) {
       launch_fill_int(L.active, 0, 1, st);
       launch_goal_check(fidelity + s, nsweeps + 1, goal, L.skip, L.active, sweeps_done, batch, st);
@@ -442,6 +527,48 @@ int run_sweep(qmx_handle* h, int n, int batch, int lmax, const int* layers_used,
       QMX_HIP(h, hipMemcpyAsync(&active, L.active, sizeof(int), hipMemcpyDeviceToHost, st));
       QMX_HIP(h, hipStreamSynchronize(st));
       if (active == 0) break;
+    }
+    PhaseTimer swt(h, st, QMX_PHASE_SWEEP);
+    if (windowed) {
+      for (size_t wi = 0; wi < wins.size(); ++wi) {
+        const Window& w = wins[wi];
+        WinPassArgs<S> pa{};
+        pa.n = n;
+        pa.lmax = lmax;
+        pa.tbits = kWinTileBits;
+        pa.s0 = w.s0;
+        pa.s1 = w.s1;
+        pa.p0 = wi ? wins[wi - 1].s0 : 0;
+        pa.p1 = wi ? wins[wi - 1].s1 : 0;
+        pa.umask = w.umask;
+        pa.target = target;
+        pa.tprime = L.tprime;
+        pa.phi = L.phi;
+        pa.gates = gates;
+        pa.widths = widths;
+        pa.layers = layers_used;
+        pa.skip = skip;
+        pa.part = L.part;
+        launch_window_pass(pa, wblk, batch, st);
+        WinFinArgs fa{};
+        fa.n = n;
+        fa.lmax = lmax;
+        fa.nblk = wblk;
+        fa.s0 = w.s0;
+        fa.s1 = w.s1;
+        fa.umask = w.umask;
+        fa.gates = gates;
+        fa.widths = widths;
+        fa.layers = layers_used;
+        fa.skip = skip;
+        fa.part = L.part;
+        fa.overlap = overlap ? overlap + s : nullptr;
+        fa.overlap_stride = nsweeps;
+        launch_window_finalize(fa, batch, st);
+      }
+      swt.stop(static_cast<long long>(wins.size()));
+      QMX_LAUNCHED(h, "windowed sweep");
+      continue;
     }
     for (int k = 0; k < kmax; ++k) {
       SweepArgs<S> a{};
@@ -462,6 +589,7 @@ int run_sweep(qmx_handle* h, int n, int batch, int lmax, const int* layers_used,
       a.overlap_stride = nsweeps;
       launch_sweep_step(a, nblk, batch, st);
     }
+    swt.stop(kmax);
     QMX_LAUNCHED(h, "sweep");
   }
   // final circuit state and fidelity
@@ -498,11 +626,48 @@ int qmx_create(int device, qmx_handle** out) {
 }
 
 int qmx_destroy(qmx_handle* h) {
+  if (!h) return QMX_OK;
+  for (auto& r : h->recs) { hipEventDestroy(r.start); hipEventDestroy(r.end); }
+  for (auto e : h->pool) hipEventDestroy(e);
   delete h;
   return QMX_OK;
 }
 
 const char* qmx_last_error(const qmx_handle* h) { return h ? h->err.c_str() : "null handle"; }
+
+int qmx_set_option(qmx_handle* h, int option, int value) {
+  if (!h) return QMX_ERR_INVALID;
+  if (option == QMX_OPT_SWEEP_MODE) {
+    if (value < QMX_SWEEP_AUTO || value > QMX_SWEEP_WINDOW) return fail(h, QMX_ERR_INVALID, "sweep mode");
+    h->sweep_mode = value;
+    return QMX_OK;
+  }
+  return fail(h, QMX_ERR_INVALID, "unknown option");
+}
+
+int qmx_profile_enable(qmx_handle* h, int enable) {
+  if (!h) return QMX_ERR_INVALID;
+  h->profiling = enable != 0;
+  return QMX_OK;
+}
+
+int qmx_profile_read(qmx_handle* h, double* ms, long long* launches) {
+  if (!h || !ms || !launches) return QMX_ERR_INVALID;
+  for (int p = 0; p < QMX_PHASE_COUNT; ++p) { ms[p] = 0.0; launches[p] = 0; }
+  for (auto& r : h->recs) {
+    QMX_HIP(h, hipEventSynchronize(r.end));
+    float t = 0.0f;
+    QMX_HIP(h, hipEventElapsedTime(&t, r.start, r.end));
+    if (r.phase >= 0 && r.phase < QMX_PHASE_COUNT) {
+      ms[r.phase] += t;
+      launches[r.phase] += r.launches;
+    }
+    h->pool.push_back(r.start);
+    h->pool.push_back(r.end);
+  }
+  h->recs.clear();
+  return QMX_OK;
+}
 
 size_t qmx_chi_truncate_workspace(int num_qubits, int batch, int bond_dimension) {
   if (num_qubits < 2 || num_qubits > kMaxQubits || batch < 1 || bond_dimension < 1) return 0;

@@ -16,6 +16,10 @@ constexpr int kApplyMaxUnion = 7;    // gate bits per pass (>= 4 coalescing bits
 constexpr double kGsTol = 1e-7;      // canonical-gauge Gram-Schmidt threshold
 constexpr double kPhaseTol = 1e-12;  // Ran two-site phase fix skipped below this
 
+constexpr int kWinTileBits = 11;     // window pass tile: 2^11 amplitudes
+constexpr int kMaxWindow = 8;        // gates per window
+constexpr int kWinMinQubits = 8;     // windowed sweep needs >= 2^4 rest indices per wave
+
 // error bits accumulated on the device
 constexpr int kErrNotUnitary = 1;
 constexpr int kErrBasis = 2;
@@ -53,6 +57,37 @@ struct SweepArgs {
   double* part;
   int* counter;
   double* overlap;     // overlap[b * overlap_stride], may be nullptr
+  int overlap_stride;
+};
+
+// Windowed sweep (qmx_window.hip): window = sweep steps [s0, s1) on the
+// 4-bit set umask; the previous window [p0, p1) (p1 == s0, empty for the
+// first) has its updated gates applied to T' in the same pass.
+template <typename S>
+struct WinPassArgs {
+  int n, lmax, tbits;
+  int s0, s1, p0, p1;
+  uint64_t umask;
+  const S* target;
+  S* tprime;
+  S* phi;
+  const double2* gates;
+  const int* widths;
+  const int* layers;
+  const int* skip;
+  double* part;        // [B][nblk][512]
+};
+
+struct WinFinArgs {
+  int n, lmax, nblk;
+  int s0, s1;
+  uint64_t umask;
+  double2* gates;
+  const int* widths;
+  const int* layers;
+  const int* skip;
+  const double* part;
+  double* overlap;
   int overlap_stride;
 };
 
@@ -108,6 +143,8 @@ void launch_scale_convert(const S* x, D* y, int n, int batch, const double* norm
 void launch_fill_int(int* p, int v, int count, hipStream_t);
 template <typename S> void launch_basis0(S* psi, int n, int batch, const int* skip, hipStream_t);
 void launch_chain2_step(const Chain2Args&, int nblk, int batch, hipStream_t);
+template <typename S> void launch_window_pass(const WinPassArgs<S>&, int nblk, int batch, hipStream_t);
+void launch_window_finalize(const WinFinArgs&, int batch, hipStream_t);
 void launch_ran_gates(const RanArgs&, int batch, hipStream_t);
 void launch_amp0(const Amp0Args&, hipStream_t);
 

@@ -161,6 +161,36 @@ QMX_DEV bool publish_partial(const double* red, double* part, int* counter, int 
   return last;
 }
 
+// Sum the NV partial values of nblk blocks (pb[q * NV + j]) with the whole
+// block, in a fixed order: lane t adds value j = t % NV of blocks
+// q = t / NV, t / NV + G, ... (G = blockDim / NV groups), then the G group
+// sums are added in order.  tot[j] receives the totals (all threads).
+template <int NV>
+QMX_DEV void reduce_partials(const double* pb, int nblk, double* scratch, double* tot) {
+  const int groups = blockDim.x / NV;
+  const int j = threadIdx.x % NV, g0 = threadIdx.x / NV;
+  double acc = 0.0;
+  if (g0 < groups) {
+    int q = g0;
+    for (; q + 3 * groups < nblk; q += 4 * groups) {
+      double a0 = pb[(size_t)q * NV + j];
+      double a1 = pb[(size_t)(q + groups) * NV + j];
+      double a2 = pb[(size_t)(q + 2 * groups) * NV + j];
+      double a3 = pb[(size_t)(q + 3 * groups) * NV + j];
+      acc += ((a0 + a1) + (a2 + a3));
+    }
+    for (; q < nblk; q += groups) acc += pb[(size_t)q * NV + j];
+  }
+  scratch[threadIdx.x] = acc;
+  __syncthreads();
+  if (threadIdx.x < NV) {
+    double s = 0.0;
+    for (int g = 0; g < groups; ++g) s += scratch[g * NV + threadIdx.x];
+    tot[threadIdx.x] = s;
+  }
+  __syncthreads();
+}
+
 // ---------------------------------------------------------------- fill ------
 __global__ void fill_int_kernel(int* p, int v, int count) {
   int i = blockIdx.x * blockDim.x + threadIdx.x;
@@ -186,6 +216,8 @@ __global__ __launch_bounds__(256) void norm2_kernel(const S* __restrict__ x, int
                                                     double* part, int* counter,
                                                     double* out) {
   __shared__ double red[4 * 1];
+  __shared__ double scratch[256];
+  __shared__ double tot[1];
   __shared__ int s_flag;
   const int b = blockIdx.y;
   const size_t len = (size_t)1 << n;
@@ -197,10 +229,9 @@ __global__ __launch_bounds__(256) void norm2_kernel(const S* __restrict__ x, int
   block_sum<1>(v, red);
   double* pb = part + (size_t)b * gridDim.x;
   if (publish_partial<1>(red, pb, counter + b, gridDim.x, &s_flag)) {
+    reduce_partials<1>(pb, gridDim.x, scratch, tot);
     if (threadIdx.x == 0) {
-      double s = 0.0;
-      for (int k = 0; k < (int)gridDim.x; ++k) s += pb[k];
-      out[b] = s;
+      out[b] = tot[0];
       counter[b] = 0;
     }
   }
@@ -233,6 +264,8 @@ __global__ __launch_bounds__(256) void apply_pass_kernel(PassArgs<S> a) {
   __shared__ double2 gm[kMaxPassGates][16];
   __shared__ int gw[kMaxPassGates], gm1[kMaxPassGates], gm0[kMaxPassGates], gcount;
   __shared__ double red[4 * 2];
+  __shared__ double scratch[256];
+  __shared__ double tot[2];
   __shared__ int s_flag;
   const int b = blockIdx.y;
   const int n = a.n;
@@ -304,10 +337,9 @@ __global__ __launch_bounds__(256) void apply_pass_kernel(PassArgs<S> a) {
     block_sum<2>(acc, red);
     double* pb = a.dot_part + (size_t)b * gridDim.x * 2;
     if (publish_partial<2>(red, pb, a.dot_counter + b, gridDim.x, &s_flag)) {
+      reduce_partials<2>(pb, gridDim.x, scratch, tot);
       if (threadIdx.x == 0) {
-        double sr = 0.0, si = 0.0;
-        for (int k = 0; k < (int)gridDim.x; ++k) { sr += pb[2 * k]; si += pb[2 * k + 1]; }
-        a.dot_out[(size_t)b * a.dot_stride] = make_double2(sr, si);
+        a.dot_out[(size_t)b * a.dot_stride] = make_double2(tot[0], tot[1]);
         a.dot_counter[b] = 0;
       }
     }
@@ -331,6 +363,7 @@ __global__ __launch_bounds__(256) void sweep_step_kernel(SweepArgs<S> a) {
   __shared__ int s_info[8];
   __shared__ double red[4 * 32];
   __shared__ double tot[32];
+  __shared__ double scratch[256];
   __shared__ int s_flag;
   const int b = blockIdx.y;
   const int n = a.n;
@@ -443,12 +476,7 @@ __global__ __launch_bounds__(256) void sweep_step_kernel(SweepArgs<S> a) {
   block_sum<32>(e, red);
   double* pb = a.part + (size_t)b * gridDim.x * 32;
   if (publish_partial<32>(red, pb, a.counter + b, gridDim.x, &s_flag)) {
-    if (threadIdx.x < 32) {
-      double s = 0.0;
-      for (int q = 0; q < (int)gridDim.x; ++q) s += pb[(size_t)q * 32 + threadIdx.x];
-      tot[threadIdx.x] = s;
-    }
-    __syncthreads();
+    reduce_partials<32>(pb, gridDim.x, scratch, tot);
     if (threadIdx.x == 0) {
       double2 em[16], u[16];
       for (int j = 0; j < 16; ++j) em[j] = make_double2(tot[2 * j], tot[2 * j + 1]);
@@ -475,6 +503,7 @@ __global__ __launch_bounds__(256) void sweep_step_kernel(SweepArgs<S> a) {
 __global__ __launch_bounds__(256) void chain2_step_kernel(Chain2Args a) {
   __shared__ double red[4 * 32];
   __shared__ double tot[32];
+  __shared__ double scratch[256];
   __shared__ int s_flag;
   const int b = blockIdx.y;
   if (a.skip && a.skip[b]) return;
@@ -537,12 +566,7 @@ __global__ __launch_bounds__(256) void chain2_step_kernel(Chain2Args a) {
   block_sum<32>(e, red);
   double* pb = a.part + (size_t)b * gridDim.x * 32;
   if (publish_partial<32>(red, pb, a.counter + b, gridDim.x, &s_flag)) {
-    if (threadIdx.x < 32) {
-      double s = 0.0;
-      for (int q = 0; q < (int)gridDim.x; ++q) s += pb[(size_t)q * 32 + threadIdx.x];
-      tot[threadIdx.x] = s;
-    }
-    __syncthreads();
+    reduce_partials<32>(pb, gridDim.x, scratch, tot);
     if (threadIdx.x == 0) {
       double2 rho[16], v[16], w[16];
       double lam[4];
