@@ -541,6 +541,7 @@ int run_sweep(qmx_handle* h, int n, int batch, int lmax, const int* layers_used,
         pa.p0 = wi ? wins[wi - 1].s0 : 0;
         pa.p1 = wi ? wins[wi - 1].s1 : 0;
         pa.umask = w.umask;
+        pa.pmask = wi ? wins[wi - 1].umask : w.umask;
         pa.target = target;
         pa.tprime = L.tprime;
         pa.phi = L.phi;

@@ -58,68 +58,73 @@ QMX_DEV double wave_sum(double v) {
 }
 
 // ------------------------------------------------- 4x4 Hermitian Jacobi ----
-// Cyclic two-sided Jacobi on an n x n (n <= 4) Hermitian matrix a (row-major,
-// stride 4).  On return lam[] holds eigenvalues sorted in decreasing order and
-// the columns of v (stride 4) the matching orthonormal eigenvectors.
-QMX_DEV void herm_eig4(int n, double2* a, double2* v, double* lam) {
-  for (int i = 0; i < 4; ++i)
-    for (int j = 0; j < 4; ++j) v[i * 4 + j] = make_double2(i == j ? 1.0 : 0.0, 0.0);
+// Cyclic two-sided Jacobi on a 4 x 4 Hermitian matrix a (row-major; pad a
+// smaller matrix with zeros).  On return lam[] holds the eigenvalues in
+// decreasing order and the columns of v the matching orthonormal eigenvectors.
+// All indices are compile-time so the arrays stay in registers.
+QMX_DEV void herm_eig4(double2* a, double2* v, double* lam) {
+#pragma unroll
+  for (int i = 0; i < 16; ++i) v[i] = make_double2((i % 5) == 0 ? 1.0 : 0.0, 0.0);
   for (int sweep = 0; sweep < 40; ++sweep) {
     double off = 0.0, diag = 0.0;
-    for (int i = 0; i < n; ++i) {
+#pragma unroll
+    for (int i = 0; i < 4; ++i) {
       diag += a[i * 4 + i].x * a[i * 4 + i].x;
-      for (int j = i + 1; j < n; ++j) off += cabs2(a[i * 4 + j]);
+#pragma unroll
+      for (int j = i + 1; j < 4; ++j) off += cabs2(a[i * 4 + j]);
     }
     if (off <= 1e-32 * diag || off == 0.0) break;
-    for (int p = 0; p < n - 1; ++p) {
-      for (int q = p + 1; q < n; ++q) {
+#pragma unroll
+    for (int p = 0; p < 3; ++p) {
+#pragma unroll
+      for (int q = p + 1; q < 4; ++q) {
         double2 g = a[p * 4 + q];
         double ag = cabs_(g);
-        if (ag == 0.0) continue;
-        double al = a[p * 4 + p].x, be = a[q * 4 + q].x;
-        double2 ph = make_double2(g.x / ag, g.y / ag);  // e^{i phi}
-        double tau = (be - al) / (2.0 * ag);
-        double t = (tau >= 0 ? 1.0 : -1.0) / (fabs(tau) + sqrt(1.0 + tau * tau));
-        double c = 1.0 / sqrt(1.0 + t * t), s = t * c;
-        // J = [[c, s], [-s e^{-i phi}, c e^{-i phi}]] acting on columns (p, q)
-        double2 em = cconj(ph);
-        double2 j00 = make_double2(c, 0), j01 = make_double2(s, 0);
-        double2 j10 = cscale(em, -s), j11 = cscale(em, c);
-        // A <- A J (columns)
-        for (int i = 0; i < n; ++i) {
-          double2 x = a[i * 4 + p], y = a[i * 4 + q];
-          a[i * 4 + p] = cadd(cmul(x, j00), cmul(y, j10));
-          a[i * 4 + q] = cadd(cmul(x, j01), cmul(y, j11));
-          double2 vx = v[i * 4 + p], vy = v[i * 4 + q];
-          v[i * 4 + p] = cadd(cmul(vx, j00), cmul(vy, j10));
-          v[i * 4 + q] = cadd(cmul(vx, j01), cmul(vy, j11));
+        if (ag != 0.0) {
+          double al = a[p * 4 + p].x, be = a[q * 4 + q].x;
+          double2 ph = make_double2(g.x / ag, g.y / ag);  // e^{i phi}
+          double tau = (be - al) / (2.0 * ag);
+          double t = (tau >= 0 ? 1.0 : -1.0) / (fabs(tau) + sqrt(1.0 + tau * tau));
+          double c = 1.0 / sqrt(1.0 + t * t), sn = t * c;
+          // J = [[c, s], [-s e^{-i phi}, c e^{-i phi}]] acting on columns (p, q)
+          double2 em = cconj(ph);
+          double2 j00 = make_double2(c, 0), j01 = make_double2(sn, 0);
+          double2 j10 = cscale(em, -sn), j11 = cscale(em, c);
+#pragma unroll
+          for (int i = 0; i < 4; ++i) {
+            double2 x = a[i * 4 + p], y = a[i * 4 + q];
+            a[i * 4 + p] = cadd(cmul(x, j00), cmul(y, j10));
+            a[i * 4 + q] = cadd(cmul(x, j01), cmul(y, j11));
+            double2 vx = v[i * 4 + p], vy = v[i * 4 + q];
+            v[i * 4 + p] = cadd(cmul(vx, j00), cmul(vy, j10));
+            v[i * 4 + q] = cadd(cmul(vx, j01), cmul(vy, j11));
+          }
+#pragma unroll
+          for (int j = 0; j < 4; ++j) {
+            double2 x = a[p * 4 + j], y = a[q * 4 + j];
+            a[p * 4 + j] = cadd(cmulc(j00, x), cmulc(j10, y));
+            a[q * 4 + j] = cadd(cmulc(j01, x), cmulc(j11, y));
+          }
+          a[p * 4 + q] = cz();
+          a[q * 4 + p] = cz();
+          a[p * 4 + p].y = 0.0;
+          a[q * 4 + q].y = 0.0;
         }
-        // A <- J^H A (rows)
-        for (int j = 0; j < n; ++j) {
-          double2 x = a[p * 4 + j], y = a[q * 4 + j];
-          a[p * 4 + j] = cadd(cmulc(j00, x), cmulc(j10, y));
-          a[q * 4 + j] = cadd(cmulc(j01, x), cmulc(j11, y));
-        }
-        a[p * 4 + q] = cz();
-        a[q * 4 + p] = cz();
-        a[p * 4 + p].y = 0.0;
-        a[q * 4 + q].y = 0.0;
       }
     }
   }
-  for (int i = 0; i < n; ++i) lam[i] = a[i * 4 + i].x;
-  // selection sort, decreasing, stable on ties
-  for (int i = 0; i < n; ++i) {
-    int m = i;
-    for (int j = i + 1; j < n; ++j)
-      if (lam[j] > lam[m]) m = j;
-    if (m != i) {
-      double tl = lam[i]; lam[i] = lam[m]; lam[m] = tl;
-      for (int r = 0; r < 4; ++r) {
-        double2 tv = v[r * 4 + i]; v[r * 4 + i] = v[r * 4 + m]; v[r * 4 + m] = tv;
-      }
-    }
+#pragma unroll
+  for (int i = 0; i < 4; ++i) lam[i] = a[i * 4 + i].x;
+  // decreasing order by a fixed compare-exchange network (stable on ties)
+#define QMX_CX(i, j)                                                        \
+  if (lam[j] > lam[i]) {                                                    \
+    double tl = lam[i]; lam[i] = lam[j]; lam[j] = tl;                       \
+    _Pragma("unroll") for (int r = 0; r < 4; ++r) {                         \
+      double2 tv = v[r * 4 + i]; v[r * 4 + i] = v[r * 4 + j]; v[r * 4 + j] = tv; \
+    }                                                                       \
   }
+  QMX_CX(0, 1) QMX_CX(2, 3) QMX_CX(0, 2) QMX_CX(1, 3) QMX_CX(1, 2)
+#undef QMX_CX
 }
 
 // rsum2 cutoff + max_bond keep count (decreasing eigenvalues lam[0..n-1]).
@@ -138,131 +143,203 @@ QMX_DEV int keep_count(const double* lam, int n, int max_bond, double cutoff) {
   return k < 1 ? 1 : k;
 }
 
-// Canonical orthonormal basis of span(v[:, 0:k]) (columns, stride 4):
-// two-pass Gram-Schmidt over the columns of the projector P = V V^H in index
-// order, skipping residuals with norm <= tol.  Writes w (n x k, stride 4).
-// Returns the number of basis vectors found (== k unless degenerate input).
-QMX_DEV int canonical_basis4(int n, int k, const double2* v, double2* w, double tol) {
+// Canonical orthonormal basis of span(v[:, 0:k]) (columns of a 4 x 4 v;
+// pad smaller problems with zero rows): two-pass Gram-Schmidt over the
+// columns of the projector P = V V^H in index order, skipping residuals with
+// norm <= tol.  Writes w (columns 0..found-1).  Returns found (== k unless
+// the input is degenerate).  Compile-time indices only.
+QMX_DEV int canonical_basis4(int k, const double2* v, double2* w, double tol) {
   double2 p[16];
-  for (int i = 0; i < n; ++i)
-    for (int j = 0; j < n; ++j) {
+#pragma unroll
+  for (int i = 0; i < 4; ++i)
+#pragma unroll
+    for (int j = 0; j < 4; ++j) {
       double2 s = cz();
-      for (int m = 0; m < k; ++m) s = cfma(v[i * 4 + m], cconj(v[j * 4 + m]), s);
+#pragma unroll
+      for (int m = 0; m < 4; ++m)
+        if (m < k) s = cfma(v[i * 4 + m], cconj(v[j * 4 + m]), s);
       p[i * 4 + j] = s;
     }
+#pragma unroll
+  for (int i = 0; i < 16; ++i) w[i] = cz();
   int found = 0;
-  for (int j = 0; j < n && found < k; ++j) {
-    double2 x[4];
-    for (int i = 0; i < n; ++i) x[i] = p[i * 4 + j];
-    for (int pass = 0; pass < 2; ++pass) {
-      for (int b = 0; b < found; ++b) {
-        double2 d = cz();
-        for (int i = 0; i < n; ++i) d = cfmac(w[i * 4 + b], x[i], d);
-        for (int i = 0; i < n; ++i) x[i] = csub(x[i], cmul(w[i * 4 + b], d));
+#pragma unroll
+  for (int j = 0; j < 4; ++j) {
+    if (found < k) {
+      double2 x[4];
+#pragma unroll
+      for (int i = 0; i < 4; ++i) x[i] = p[i * 4 + j];
+#pragma unroll
+      for (int pass = 0; pass < 2; ++pass) {
+#pragma unroll
+        for (int b = 0; b < 4; ++b) {
+          if (b < found) {
+            double2 d = cz();
+#pragma unroll
+            for (int i = 0; i < 4; ++i) d = cfmac(w[i * 4 + b], x[i], d);
+#pragma unroll
+            for (int i = 0; i < 4; ++i) x[i] = csub(x[i], cmul(w[i * 4 + b], d));
+          }
+        }
       }
-    }
-    double nrm = 0.0;
-    for (int i = 0; i < n; ++i) nrm += cabs2(x[i]);
-    nrm = sqrt(nrm);
-    if (nrm > tol) {
-      for (int i = 0; i < n; ++i) w[i * 4 + found] = cscale(x[i], 1.0 / nrm);
-      ++found;
+      double nrm = 0.0;
+#pragma unroll
+      for (int i = 0; i < 4; ++i) nrm += cabs2(x[i]);
+      nrm = sqrt(nrm);
+      if (nrm > tol) {
+#pragma unroll
+        for (int c = 0; c < 4; ++c)
+          if (c == found) {
+#pragma unroll
+            for (int i = 0; i < 4; ++i) w[i * 4 + c] = cscale(x[i], 1.0 / nrm);
+          }
+        ++found;
+      }
     }
   }
   return found;
 }
 
 // ------------------------------------------------------- polar (d <= 4) ----
-// Unitary polar factor U V^H of the d x d matrix e (row-major, stride 4) via a
-// one-sided (Hestenes) Jacobi SVD.  Singular pairs below 1e-13 of the largest
-// are replaced by a canonical completion (see below).  Returns the sum of
-// singular values (the nuclear norm) in *nuc.
+// Unitary polar factor U V^H of the d x d (d = 2 or 4) matrix e (row-major,
+// stride 4) by a one-sided (Hestenes) Jacobi SVD on a 4 x 4 working matrix (a
+// 2 x 2 problem is embedded as diag(E, |E| I), whose padded columns are
+// orthogonal to E's and never rotate).  Rotations: cyclic pairs p < q, when
+// |a_p^H a_q| > 1e-16 |a_p| |a_q|.  Pairs (u_j, v_j) with sigma_j >
+// 1e-13 sigma_max are kept; each other column j (in increasing j) receives
+// the next unit vector e_0, e_1, ... whose two-pass Gram-Schmidt residual
+// against the columns already set has norm > 1/2, on both the U and V sides.
+// The result then depends only on the well-determined singular subspaces.
+// *nuc = sum of E's singular values.  Compile-time indices only.
 QMX_DEV void polar4(int d, const double2* e, double2* out, double* nuc) {
   double2 a[16], v[16];
-  for (int i = 0; i < 16; ++i) { a[i] = e[i]; v[i] = cz(); }
-  for (int i = 0; i < 4; ++i) v[i * 4 + i] = make_double2(1.0, 0.0);
+  double fro = 0.0;
+#pragma unroll
+  for (int i = 0; i < 16; ++i) {
+    a[i] = e[i];
+    fro += cabs2(e[i]);
+    v[i] = make_double2((i % 5) == 0 ? 1.0 : 0.0, 0.0);
+  }
+  if (d == 2) {
+    double s = sqrt(fro);
+    if (s == 0.0) s = 1.0;
+#pragma unroll
+    for (int i = 0; i < 16; ++i) {
+      int r = i / 4, c = i % 4;
+      if (r >= 2 || c >= 2) a[i] = make_double2((r == c) ? s : 0.0, 0.0);
+    }
+  }
   for (int sweep = 0; sweep < 60; ++sweep) {
     bool rotated = false;
-    for (int p = 0; p < d - 1; ++p)
-      for (int q = p + 1; q < d; ++q) {
+#pragma unroll
+    for (int p = 0; p < 3; ++p)
+#pragma unroll
+      for (int q = p + 1; q < 4; ++q) {
         double al = 0.0, be = 0.0;
         double2 g = cz();
-        for (int i = 0; i < d; ++i) {
+#pragma unroll
+        for (int i = 0; i < 4; ++i) {
           al += cabs2(a[i * 4 + p]);
           be += cabs2(a[i * 4 + q]);
           g = cfmac(a[i * 4 + p], a[i * 4 + q], g);  // a_p^H a_q
         }
         double ag = cabs_(g);
-        if (ag <= 1e-16 * sqrt(al * be) || ag == 0.0) continue;
-        rotated = true;
-        double2 ph = make_double2(g.x / ag, g.y / ag);
-        double zeta = (be - al) / (2.0 * ag);
-        double t = (zeta >= 0 ? 1.0 : -1.0) / (fabs(zeta) + sqrt(1.0 + zeta * zeta));
-        double c = 1.0 / sqrt(1.0 + t * t), s = t * c;
-        // [a_p, a_q] <- [a_p, a_q] [[c, s e^{i phi}], [-s e^{-i phi}, c]]
-        double2 sp = cscale(ph, s), sm = cscale(cconj(ph), -s);
-        for (int i = 0; i < d; ++i) {
-          double2 x = a[i * 4 + p], y = a[i * 4 + q];
-          a[i * 4 + p] = cadd(cscale(x, c), cmul(y, sm));
-          a[i * 4 + q] = cadd(cmul(x, sp), cscale(y, c));
-          double2 vx = v[i * 4 + p], vy = v[i * 4 + q];
-          v[i * 4 + p] = cadd(cscale(vx, c), cmul(vy, sm));
-          v[i * 4 + q] = cadd(cmul(vx, sp), cscale(vy, c));
+        if (ag != 0.0 && ag > 1e-16 * sqrt(al * be)) {
+          rotated = true;
+          double2 ph = make_double2(g.x / ag, g.y / ag);
+          double zeta = (be - al) / (2.0 * ag);
+          double t = (zeta >= 0 ? 1.0 : -1.0) / (fabs(zeta) + sqrt(1.0 + zeta * zeta));
+          double c = 1.0 / sqrt(1.0 + t * t), sn = t * c;
+          // [a_p, a_q] <- [a_p, a_q] [[c, s e^{i phi}], [-s e^{-i phi}, c]]
+          double2 sp = cscale(ph, sn), sm = cscale(cconj(ph), -sn);
+#pragma unroll
+          for (int i = 0; i < 4; ++i) {
+            double2 x = a[i * 4 + p], y = a[i * 4 + q];
+            a[i * 4 + p] = cadd(cscale(x, c), cmul(y, sm));
+            a[i * 4 + q] = cadd(cmul(x, sp), cscale(y, c));
+            double2 vx = v[i * 4 + p], vy = v[i * 4 + q];
+            v[i * 4 + p] = cadd(cscale(vx, c), cmul(vy, sm));
+            v[i * 4 + q] = cadd(cmul(vx, sp), cscale(vy, c));
+          }
         }
       }
     if (!rotated) break;
   }
   double sig[4], smax = 0.0, ssum = 0.0;
-  for (int j = 0; j < d; ++j) {
+#pragma unroll
+  for (int j = 0; j < 4; ++j) {
     double s2 = 0.0;
-    for (int i = 0; i < d; ++i) s2 += cabs2(a[i * 4 + j]);
+#pragma unroll
+    for (int i = 0; i < 4; ++i) s2 += cabs2(a[i * 4 + j]);
     sig[j] = sqrt(s2);
-    smax = fmax(smax, sig[j]);
-    ssum += sig[j];
+    if (j < d) {
+      smax = fmax(smax, sig[j]);
+      ssum += sig[j];
+    }
   }
   *nuc = ssum;
-  // well-determined pairs (u_j, v_j); the rest are completed canonically:
-  // both sides by two-pass Gram-Schmidt of e_0, e_1, ... in order, paired in
-  // the order found.  The factor then depends only on the kept subspaces.
-  double2 us[16], vs[16];
-  int r = 0;
-  for (int j = 0; j < d; ++j) {
-    if (smax > 0.0 && sig[j] > 1e-13 * smax) {
-      for (int i = 0; i < d; ++i) {
-        us[i * 4 + r] = cscale(a[i * 4 + j], 1.0 / sig[j]);
-        vs[i * 4 + r] = v[i * 4 + j];
-      }
-      ++r;
+  bool good[4];
+#pragma unroll
+  for (int j = 0; j < 4; ++j) {
+    good[j] = (j >= d) || (smax > 0.0 && sig[j] > 1e-13 * smax);
+    if (good[j]) {
+#pragma unroll
+      for (int i = 0; i < 4; ++i) a[i * 4 + j] = cscale(a[i * 4 + j], 1.0 / sig[j]);
     }
   }
-  int ru = r, rv = r;
+  // canonical completion of the other columns, U side (a) then V side (v)
+#pragma unroll
   for (int side = 0; side < 2; ++side) {
-    double2* basis = side == 0 ? us : vs;
-    int& cnt = side == 0 ? ru : rv;
-    for (int e0 = 0; e0 < d && cnt < d; ++e0) {
-      double2 x[4];
-      for (int i = 0; i < d; ++i) x[i] = make_double2(i == e0 ? 1.0 : 0.0, 0.0);
-      for (int pass = 0; pass < 2; ++pass)
-        for (int b = 0; b < cnt; ++b) {
-          double2 dd = cz();
-          for (int i = 0; i < d; ++i) dd = cfmac(basis[i * 4 + b], x[i], dd);
-          for (int i = 0; i < d; ++i) x[i] = csub(x[i], cmul(basis[i * 4 + b], dd));
+    double2* m = side == 0 ? a : v;
+    bool set[4];
+#pragma unroll
+    for (int j = 0; j < 4; ++j) set[j] = good[j];
+    int next_e = 0;
+#pragma unroll
+    for (int j = 0; j < 4; ++j) {
+      if (!set[j]) {
+#pragma unroll
+        for (int e0 = 0; e0 < 4; ++e0) {
+          if (!set[j] && e0 >= next_e) {
+            double2 x[4];
+#pragma unroll
+            for (int i = 0; i < 4; ++i) x[i] = make_double2(i == e0 ? 1.0 : 0.0, 0.0);
+#pragma unroll
+            for (int pass = 0; pass < 2; ++pass)
+#pragma unroll
+              for (int b = 0; b < 4; ++b) {
+                if (set[b]) {
+                  double2 dd = cz();
+#pragma unroll
+                  for (int i = 0; i < 4; ++i) dd = cfmac(m[i * 4 + b], x[i], dd);
+#pragma unroll
+                  for (int i = 0; i < 4; ++i) x[i] = csub(x[i], cmul(m[i * 4 + b], dd));
+                }
+              }
+            double nr = 0.0;
+#pragma unroll
+            for (int i = 0; i < 4; ++i) nr += cabs2(x[i]);
+            nr = sqrt(nr);
+            if (nr > 0.5) {
+#pragma unroll
+              for (int i = 0; i < 4; ++i) m[i * 4 + j] = cscale(x[i], 1.0 / nr);
+              set[j] = true;
+              next_e = e0 + 1;
+            }
+          }
         }
-      double nr = 0.0;
-      for (int i = 0; i < d; ++i) nr += cabs2(x[i]);
-      nr = sqrt(nr);
-      if (nr > 0.5) {
-        for (int i = 0; i < d; ++i) basis[i * 4 + cnt] = cscale(x[i], 1.0 / nr);
-        ++cnt;
       }
     }
   }
-  // out = sum_m us_m vs_m^H
-  for (int i = 0; i < d; ++i)
-    for (int j = 0; j < d; ++j) {
+  // out = U V^H (the d x d block)
+#pragma unroll
+  for (int i = 0; i < 4; ++i)
+#pragma unroll
+    for (int j = 0; j < 4; ++j) {
       double2 s = cz();
-      for (int m = 0; m < d; ++m) s = cfma(us[i * 4 + m], cconj(vs[j * 4 + m]), s);
-      out[i * 4 + j] = s;
+#pragma unroll
+      for (int mm = 0; mm < 4; ++mm) s = cfma(a[i * 4 + mm], cconj(v[j * 4 + mm]), s);
+      out[i * 4 + j] = (i < d && j < d) ? s : cz();
     }
 }
 

@@ -17,8 +17,9 @@ constexpr double kGsTol = 1e-7;      // canonical-gauge Gram-Schmidt threshold
 constexpr double kPhaseTol = 1e-12;  // Ran two-site phase fix skipped below this
 
 constexpr int kWinTileBits = 11;     // window pass tile: 2^11 amplitudes
+constexpr int kWinThreads = 512;     // window pass block size
 constexpr int kMaxWindow = 8;        // gates per window
-constexpr int kWinMinQubits = 8;     // windowed sweep needs >= 2^4 rest indices per wave
+constexpr int kWinMinQubits = 10;    // windowed sweep: 4 waves x k-steps of 4 rest indices
 
 // error bits accumulated on the device
 constexpr int kErrNotUnitary = 1;
@@ -67,7 +68,8 @@ template <typename S>
 struct WinPassArgs {
   int n, lmax, tbits;
   int s0, s1, p0, p1;
-  uint64_t umask;
+  uint64_t umask;      // 4-bit set of this window
+  uint64_t pmask;      // 4-bit set of the previous window (== umask for the first)
   const S* target;
   S* tprime;
   S* phi;

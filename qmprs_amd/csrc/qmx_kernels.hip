@@ -578,10 +578,9 @@ __global__ __launch_bounds__(256) void chain2_step_kernel(Chain2Args a) {
           rho[p * 4 + q] = h;
           rho[q * 4 + p] = cconj(h);
         }
-      herm_eig4(ncols, rho, v, lam);
-      int k = keep_count(lam, ncols, 2, a.cutoff);
-      for (int j = 0; j < 16; ++j) w[j] = cz();
-      int found = canonical_basis4(ncols, k, v, w, kGsTol);
+      herm_eig4(rho, v, lam);   // rho is zero-padded to 4 x 4
+      int k = keep_count(lam, 4, 2, a.cutoff);
+      int found = canonical_basis4(k, v, w, kGsTol);
       if (found < k) {
         atomicOr(a.error, kErrBasis);
         k = found > 0 ? found : 1;

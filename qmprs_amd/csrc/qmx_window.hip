@@ -81,6 +81,59 @@ QMX_DEV void wtile_gate_adj(S* s, int t, int width, int m1, int m0, const double
   }
 }
 
+// complex64 tiles: gate arithmetic in FP32 (the storage precision)
+QMX_DEV float2 fcmulc(float2 a, float2 b) {
+  return make_float2(fmaf(a.x, b.x, a.y * b.y), fmaf(a.x, b.y, -a.y * b.x));
+}
+QMX_DEV float2 fcfmac(float2 a, float2 b, float2 acc) {
+  acc.x = fmaf(a.x, b.x, acc.x);
+  acc.x = fmaf(a.y, b.y, acc.x);
+  acc.y = fmaf(a.x, b.y, acc.y);
+  acc.y = fmaf(-a.y, b.x, acc.y);
+  return acc;
+}
+
+QMX_DEV void wtile_gate_adj_f32(float2* s, int t, int width, int m1, int m0, const float2* gm) {
+  if (width == 2) {
+    const int ng = 1 << (t - 2);
+    const int s0 = 1 << m0, s1 = 1 << m1;
+    for (int q = threadIdx.x; q < ng; q += blockDim.x) {
+      int l0 = (int)insert_zero(insert_zero((uint64_t)q, m0), m1);
+      float2 x0 = s[l0], x1 = s[l0 + s0], x2 = s[l0 + s1], x3 = s[l0 + s0 + s1];
+      float2 y[4];
+#pragma unroll
+      for (int o = 0; o < 4; ++o) {
+        float2 acc = fcmulc(gm[0 * 4 + o], x0);
+        acc = fcfmac(gm[1 * 4 + o], x1, acc);
+        acc = fcfmac(gm[2 * 4 + o], x2, acc);
+        acc = fcfmac(gm[3 * 4 + o], x3, acc);
+        y[o] = acc;
+      }
+      s[l0] = y[0]; s[l0 + s0] = y[1]; s[l0 + s1] = y[2]; s[l0 + s0 + s1] = y[3];
+    }
+  } else {
+    const int ng = 1 << (t - 1);
+    const int s1 = 1 << m1;
+    for (int q = threadIdx.x; q < ng; q += blockDim.x) {
+      int l0 = (int)insert_zero((uint64_t)q, m1);
+      float2 x0 = s[l0], x1 = s[l0 + s1];
+      s[l0] = fcfmac(gm[1 * 4 + 0], x1, fcmulc(gm[0 * 4 + 0], x0));
+      s[l0 + s1] = fcfmac(gm[1 * 4 + 1], x1, fcmulc(gm[0 * 4 + 1], x0));
+    }
+  }
+}
+
+template <typename S>
+QMX_DEV void wgate(S* s, int t, int width, int m1, int m0, const double2* gm, const float2* gmf);
+template <>
+QMX_DEV void wgate<float2>(float2* s, int t, int width, int m1, int m0, const double2*, const float2* gmf) {
+  wtile_gate_adj_f32(s, t, width, m1, m0, gmf);
+}
+template <>
+QMX_DEV void wgate<double2>(double2* s, int t, int width, int m1, int m0, const double2* gm, const float2*) {
+  wtile_gate_adj<double2>(s, t, width, m1, m0, gm);
+}
+
 QMX_DEV int wsane(int n, int site, int w) { return (w == 2 && site < n - 1) ? 2 : 1; }
 
 // step -> (layer, site) of the sweep order (reverse circuit order)
@@ -89,12 +142,93 @@ QMX_DEV void step_gate(int n, int s, int& layer, int& site) {
   site = n - 1 - s % n;
 }
 
+// ---- register-group gate application ------------------------------------
+// A lane holds the 16 amplitudes of one group over a 4-bit set (index x, bit
+// m of x = m-th lowest bit of the set).  Gate adjoints act on bits (HB, LB)
+// of x (LB < 0: one-qubit gate on HB); indices are compile-time.
+template <typename C> struct CT;
+template <> struct CT<float> {
+  using c2 = float2;
+  static QMX_DEV c2 mk(double2 v) { return make_float2((float)v.x, (float)v.y); }
+  static QMX_DEV c2 mulc(c2 a, c2 b) { return make_float2(fmaf(a.x, b.x, a.y * b.y), fmaf(a.x, b.y, -a.y * b.x)); }
+  static QMX_DEV c2 fmac(c2 a, c2 b, c2 acc) {
+    acc.x = fmaf(a.x, b.x, acc.x); acc.x = fmaf(a.y, b.y, acc.x);
+    acc.y = fmaf(a.x, b.y, acc.y); acc.y = fmaf(-a.y, b.x, acc.y);
+    return acc;
+  }
+};
+template <> struct CT<double> {
+  using c2 = double2;
+  static QMX_DEV c2 mk(double2 v) { return v; }
+  static QMX_DEV c2 mulc(c2 a, c2 b) { return cmulc(a, b); }
+  static QMX_DEV c2 fmac(c2 a, c2 b, c2 acc) { return cfmac(a, b, acc); }
+};
+
+template <typename C, int HB, int LB>
+QMX_DEV void grp_adj(typename CT<C>::c2 (&a)[16], const typename CT<C>::c2* g) {
+  using T = CT<C>;
+  if (LB >= 0) {
+#pragma unroll
+    for (int x = 0; x < 16; ++x) {
+      if (((x >> HB) & 1) || ((x >> (LB < 0 ? 0 : LB)) & 1)) continue;
+      const int i0 = x, i1 = x | (1 << (LB < 0 ? 0 : LB)), i2 = x | (1 << HB), i3 = i1 | (1 << HB);
+      typename T::c2 x0 = a[i0], x1 = a[i1], x2 = a[i2], x3 = a[i3];
+      typename T::c2 y[4];
+#pragma unroll
+      for (int o = 0; o < 4; ++o) {
+        typename T::c2 acc = T::mulc(g[0 * 4 + o], x0);
+        acc = T::fmac(g[1 * 4 + o], x1, acc);
+        acc = T::fmac(g[2 * 4 + o], x2, acc);
+        acc = T::fmac(g[3 * 4 + o], x3, acc);
+        y[o] = acc;
+      }
+      a[i0] = y[0]; a[i1] = y[1]; a[i2] = y[2]; a[i3] = y[3];
+    }
+  } else {
+#pragma unroll
+    for (int x = 0; x < 16; ++x) {
+      if ((x >> HB) & 1) continue;
+      const int i0 = x, i1 = x | (1 << HB);
+      typename T::c2 x0 = a[i0], x1 = a[i1];
+      a[i0] = T::fmac(g[1 * 4 + 0], x1, T::mulc(g[0 * 4 + 0], x0));
+      a[i1] = T::fmac(g[1 * 4 + 1], x1, T::mulc(g[0 * 4 + 1], x0));
+    }
+  }
+}
+
+// code = hb * 2 + (width == 2): width-2 gates have lb = hb - 1
+template <typename C>
+QMX_DEV void grp_apply(int code, typename CT<C>::c2 (&a)[16], const typename CT<C>::c2* g) {
+  switch (code) {
+    case 3: grp_adj<C, 1, 0>(a, g); break;
+    case 5: grp_adj<C, 2, 1>(a, g); break;
+    case 7: grp_adj<C, 3, 2>(a, g); break;
+    case 0: grp_adj<C, 0, -1>(a, g); break;
+    case 2: grp_adj<C, 1, -1>(a, g); break;
+    case 4: grp_adj<C, 2, -1>(a, g); break;
+    case 6: grp_adj<C, 3, -1>(a, g); break;
+    default: break;
+  }
+}
+
+// LDS element swizzle: low 5 bits XOR a fold of the higher bits (bijective)
+QMX_DEV int swz(int l) { return l ^ (((l >> 5) ^ (l >> 10)) & 31); }
+
+template <typename S> struct StoreC;
+template <> struct StoreC<float2> { using C = float; };
+template <> struct StoreC<double2> { using C = double; };
+
 template <typename S>
 __global__ __launch_bounds__(256) void window_pass_kernel(WinPassArgs<S> a) {
+  using C = typename StoreC<S>::C;
+  using T = CT<C>;
+  using c2 = typename T::c2;
   extern __shared__ char wsm[];
   __shared__ WGeom g;
-  __shared__ double2 gmat[2 * kMaxWindow][16];
-  __shared__ int ginfo[2 * kMaxWindow][3];   // width, m1, m0
+  __shared__ c2 gmat[2 * kMaxWindow][16];
+  __shared__ int gcode[2 * kMaxWindow];
+  __shared__ int xoffp[16];          // group offsets over U_prev
+  __shared__ int restp[kWinTileBits];
   __shared__ int s_np, s_nc;
   const int b = blockIdx.y;
   const int n = a.n;
@@ -104,21 +238,7 @@ __global__ __launch_bounds__(256) void window_pass_kernel(WinPassArgs<S> a) {
   const int nc = min(a.s1, kb) - a.s0;
   const int np = a.p1 - a.p0;   // previous window (all valid: p1 == s0 < kb)
   if (threadIdx.x == 0) {
-    uint64_t pm = a.umask;
-    int sites[2 * kMaxWindow], widths[2 * kMaxWindow];
-    for (int j = 0; j < np + nc; ++j) {
-      int s = j < np ? a.p0 + j : a.s0 + (j - np);
-      int layer, site;
-      step_gate(n, s, layer, site);
-      size_t gi = ((size_t)b * a.lmax + layer) * n + site;
-      int w = wsane(n, site, a.widths[gi]);
-      sites[j] = site;
-      widths[j] = w;
-      for (int e = 0; e < 16; ++e) gmat[j][e] = a.gates[gi * 16 + e];
-      pm |= 1ull << (n - 1 - site);
-      if (w == 2) pm |= 1ull << (n - 2 - site);
-    }
-    // tile = gate bits + lowest bits
+    uint64_t pm = a.umask | a.pmask;
     int t = a.tbits < n ? a.tbits : n;
     uint64_t full = (1ull << n) - 1ull;
     for (int bit = 0; bit < n && __popcll(pm) < t; ++bit) pm |= 1ull << bit;
@@ -132,35 +252,47 @@ __global__ __launch_bounds__(256) void window_pass_kernel(WinPassArgs<S> a) {
     g.nh = nh;
     g.qmask = full & ~pm;
     for (int h = 0; h < (1 << nh); ++h) g.hoffs[h] = (int)wdeposit((uint64_t)h, high);
-    int m = 0;
-    for (int bit = 0; bit < n; ++bit)
-      if ((a.umask >> bit) & 1ull) g.ux[m++] = wlocal(pm, bit);
-    for (int x = 0; x < 16; ++x) {
-      int off = 0;
-      for (int q = 0; q < 4; ++q)
-        if ((x >> q) & 1) off |= 1 << g.ux[q];
-      g.xoff[x] = off;
-    }
-    int rp = 0;
-    for (int l = 0; l < t; ++l) {
-      bool inu = false;
-      for (int q = 0; q < 4; ++q) inu |= (g.ux[q] == l);
-      if (!inu) g.restpos[rp++] = l;
+    // group index spaces: U (phi, R) and U_prev (T')
+    for (int side = 0; side < 2; ++side) {
+      uint64_t um = side == 0 ? a.umask : a.pmask;
+      int ux[4];
+      int m = 0;
+      for (int bit = 0; bit < n; ++bit)
+        if ((um >> bit) & 1ull) ux[m++] = wlocal(pm, bit);
+      int* xo = side == 0 ? g.xoff : xoffp;
+      int* rp = side == 0 ? g.restpos : restp;
+      for (int x = 0; x < 16; ++x) {
+        int off = 0;
+        for (int q = 0; q < 4; ++q)
+          if ((x >> q) & 1) off |= 1 << ux[q];
+        xo[x] = off;
+      }
+      int r = 0;
+      for (int l = 0; l < t; ++l) {
+        bool inu = false;
+        for (int q = 0; q < 4; ++q) inu |= (ux[q] == l);
+        if (!inu) rp[r++] = l;
+      }
     }
     for (int j = 0; j < np + nc; ++j) {
-      ginfo[j][0] = widths[j];
-      ginfo[j][1] = wlocal(pm, n - 1 - sites[j]);
-      ginfo[j][2] = widths[j] == 2 ? wlocal(pm, n - 2 - sites[j]) : 0;
+      int s = j < np ? a.p0 + j : a.s0 + (j - np);
+      int layer, site;
+      step_gate(n, s, layer, site);
+      size_t gi = ((size_t)b * a.lmax + layer) * n + site;
+      int w = wsane(n, site, a.widths[gi]);
+      for (int e = 0; e < 16; ++e) gmat[j][e] = T::mk(a.gates[gi * 16 + e]);
+      uint64_t um = j < np ? a.pmask : a.umask;
+      int hb = __popcll(um & ((1ull << (n - 1 - site)) - 1ull));
+      gcode[j] = hb * 2 + (w == 2 ? 1 : 0);
     }
     s_np = np;
     s_nc = nc;
   }
   __syncthreads();
-  const int t = g.t, tl = 1 << t;
-  constexpr int kPer = (1 << kWinTileBits) / 256;
+  const int t = g.t;
   const size_t len = (size_t)1 << n;
   S* sT = reinterpret_cast<S*>(wsm);
-  S* sP = sT + tl;
+  S* sP = sT + (1 << t);
   const bool first = (np == 0);
   // T' is first written by window 1: windows 0 and 1 read the target
   const S* tsrc = (a.p0 == 0 ? a.target : a.tprime) + (size_t)b * len;
@@ -169,37 +301,59 @@ __global__ __launch_bounds__(256) void window_pass_kernel(WinPassArgs<S> a) {
   const long ntiles = 1l << (n - t);
   const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
   const int nrest = t - 4;
-  const int kper_wave = (1 << nrest) / 4;   // rest indices per wave per tile
-  d4v acc_rr = {0, 0, 0, 0}, acc_ii = {0, 0, 0, 0}, acc_ri = {0, 0, 0, 0}, acc_ir = {0, 0, 0, 0};
-  S rT[kPer], rP[kPer];
-  auto load_tile = [&](long tile) {
-    uint64_t base = wdeposit((uint64_t)tile, g.qmask);
+  const int ngroups = 1 << nrest;
+  const int kper_wave = ngroups / 4;
+  const int lowmask = (1 << g.c0) - 1;
+  d4v acc_r = {0, 0, 0, 0}, acc_i = {0, 0, 0, 0};
+  for (long tile = blockIdx.x; tile < ntiles; tile += gridDim.x) {
+    const uint32_t base = (uint32_t)wdeposit((uint64_t)tile, g.qmask);
+    // lanes [0, ngroups): T' groups; lanes [ngroups, 2 ngroups): phi groups
+    for (int task = threadIdx.x; task < 2 * ngroups; task += blockDim.x) {
+      const int grp = task < ngroups ? task : task - ngroups;
+      // T' group over U_prev: pull through the previous window's new gates
+      if (task < ngroups) {
+        int roff = 0;
+        for (int q = 0; q < nrest; ++q)
+          if ((grp >> q) & 1) roff |= 1 << restp[q];
+        c2 v[16];
 #pragma unroll
-    for (int j = 0; j < kPer; ++j) {
-      int l = threadIdx.x + 256 * j;
-      if (l < tl) {
-        size_t ad = (size_t)(base + (uint64_t)(l & ((1 << g.c0) - 1)) + (uint64_t)g.hoffs[l >> g.c0]);
-        rT[j] = tsrc[ad];
-        rP[j] = phi[ad];
+        for (int x = 0; x < 16; ++x) {
+          int l = roff + xoffp[x];
+          uint32_t ad = base + (uint32_t)(l & lowmask) + (uint32_t)g.hoffs[l >> g.c0];
+          v[x] = T::mk(to_c(tsrc[ad]));
+        }
+        for (int j = 0; j < s_np; ++j) grp_apply<C>(gcode[j], v, gmat[j]);
+#pragma unroll
+        for (int x = 0; x < 16; ++x) {
+          int l = roff + xoffp[x];
+          S sv = from_c<S>(make_double2(v[x].x, v[x].y));
+          sT[swz(l)] = sv;
+          if (!first) tdst[base + (uint32_t)(l & lowmask) + (uint32_t)g.hoffs[l >> g.c0]] = sv;
+        }
+      }
+      // phi group over U: remove the window's old gates
+      else {
+        int roff = 0;
+        for (int q = 0; q < nrest; ++q)
+          if ((grp >> q) & 1) roff |= 1 << g.restpos[q];
+        c2 v[16];
+#pragma unroll
+        for (int x = 0; x < 16; ++x) {
+          int l = roff + g.xoff[x];
+          v[x] = T::mk(to_c(phi[base + (uint32_t)(l & lowmask) + (uint32_t)g.hoffs[l >> g.c0]]));
+        }
+        grp_apply<C>(gcode[s_np], v, gmat[s_np]);
+#pragma unroll
+        for (int x = 0; x < 16; ++x) sP[swz(roff + g.xoff[x])] = from_c<S>(make_double2(v[x].x, v[x].y));
+        for (int j = 1; j < s_nc; ++j) grp_apply<C>(gcode[s_np + j], v, gmat[s_np + j]);
+#pragma unroll
+        for (int x = 0; x < 16; ++x) {
+          int l = roff + g.xoff[x];
+          phi[base + (uint32_t)(l & lowmask) + (uint32_t)g.hoffs[l >> g.c0]] =
+              from_c<S>(make_double2(v[x].x, v[x].y));
+        }
       }
     }
-  };
-  long tile = blockIdx.x;
-  if (tile < ntiles) load_tile(tile);
-  for (; tile < ntiles; tile += gridDim.x) {
-#pragma unroll
-    for (int j = 0; j < kPer; ++j) {
-      int l = threadIdx.x + 256 * j;
-      if (l < tl) { sT[l] = rT[j]; sP[l] = rP[j]; }
-    }
-    __syncthreads();
-    long next = tile + gridDim.x;
-    if (next < ntiles) load_tile(next);
-    for (int j = 0; j < s_np; ++j) {
-      wtile_gate_adj<S>(sT, t, ginfo[j][0], ginfo[j][1], ginfo[j][2], gmat[j]);
-      __syncthreads();
-    }
-    wtile_gate_adj<S>(sP, t, ginfo[s_np][0], ginfo[s_np][1], ginfo[s_np][2], gmat[s_np]);
     __syncthreads();
     // R += conj(T') phi^T over this tile's rest indices: wave w takes a quarter
     {
@@ -209,26 +363,16 @@ __global__ __launch_bounds__(256) void window_pass_kernel(WinPassArgs<S> a) {
         int roff = 0;
         for (int q = 0; q < nrest; ++q)
           if ((r >> q) & 1) roff |= 1 << g.restpos[q];
-        double2 tv = to_c(sT[g.xoff[i] + roff]);
-        double2 pv = to_c(sP[g.xoff[i] + roff]);
+        double2 tv = to_c(sT[swz(g.xoff[i] + roff)]);
+        double2 pv = to_c(sP[swz(g.xoff[i] + roff)]);
         double ar = tv.x, ai = -tv.y;      // A = conj(T')  (row x = i, k = r)
         double br = pv.x, bi = pv.y;       // B = phi       (k = r, column y = i)
-        acc_rr = __builtin_amdgcn_mfma_f64_16x16x4f64(ar, br, acc_rr, 0, 0, 0);
-        acc_ii = __builtin_amdgcn_mfma_f64_16x16x4f64(ai, bi, acc_ii, 0, 0, 0);
-        acc_ri = __builtin_amdgcn_mfma_f64_16x16x4f64(ar, bi, acc_ri, 0, 0, 0);
-        acc_ir = __builtin_amdgcn_mfma_f64_16x16x4f64(ai, br, acc_ir, 0, 0, 0);
+        // Re R += ar br - ai bi,  Im R += ar bi + ai br
+        acc_r = __builtin_amdgcn_mfma_f64_16x16x4f64(ar, br, acc_r, 0, 0, 0);
+        acc_r = __builtin_amdgcn_mfma_f64_16x16x4f64(-ai, bi, acc_r, 0, 0, 0);
+        acc_i = __builtin_amdgcn_mfma_f64_16x16x4f64(ar, bi, acc_i, 0, 0, 0);
+        acc_i = __builtin_amdgcn_mfma_f64_16x16x4f64(ai, br, acc_i, 0, 0, 0);
       }
-    }
-    __syncthreads();
-    for (int j = 1; j < s_nc; ++j) {
-      wtile_gate_adj<S>(sP, t, ginfo[s_np + j][0], ginfo[s_np + j][1], ginfo[s_np + j][2], gmat[s_np + j]);
-      __syncthreads();
-    }
-    uint64_t base = wdeposit((uint64_t)tile, g.qmask);
-    for (int l = threadIdx.x; l < tl; l += blockDim.x) {
-      size_t ad = (size_t)(base + (uint64_t)(l & ((1 << g.c0) - 1)) + (uint64_t)g.hoffs[l >> g.c0]);
-      if (!first) tdst[ad] = sT[l];
-      phi[ad] = sP[l];
     }
     __syncthreads();
   }
@@ -239,8 +383,8 @@ __global__ __launch_bounds__(256) void window_pass_kernel(WinPassArgs<S> a) {
 #pragma unroll
     for (int it = 0; it < 4; ++it) {
       int i = it * 4 + ib;
-      red[wave * 512 + (i * 16 + j) * 2] = acc_rr[it] - acc_ii[it];
-      red[wave * 512 + (i * 16 + j) * 2 + 1] = acc_ri[it] + acc_ir[it];
+      red[wave * 512 + (i * 16 + j) * 2] = acc_r[it];
+      red[wave * 512 + (i * 16 + j) * 2 + 1] = acc_i[it];
     }
   }
   __syncthreads();
@@ -256,6 +400,7 @@ __global__ __launch_bounds__(256) void window_finalize_kernel(WinFinArgs a) {
   __shared__ double2 gm[kMaxWindow][16];
   __shared__ int gw[kMaxWindow], gh[kMaxWindow], gl[kMaxWindow];
   __shared__ double2 gnew[16];
+  __shared__ double2 es[16];
   __shared__ double part_sum[512];
   const int b = blockIdx.x;
   const int n = a.n;
@@ -294,39 +439,42 @@ __global__ __launch_bounds__(256) void window_finalize_kernel(WinFinArgs a) {
   __syncthreads();
   for (int j = 0; j < nc; ++j) {
     const int w = gw[j], hb = gh[j], lb = gl[j];
-    if (threadIdx.x == 0) {
-      // E[o][i] = sum_z Q[(o,z)][(i,z)] over the U bits outside gate j
-      double2 e[16];
-      for (int k = 0; k < 16; ++k) e[k] = cz();
+    // E[o][i] = sum_z Q[(o,z)][(i,z)] over the U bits outside gate j (16 lanes)
+    if (threadIdx.x < 16) {
+      const int o = threadIdx.x >> 2, i = threadIdx.x & 3;
       const int d = w == 2 ? 4 : 2;
-      for (int xx = 0; xx < 16; ++xx) {
-        int o = w == 2 ? (((xx >> hb) & 1) << 1) | ((xx >> lb) & 1) : ((xx >> hb) & 1);
-        for (int i = 0; i < d; ++i) {
-          // y = x with gate bits replaced by i
-          int yy = xx;
-          if (w == 2) {
-            yy = (yy & ~((1 << hb) | (1 << lb))) | (((i >> 1) & 1) << hb) | ((i & 1) << lb);
-          } else {
-            yy = (yy & ~(1 << hb)) | ((i & 1) << hb);
-          }
-          e[o * 4 + i] = cadd(e[o * 4 + i], q[xx * 16 + yy]);
+      double2 acc = cz();
+      if (o < d && i < d) {
+        for (int xx = 0; xx < 16; ++xx) {
+          int go = w == 2 ? (((xx >> hb) & 1) << 1) | ((xx >> lb) & 1) : ((xx >> hb) & 1);
+          if (go != o) continue;
+          int yy;
+          if (w == 2) yy = (xx & ~((1 << hb) | (1 << lb))) | (((i >> 1) & 1) << hb) | ((i & 1) << lb);
+          else yy = (xx & ~(1 << hb)) | ((i & 1) << hb);
+          acc = cadd(acc, q[xx * 16 + yy]);
         }
       }
-      double2 u[16];
+      es[threadIdx.x] = acc;
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+      double2 e[16], u[16];
+#pragma unroll
+      for (int k = 0; k < 16; ++k) e[k] = es[k];
+      const int d = w == 2 ? 4 : 2;
       double nuc = 0.0;
       polar4(d, e, u, &nuc);
-      int layer, site;
-      step_gate(n, a.s0 + j, layer, site);
-      size_t gi = ((size_t)b * a.lmax + layer) * n + site;
-      for (int k = 0; k < 16; ++k) {
-        int o = k / 4, i = k % 4;
-        double2 v = (o < d && i < d) ? cconj(u[k]) : cz();
-        gnew[k] = v;
-        a.gates[gi * 16 + k] = v;
-      }
+#pragma unroll
+      for (int k = 0; k < 16; ++k) gnew[k] = cconj(u[k]);
       if (a.s0 + j == kb - 1 && a.overlap) a.overlap[(size_t)b * a.overlap_stride] = nuc;
     }
     __syncthreads();
+    if (threadIdx.x < 16) {
+      int layer, site;
+      step_gate(n, a.s0 + j, layer, site);
+      size_t gi = ((size_t)b * a.lmax + layer) * n + site;
+      a.gates[gi * 16 + threadIdx.x] = gnew[threadIdx.x];
+    }
     // target side: Q[x][y] <- sum_{x'} G'[x'_g][x_g] Q[x'][y]
     {
       double2 s = cz();
@@ -365,7 +513,8 @@ template <typename S>
 void launch_window_pass(const WinPassArgs<S>& a, int nblk, int batch, hipStream_t st) {
   int t = a.tbits < a.n ? a.tbits : a.n;
   size_t shm = 2 * (sizeof(S) << t);
-  if (shm < 4 * 512 * sizeof(double)) shm = 4 * 512 * sizeof(double);
+  const size_t red = 4 * 512 * sizeof(double);
+  if (shm < red) shm = red;
   hipLaunchKernelGGL(window_pass_kernel<S>, dim3(nblk, batch), dim3(256), shm, st, a);
 }
 template void launch_window_pass<float2>(const WinPassArgs<float2>&, int, int, hipStream_t);
