@@ -239,6 +239,72 @@ struct Window {
   uint64_t umask;
 };
 
+// ---- tile geometry (host side) ---------------------------------------------
+uint64_t tile_bits(int n, int t, uint64_t need) {
+  uint64_t p = need;
+  for (int bit = 0; bit < n && __builtin_popcountll(p) < t; ++bit) p |= 1ull << bit;
+  return p;
+}
+
+GroupMap make_group(int n, uint64_t p, uint64_t u) {
+  GroupMap g{};
+  g.mask = u;
+  int m = 0;
+  for (int bit = 0; bit < n && m < 4; ++bit)
+    if ((u >> bit) & 1ull) g.ubit[m++] = bit;
+  auto local = [&](int bit) { return __builtin_popcountll(p & ((1ull << bit) - 1ull)); };
+  for (int q = 0; q < 4; ++q) g.ux[q] = local(g.ubit[q]);
+  for (int bit = 0; bit < n; ++bit)
+    if (((p >> bit) & 1ull) && !((u >> bit) & 1ull)) {
+      g.restg |= 1ull << bit;
+      g.restl |= 1u << local(bit);
+    }
+  return g;
+}
+
+// Forward (circuit-order) stages: windows of consecutive gates on 4-bit sets,
+// grouped into passes of up to kMaxStages stages whose bits fit the tile.
+struct FwdPass {
+  int nstage;
+  int f0[kMaxStages], f1[kMaxStages];
+  uint64_t mask[kMaxStages];
+};
+
+std::vector<FwdPass> plan_fwd(int n, int kmax) {
+  std::vector<Window> st;
+  int f = 0;
+  while (f < kmax) {
+    Window w{f, f, 0};
+    while (w.s1 < kmax && w.s1 - w.s0 < kMaxWindow) {
+      uint64_t nb = w.umask | worst_bits(n, w.s1 % n);
+      if (__builtin_popcountll(nb) > 4) break;
+      w.umask = nb;
+      ++w.s1;
+    }
+    for (int bit = 0; bit < n && __builtin_popcountll(w.umask) < 4; ++bit) w.umask |= 1ull << bit;
+    st.push_back(w);
+    f = w.s1;
+  }
+  std::vector<FwdPass> out;
+  size_t i = 0;
+  while (i < st.size()) {
+    FwdPass p{};
+    uint64_t um = 0;
+    while (i < st.size() && p.nstage < kMaxStages) {
+      uint64_t nb = um | st[i].umask;
+      if (p.nstage > 0 && __builtin_popcountll(nb) > kFwdMaxUnion) break;
+      um = nb;
+      p.f0[p.nstage] = st[i].s0;
+      p.f1[p.nstage] = st[i].s1;
+      p.mask[p.nstage] = st[i].umask;
+      ++p.nstage;
+      ++i;
+    }
+    out.push_back(p);
+  }
+  return out;
+}
+
 // Sweep windows: maximal runs of consecutive steps (reverse circuit order)
 // whose worst-case qubits fit in 4 bits; each bit set is padded to exactly 4
 // bits with the lowest free bits.
@@ -460,6 +526,47 @@ std::vector<std::vector<int>> forward_circuit_passes(int n, int lmax) {
   return plan_passes(n, pos, sites, kApplyMaxUnion);
 }
 
+// Forward circuit passes (register-group stages).  Needs n >= 5.
+template <typename S>
+int run_fwd(qmx_handle* h, int n, int batch, int lmax, const std::vector<FwdPass>& passes, S* psi,
+            const double2* gates, const int* widths, const int* layers, const int* skip,
+            const double2* ref, double* dot_part, int* dot_counter, double2* dot_out,
+            int dot_stride, hipStream_t st) {
+  const int t = std::min(n, kFwdTileBits);
+  const int nblk = blocks_for(1l << (n - t), batch, h->cus);
+  for (size_t pi = 0; pi < passes.size(); ++pi) {
+    const FwdPass& fp = passes[pi];
+    uint64_t um = 0;
+    for (int k = 0; k < fp.nstage; ++k) um |= fp.mask[k];
+    const uint64_t p = tile_bits(n, t, um);
+    FwdArgs<S> a{};
+    a.n = n;
+    a.lmax = lmax;
+    a.t = t;
+    a.nstage = fp.nstage;
+    a.qmask = ((1ull << n) - 1ull) & ~p;
+    for (int k = 0; k < fp.nstage; ++k) {
+      a.gs[k] = make_group(n, p, fp.mask[k]);
+      a.f0[k] = fp.f0[k];
+      a.f1[k] = fp.f1[k];
+    }
+    a.psi = psi;
+    a.gates = gates;
+    a.widths = widths;
+    a.layers = layers;
+    a.skip = skip;
+    const bool last = (pi + 1 == passes.size());
+    a.ref = last ? ref : nullptr;
+    a.dot_part = dot_part;
+    a.dot_counter = dot_counter;
+    a.dot_out = dot_out;
+    a.dot_stride = dot_stride;
+    launch_fwd_pass(a, nblk, batch, st);
+  }
+  QMX_LAUNCHED(h, "forward passes");
+  return QMX_OK;
+}
+
 int run_extract(qmx_handle* h, int n, int batch, int lmax, double thr, double cutoff,
                 double2* psi_w, double2* gates, int* widths, int* bonds, int* layers_used,
                 double2* amp0, const LayerLayout& L, hipStream_t st) {
@@ -511,15 +618,23 @@ int run_sweep(qmx_handle* h, int n, int batch, int lmax, const int* layers_used,
   const bool windowed = (h->sweep_mode == QMX_SWEEP_WINDOW ||
                          (h->sweep_mode == QMX_SWEEP_AUTO)) && n >= kWinMinQubits;
   const auto wins = plan_windows(n, kmax);
+  const bool use_fwd = n >= 5 && h->sweep_mode != QMX_SWEEP_STEP;
+  const auto fwd = plan_fwd(n, kmax);
   const int wblk = std::min(kWinMaxBlocks,
                             blocks_for(1l << (n - std::min(n, kWinTileBits)), batch, h->cus));
   for (int s = 0; s < nsweeps; ++s) {
     PhaseTimer rt(h, st, QMX_PHASE_REBUILD);
     launch_basis0(L.phi, n, batch, skip, st);
-    QMX_CALL(run_passes<S>(h, n, batch, lmax, 1, 0, rebuild, L.phi, gates, widths, layers_used,
-                           skip, fidelity ? psi_ref : nullptr, L.dot_part, L.dot_counter,
-                           fidelity ? fidelity + s : nullptr, nsweeps + 1, h->cus, st));
-    rt.stop(static_cast<long long>(rebuild.size()));
+    if (use_fwd) {
+      QMX_CALL(run_fwd<S>(h, n, batch, lmax, fwd, L.phi, gates, widths, layers_used, skip,
+                          fidelity ? psi_ref : nullptr, L.dot_part, L.dot_counter,
+                          fidelity ? fidelity + s : nullptr, nsweeps + 1, st));
+    } else {
+      QMX_CALL(run_passes<S>(h, n, batch, lmax, 1, 0, rebuild, L.phi, gates, widths, layers_used,
+                             skip, fidelity ? psi_ref : nullptr, L.dot_part, L.dot_counter,
+                             fidelity ? fidelity + s : nullptr, nsweeps + 1, h->cus, st));
+    }
+    rt.stop(static_cast<long long>(use_fwd ? fwd.size() : rebuild.size()));
     if (use_goal) {
       launch_fill_int(L.active, 0, 1, st);
       launch_goal_check(fidelity + s, nsweeps + 1, goal, L.skip, L.active, sweeps_done, batch, st);
@@ -535,13 +650,16 @@ int run_sweep(qmx_handle* h, int n, int batch, int lmax, const int* layers_used,
         WinPassArgs<S> pa{};
         pa.n = n;
         pa.lmax = lmax;
-        pa.tbits = kWinTileBits;
+        pa.t = std::min(n, kWinTileBits);
         pa.s0 = w.s0;
         pa.s1 = w.s1;
         pa.p0 = wi ? wins[wi - 1].s0 : 0;
         pa.p1 = wi ? wins[wi - 1].s1 : 0;
-        pa.umask = w.umask;
-        pa.pmask = wi ? wins[wi - 1].umask : w.umask;
+        const uint64_t pmask = wi ? wins[wi - 1].umask : w.umask;
+        const uint64_t pbits = tile_bits(n, pa.t, w.umask | pmask);
+        pa.qmask = ((1ull << n) - 1ull) & ~pbits;
+        pa.gu = make_group(n, pbits, w.umask);
+        pa.gp = make_group(n, pbits, pmask);
         pa.target = target;
         pa.tprime = L.tprime;
         pa.phi = L.phi;
@@ -595,9 +713,15 @@ int run_sweep(qmx_handle* h, int n, int batch, int lmax, const int* layers_used,
   }
   // final circuit state and fidelity
   launch_basis0(L.phi, n, batch, nullptr, st);
-  QMX_CALL(run_passes<S>(h, n, batch, lmax, 1, 0, rebuild, L.phi, gates, widths, layers_used,
-                         nullptr, fidelity ? psi_ref : nullptr, L.dot_part, L.dot_counter,
-                         fidelity ? fidelity + nsweeps : nullptr, nsweeps + 1, h->cus, st));
+  if (use_fwd) {
+    QMX_CALL(run_fwd<S>(h, n, batch, lmax, fwd, L.phi, gates, widths, layers_used, nullptr,
+                        fidelity ? psi_ref : nullptr, L.dot_part, L.dot_counter,
+                        fidelity ? fidelity + nsweeps : nullptr, nsweeps + 1, st));
+  } else {
+    QMX_CALL(run_passes<S>(h, n, batch, lmax, 1, 0, rebuild, L.phi, gates, widths, layers_used,
+                           nullptr, fidelity ? psi_ref : nullptr, L.dot_part, L.dot_counter,
+                           fidelity ? fidelity + nsweeps : nullptr, nsweeps + 1, h->cus, st));
+  }
   if (circuit_out) launch_scale_convert<S, S>(L.phi, circuit_out, n, batch, nullptr, st);
   QMX_LAUNCHED(h, "final circuit state");
   return QMX_OK;

@@ -61,15 +61,33 @@ struct SweepArgs {
   int overlap_stride;
 };
 
+// Host-computed tile geometry.  A tile is a set P of t bits (gate bits plus
+// the lowest free bits); tiles partition the state and tile id k sits at the
+// deposit of k into the bits outside P (qmask).
+//
+// A 4-bit group set U inside a tile: element x of group g (bit m of x <->
+// m-th lowest bit of U) sits at global offset base + dep(g, restg) + xg(x)
+// and at tile-local index dep(g, restl) + xl(x), where restg / restl are the
+// tile bits outside U as global / tile-local masks and xg / xl deposit x
+// into ubit / ux.
+struct GroupMap {
+  uint64_t mask;       // the 4 global bits
+  uint64_t restg;      // tile bits outside U (global positions)
+  uint32_t restl;      // the same bits as tile-local positions
+  int ubit[4];         // global positions of U's bits (ascending)
+  int ux[4];           // their tile-local positions
+};
+
 // Windowed sweep (qmx_window.hip): window = sweep steps [s0, s1) on the
 // 4-bit set umask; the previous window [p0, p1) (p1 == s0, empty for the
 // first) has its updated gates applied to T' in the same pass.
 template <typename S>
 struct WinPassArgs {
-  int n, lmax, tbits;
+  int n, lmax, t;
   int s0, s1, p0, p1;
-  uint64_t umask;      // 4-bit set of this window
-  uint64_t pmask;      // 4-bit set of the previous window (== umask for the first)
+  uint64_t qmask;      // bits outside the tile
+  GroupMap gu;         // this window's set (phi groups, R)
+  GroupMap gp;         // previous window's set (T' groups; == gu for the first)
   const S* target;
   S* tprime;
   S* phi;
@@ -78,6 +96,29 @@ struct WinPassArgs {
   const int* layers;
   const int* skip;
   double* part;        // [B][nblk][512]
+};
+
+// Forward circuit pass (qmx_window.hip): up to kMaxStages stages of gates,
+// each stage acting inside its own 4-bit group set, exchanged through LDS.
+constexpr int kMaxStages = 3;
+constexpr int kFwdTileBits = 12;
+constexpr int kFwdMaxUnion = 8;      // stage bits per pass (>= 4 coalescing bits)
+template <typename S>
+struct FwdArgs {
+  int n, lmax, t, nstage;
+  uint64_t qmask;
+  GroupMap gs[kMaxStages];
+  int f0[kMaxStages], f1[kMaxStages];   // forward positions of each stage
+  S* psi;
+  const double2* gates;
+  const int* widths;
+  const int* layers;
+  const int* skip;
+  const double2* ref;  // optional overlap reference (complex128)
+  double* dot_part;
+  int* dot_counter;
+  double2* dot_out;
+  int dot_stride;
 };
 
 struct WinFinArgs {
@@ -146,6 +187,7 @@ void launch_fill_int(int* p, int v, int count, hipStream_t);
 template <typename S> void launch_basis0(S* psi, int n, int batch, const int* skip, hipStream_t);
 void launch_chain2_step(const Chain2Args&, int nblk, int batch, hipStream_t);
 template <typename S> void launch_window_pass(const WinPassArgs<S>&, int nblk, int batch, hipStream_t);
+template <typename S> void launch_fwd_pass(const FwdArgs<S>&, int nblk, int batch, hipStream_t);
 void launch_window_finalize(const WinFinArgs&, int batch, hipStream_t);
 void launch_ran_gates(const RanArgs&, int batch, hipStream_t);
 void launch_amp0(const Amp0Args&, hipStream_t);

@@ -23,15 +23,6 @@ namespace qmx {
 
 typedef double d4v __attribute__((ext_vector_type(4)));
 
-struct WGeom {
-  int t, c0, nh;
-  uint64_t qmask;
-  int hoffs[1 << kMaxHighBits];
-  int ux[4];            // local positions of U's bits (ascending)
-  int xoff[16];         // local offset of x (bit m of x -> ux[m])
-  int restpos[kWinTileBits];  // local positions outside U (ascending)
-};
-
 QMX_DEV uint64_t wdeposit(uint64_t x, uint64_t mask) {
   uint64_t r = 0;
   while (mask) {
@@ -136,6 +127,22 @@ QMX_DEV void wgate<double2>(double2* s, int t, int width, int m1, int m0, const 
 
 QMX_DEV int wsane(int n, int site, int w) { return (w == 2 && site < n - 1) ? 2 : 1; }
 
+// block sum of two doubles per lane; red[0..1] valid for all lanes after return
+QMX_DEV void block_sum2(double (&v)[2], double* red) {
+  const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6, nw = blockDim.x >> 6;
+  for (int j = 0; j < 2; ++j) {
+    double s = wave_sum(v[j]);
+    if (lane == 0) red[wid * 2 + j] = s;
+  }
+  __syncthreads();
+  if (threadIdx.x < 2) {
+    double s = 0.0;
+    for (int w = 0; w < nw; ++w) s += red[w * 2 + threadIdx.x];
+    red[threadIdx.x] = s;
+  }
+  __syncthreads();
+}
+
 // step -> (layer, site) of the sweep order (reverse circuit order)
 QMX_DEV void step_gate(int n, int s, int& layer, int& site) {
   layer = s / n;
@@ -218,18 +225,49 @@ template <typename S> struct StoreC;
 template <> struct StoreC<float2> { using C = float; };
 template <> struct StoreC<double2> { using C = double; };
 
-template <typename S>
-__global__ __launch_bounds__(256) void window_pass_kernel(WinPassArgs<S> a) {
+// deposit of the low bits of x into the set bits of mask (32-bit)
+QMX_DEV uint32_t dep32(uint32_t x, uint32_t mask) {
+  uint32_t r = 0;
+  while (mask) {
+    uint32_t low = mask & (~mask + 1u);
+    if (x & 1u) r |= low;
+    x >>= 1;
+    mask &= mask - 1u;
+  }
+  return r;
+}
+
+// offset of group element x from the 4 positions p[] of its set
+QMX_DEV uint32_t xoffs(const int* p, int x) {
+  return (((uint32_t)x & 1u) << p[0]) | ((((uint32_t)x >> 1) & 1u) << p[1]) |
+         ((((uint32_t)x >> 2) & 1u) << p[2]) | ((((uint32_t)x >> 3) & 1u) << p[3]);
+}
+
+QMX_DEV uint32_t group_roffg(const GroupMap& m, int g) { return (uint32_t)wdeposit((uint64_t)g, m.restg); }
+QMX_DEV int group_roffl(const GroupMap& m, int g) { return (int)dep32((uint32_t)g, m.restl); }
+
+// gate code of (site, width) inside a 4-bit group set: hb * 2 + (width == 2)
+QMX_DEV int gate_code(uint64_t mask, int n, int site, int w) {
+  int hb = __popcll(mask & ((1ull << (n - 1 - site)) - 1ull));
+  return hb * 2 + (w == 2 ? 1 : 0);
+}
+
+// complex64 fits 4 waves per SIMD without spills; complex128 runs at 2 (a
+// 4-wave budget would spill its FP64 groups)
+template <typename S, int WPE>
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(WPE))) void window_pass_kernel(
+    WinPassArgs<S> a) {
   using C = typename StoreC<S>::C;
   using T = CT<C>;
   using c2 = typename T::c2;
   extern __shared__ char wsm[];
-  __shared__ WGeom g;
   __shared__ c2 gmat[2 * kMaxWindow][16];
   __shared__ int gcode[2 * kMaxWindow];
-  __shared__ int xoffp[16];          // group offsets over U_prev
-  __shared__ int restp[kWinTileBits];
-  __shared__ int s_np, s_nc;
+  // group element offsets, [side][x]: side 0 = T' over the previous set,
+  // side 1 = phi over the current set; global (state) and tile-local
+  __shared__ uint32_t xog[2][16];
+  __shared__ int xol[2][16];
+  __shared__ int rdep[8];
   const int b = blockIdx.y;
   const int n = a.n;
   if (a.skip && a.skip[b]) return;
@@ -237,59 +275,24 @@ __global__ __launch_bounds__(256) void window_pass_kernel(WinPassArgs<S> a) {
   if (a.s0 >= kb) return;
   const int nc = min(a.s1, kb) - a.s0;
   const int np = a.p1 - a.p0;   // previous window (all valid: p1 == s0 < kb)
-  if (threadIdx.x == 0) {
-    uint64_t pm = a.umask | a.pmask;
-    int t = a.tbits < n ? a.tbits : n;
-    uint64_t full = (1ull << n) - 1ull;
-    for (int bit = 0; bit < n && __popcll(pm) < t; ++bit) pm |= 1ull << bit;
-    g.t = t;
-    int c0 = 0;
-    while (c0 < n && ((pm >> c0) & 1ull)) ++c0;
-    g.c0 = c0;
-    uint64_t high = pm & ~((1ull << c0) - 1ull);
-    int nh = __popcll(high);
-    if (nh > kMaxHighBits) nh = kMaxHighBits;
-    g.nh = nh;
-    g.qmask = full & ~pm;
-    for (int h = 0; h < (1 << nh); ++h) g.hoffs[h] = (int)wdeposit((uint64_t)h, high);
-    // group index spaces: U (phi, R) and U_prev (T')
-    for (int side = 0; side < 2; ++side) {
-      uint64_t um = side == 0 ? a.umask : a.pmask;
-      int ux[4];
-      int m = 0;
-      for (int bit = 0; bit < n; ++bit)
-        if ((um >> bit) & 1ull) ux[m++] = wlocal(pm, bit);
-      int* xo = side == 0 ? g.xoff : xoffp;
-      int* rp = side == 0 ? g.restpos : restp;
-      for (int x = 0; x < 16; ++x) {
-        int off = 0;
-        for (int q = 0; q < 4; ++q)
-          if ((x >> q) & 1) off |= 1 << ux[q];
-        xo[x] = off;
-      }
-      int r = 0;
-      for (int l = 0; l < t; ++l) {
-        bool inu = false;
-        for (int q = 0; q < 4; ++q) inu |= (ux[q] == l);
-        if (!inu) rp[r++] = l;
-      }
-    }
-    for (int j = 0; j < np + nc; ++j) {
-      int s = j < np ? a.p0 + j : a.s0 + (j - np);
-      int layer, site;
-      step_gate(n, s, layer, site);
-      size_t gi = ((size_t)b * a.lmax + layer) * n + site;
-      int w = wsane(n, site, a.widths[gi]);
-      for (int e = 0; e < 16; ++e) gmat[j][e] = T::mk(a.gates[gi * 16 + e]);
-      uint64_t um = j < np ? a.pmask : a.umask;
-      int hb = __popcll(um & ((1ull << (n - 1 - site)) - 1ull));
-      gcode[j] = hb * 2 + (w == 2 ? 1 : 0);
-    }
-    s_np = np;
-    s_nc = nc;
+  if (threadIdx.x >= 64 && threadIdx.x < 96) {
+    const int x = threadIdx.x & 15, side = (threadIdx.x >> 4) & 1;
+    xog[side][x] = side ? xoffs(a.gu.ubit, x) : xoffs(a.gp.ubit, x);
+    xol[side][x] = (int)(side ? xoffs(a.gu.ux, x) : xoffs(a.gp.ux, x));
+    if (threadIdx.x < 72) rdep[x] = group_roffl(a.gu, 4 * x);
+  }
+  if (threadIdx.x < np + nc) {
+    const int j = threadIdx.x;
+    const int s = j < np ? a.p0 + j : a.s0 + (j - np);
+    int layer, site;
+    step_gate(n, s, layer, site);
+    size_t gi = ((size_t)b * a.lmax + layer) * n + site;
+    int w = wsane(n, site, a.widths[gi]);
+    for (int e = 0; e < 16; ++e) gmat[j][e] = T::mk(a.gates[gi * 16 + e]);
+    gcode[j] = gate_code(j < np ? a.gp.mask : a.gu.mask, n, site, w);
   }
   __syncthreads();
-  const int t = g.t;
+  const int t = a.t;
   const size_t len = (size_t)1 << n;
   S* sT = reinterpret_cast<S*>(wsm);
   S* sP = sT + (1 << t);
@@ -300,79 +303,69 @@ __global__ __launch_bounds__(256) void window_pass_kernel(WinPassArgs<S> a) {
   S* phi = a.phi + (size_t)b * len;
   const long ntiles = 1l << (n - t);
   const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
-  const int nrest = t - 4;
-  const int ngroups = 1 << nrest;
-  const int kper_wave = ngroups / 4;
-  const int lowmask = (1 << g.c0) - 1;
+  const int ngroups = 1 << (t - 4);
+  // lanes [0, ngroups): T' groups over the previous set; [ngroups, 2 ngroups): phi groups
+  const int task = threadIdx.x;
+  const bool tside = task < ngroups;
+  const bool active = task < 2 * ngroups;
+  const int grp = tside ? task : task - ngroups;
+  // One code path for both sides: this lane's set (bit positions in the state
+  // and in the tile), source/destination, gate range and the point at which
+  // the group is deposited in LDS (after all previous-window gates for T',
+  // after the first current-window gate for phi).
+  const uint64_t restg = tside ? a.gp.restg : a.gu.restg;
+  const uint32_t restl = tside ? a.gp.restl : a.gu.restl;
+  const uint32_t roffg = active ? (uint32_t)wdeposit((uint64_t)grp, restg) : 0u;
+  const int roffl = active ? (int)dep32((uint32_t)grp, restl) : 0;
+  const uint32_t* xg = xog[tside ? 0 : 1];
+  const int* xl = xol[tside ? 0 : 1];
+  const S* src = tside ? tsrc : phi;
+  S* dst = tside ? tdst : phi;
+  S* lds = tside ? sT : sP;
+  const bool store_dst = tside ? !first : true;
+  const int g0 = tside ? 0 : np;
+  const int ngate = tside ? np : nc;
+  const int lds_after = tside ? np : 1;
+  // R phase: wave w takes rest indices [w kper, (w + 1) kper), 4 per k-step
+  const int kper = ngroups / 4;
+  const int xl_i = (int)xoffs(a.gu.ux, lane & 15);
+  // rest index of this lane at k-step k: r = wave kper + 4k + lane / 16, three
+  // disjoint bit fields (kper = 2^(t-6) >= 4), so its tile offset is the OR of
+  // a per-lane part and the per-step part rdep[k]
+  const int rlane = xl_i + group_roffl(a.gu, wave * kper + (lane >> 4));
   d4v acc_r = {0, 0, 0, 0}, acc_i = {0, 0, 0, 0};
   for (long tile = blockIdx.x; tile < ntiles; tile += gridDim.x) {
-    const uint32_t base = (uint32_t)wdeposit((uint64_t)tile, g.qmask);
-    // lanes [0, ngroups): T' groups; lanes [ngroups, 2 ngroups): phi groups
-    for (int task = threadIdx.x; task < 2 * ngroups; task += blockDim.x) {
-      const int grp = task < ngroups ? task : task - ngroups;
-      // T' group over U_prev: pull through the previous window's new gates
-      if (task < ngroups) {
-        int roff = 0;
-        for (int q = 0; q < nrest; ++q)
-          if ((grp >> q) & 1) roff |= 1 << restp[q];
-        c2 v[16];
+    const uint32_t gb = (uint32_t)wdeposit((uint64_t)tile, a.qmask) + roffg;
+    if (active) {
+      c2 v[16];
 #pragma unroll
-        for (int x = 0; x < 16; ++x) {
-          int l = roff + xoffp[x];
-          uint32_t ad = base + (uint32_t)(l & lowmask) + (uint32_t)g.hoffs[l >> g.c0];
-          v[x] = T::mk(to_c(tsrc[ad]));
-        }
-        for (int j = 0; j < s_np; ++j) grp_apply<C>(gcode[j], v, gmat[j]);
+      for (int x = 0; x < 16; ++x) v[x] = T::mk(to_c(src[gb + xg[x]]));
+      for (int j = 0; j <= ngate; ++j) {
+        if (j == lds_after) {
 #pragma unroll
-        for (int x = 0; x < 16; ++x) {
-          int l = roff + xoffp[x];
-          S sv = from_c<S>(make_double2(v[x].x, v[x].y));
-          sT[swz(l)] = sv;
-          if (!first) tdst[base + (uint32_t)(l & lowmask) + (uint32_t)g.hoffs[l >> g.c0]] = sv;
+          for (int x = 0; x < 16; ++x)
+            lds[swz(roffl + xl[x])] = from_c<S>(make_double2(v[x].x, v[x].y));
         }
+        if (j < ngate) grp_apply<C>(gcode[g0 + j], v, gmat[g0 + j]);
       }
-      // phi group over U: remove the window's old gates
-      else {
-        int roff = 0;
-        for (int q = 0; q < nrest; ++q)
-          if ((grp >> q) & 1) roff |= 1 << g.restpos[q];
-        c2 v[16];
+      if (store_dst) {
 #pragma unroll
-        for (int x = 0; x < 16; ++x) {
-          int l = roff + g.xoff[x];
-          v[x] = T::mk(to_c(phi[base + (uint32_t)(l & lowmask) + (uint32_t)g.hoffs[l >> g.c0]]));
-        }
-        grp_apply<C>(gcode[s_np], v, gmat[s_np]);
-#pragma unroll
-        for (int x = 0; x < 16; ++x) sP[swz(roff + g.xoff[x])] = from_c<S>(make_double2(v[x].x, v[x].y));
-        for (int j = 1; j < s_nc; ++j) grp_apply<C>(gcode[s_np + j], v, gmat[s_np + j]);
-#pragma unroll
-        for (int x = 0; x < 16; ++x) {
-          int l = roff + g.xoff[x];
-          phi[base + (uint32_t)(l & lowmask) + (uint32_t)g.hoffs[l >> g.c0]] =
-              from_c<S>(make_double2(v[x].x, v[x].y));
-        }
+        for (int x = 0; x < 16; ++x) dst[gb + xg[x]] = from_c<S>(make_double2(v[x].x, v[x].y));
       }
     }
     __syncthreads();
-    // R += conj(T') phi^T over this tile's rest indices: wave w takes a quarter
-    {
-      const int i = lane & 15, kk = lane >> 4;
-      for (int ks = 0; ks < kper_wave; ks += 4) {
-        int r = wave * kper_wave + ks + kk;
-        int roff = 0;
-        for (int q = 0; q < nrest; ++q)
-          if ((r >> q) & 1) roff |= 1 << g.restpos[q];
-        double2 tv = to_c(sT[swz(g.xoff[i] + roff)]);
-        double2 pv = to_c(sP[swz(g.xoff[i] + roff)]);
-        double ar = tv.x, ai = -tv.y;      // A = conj(T')  (row x = i, k = r)
-        double br = pv.x, bi = pv.y;       // B = phi       (k = r, column y = i)
-        // Re R += ar br - ai bi,  Im R += ar bi + ai br
-        acc_r = __builtin_amdgcn_mfma_f64_16x16x4f64(ar, br, acc_r, 0, 0, 0);
-        acc_r = __builtin_amdgcn_mfma_f64_16x16x4f64(-ai, bi, acc_r, 0, 0, 0);
-        acc_i = __builtin_amdgcn_mfma_f64_16x16x4f64(ar, bi, acc_i, 0, 0, 0);
-        acc_i = __builtin_amdgcn_mfma_f64_16x16x4f64(ai, br, acc_i, 0, 0, 0);
-      }
+    // R += conj(T') phi^T over this tile: A[x][k] = conj(T'[x, r]), B[k][y] = phi[y, r]
+#pragma unroll
+    for (int k = 0; k < 8; ++k) {
+      if (4 * k >= kper) break;
+      const int li = swz(rlane | rdep[k]);
+      double2 tv = to_c(sT[li]);
+      double2 pv = to_c(sP[li]);
+      // Re R += Re T Re phi + Im T Im phi,  Im R += Re T Im phi - Im T Re phi
+      acc_r = __builtin_amdgcn_mfma_f64_16x16x4f64(tv.x, pv.x, acc_r, 0, 0, 0);
+      acc_r = __builtin_amdgcn_mfma_f64_16x16x4f64(tv.y, pv.y, acc_r, 0, 0, 0);
+      acc_i = __builtin_amdgcn_mfma_f64_16x16x4f64(tv.x, pv.y, acc_i, 0, 0, 0);
+      acc_i = __builtin_amdgcn_mfma_f64_16x16x4f64(-tv.y, pv.x, acc_i, 0, 0, 0);
     }
     __syncthreads();
   }
@@ -391,6 +384,136 @@ __global__ __launch_bounds__(256) void window_pass_kernel(WinPassArgs<S> a) {
   double* pb = a.part + ((size_t)b * gridDim.x + blockIdx.x) * 512;
   for (int e = threadIdx.x; e < 512; e += blockDim.x)
     pb[e] = ((red[e] + red[512 + e]) + (red[1024 + e] + red[1536 + e]));
+}
+
+// ---- forward circuit pass ------------------------------------------------
+// Applies the gates of up to kMaxStages stages (forward circuit positions
+// f0..f1-1 each; gate (layer L_b - 1 - f / N, site f % N)) to every state.
+// Stage 0 reads HBM, the last stage writes HBM (and accumulates
+// sum conj(ref) * psi when ref is given), intermediate stages exchange
+// through the LDS tile.
+template <typename S>
+__global__ __launch_bounds__(256) void fwd_pass_kernel(FwdArgs<S> a) {
+  using C = typename StoreC<S>::C;
+  using T = CT<C>;
+  using c2 = typename T::c2;
+  extern __shared__ char wsm[];
+  __shared__ c2 gmat[kMaxStages][kMaxWindow][16];
+  __shared__ int gcode[kMaxStages][kMaxWindow];
+  __shared__ int gcount[kMaxStages];
+  __shared__ double red[4 * 2];
+  __shared__ double scratch[256];
+  __shared__ double tot[2];
+  __shared__ int s_flag;
+  const int b = blockIdx.y;
+  const int n = a.n;
+  if (a.skip && a.skip[b]) return;
+  const int lb = a.layers[b];
+  if (threadIdx.x < kMaxStages * kMaxWindow) {
+    const int st = threadIdx.x / kMaxWindow, j = threadIdx.x % kMaxWindow;
+    if (st < a.nstage) {
+      const int f = a.f0[st] + j;
+      if (f < a.f1[st] && f < lb * n) {
+        const int layer = lb - 1 - f / n, site = f % n;
+        size_t gi = ((size_t)b * a.lmax + layer) * n + site;
+        int w = wsane(n, site, a.widths[gi]);
+        // grp_apply applies the adjoint of what it is given: stage G^dagger
+        for (int e = 0; e < 16; ++e) {
+          const double2 gv = a.gates[gi * 16 + (e & 3) * 4 + (e >> 2)];
+          gmat[st][j][e] = T::mk(make_double2(gv.x, -gv.y));
+        }
+        gcode[st][j] = gate_code(a.gs[st].mask, n, site, w);
+      }
+    }
+  }
+  if (threadIdx.x < kMaxStages) {
+    const int st = threadIdx.x;
+    int c = 0;
+    if (st < a.nstage) c = max(0, min(a.f1[st], lb * n) - a.f0[st]);
+    gcount[st] = c;
+  }
+  __syncthreads();
+  int total = 0;
+  for (int st = 0; st < a.nstage; ++st) total += gcount[st];
+  if (total == 0 && a.ref == nullptr) return;
+  const int t = a.t;
+  const size_t len = (size_t)1 << n;
+  S* lds = reinterpret_cast<S*>(wsm);
+  S* psi = a.psi + (size_t)b * len;
+  const double2* ref = a.ref ? a.ref + (size_t)b * len : nullptr;
+  const long ntiles = 1l << (n - t);
+  const int ngroups = 1 << (t - 4);
+  double acc[2] = {0.0, 0.0};
+  for (long tile = blockIdx.x; tile < ntiles; tile += gridDim.x) {
+    const uint32_t base = (uint32_t)wdeposit((uint64_t)tile, a.qmask);
+    for (int st = 0; st < a.nstage; ++st) {
+      const GroupMap& gm = a.gs[st];
+      const bool last = (st + 1 == a.nstage);
+      for (int grp = threadIdx.x; grp < ngroups; grp += blockDim.x) {
+        const uint32_t roffg = group_roffg(gm, grp);
+        const int roffl = group_roffl(gm, grp);
+        c2 v[16];
+        if (st == 0) {
+#pragma unroll
+          for (int x = 0; x < 16; ++x) v[x] = T::mk(to_c(psi[base + roffg + xoffs(gm.ubit, x)]));
+        } else {
+#pragma unroll
+          for (int x = 0; x < 16; ++x) v[x] = T::mk(to_c(lds[swz(roffl + (int)xoffs(gm.ux, x))]));
+        }
+        for (int j = 0; j < gcount[st]; ++j) grp_apply<C>(gcode[st][j], v, gmat[st][j]);
+        if (last) {
+#pragma unroll
+          for (int x = 0; x < 16; ++x) {
+            const uint32_t ad = base + roffg + xoffs(gm.ubit, x);
+            S sv = from_c<S>(make_double2(v[x].x, v[x].y));
+            if (total > 0) psi[ad] = sv;
+            if (ref) {
+              double2 r = ref[ad], pv = to_c(sv);
+              acc[0] = fma(r.x, pv.x, fma(r.y, pv.y, acc[0]));
+              acc[1] = fma(r.x, pv.y, fma(-r.y, pv.x, acc[1]));
+            }
+          }
+        } else {
+#pragma unroll
+          for (int x = 0; x < 16; ++x) lds[swz(roffl + (int)xoffs(gm.ux, x))] = from_c<S>(make_double2(v[x].x, v[x].y));
+        }
+      }
+      // stage exchange, and the next tile's stage 0 must not overwrite LDS early
+      if (a.nstage > 1) __syncthreads();
+    }
+  }
+  if (ref) {
+    block_sum2(acc, red);
+    double* pb = a.dot_part + (size_t)b * gridDim.x * 2;
+    if (threadIdx.x < 2) pb[(size_t)blockIdx.x * 2 + threadIdx.x] = red[threadIdx.x];
+    __threadfence();
+    __syncthreads();
+    if (threadIdx.x == 0) {
+      int prev = atomicAdd(a.dot_counter + b, 1);
+      s_flag = (prev == (int)gridDim.x - 1) ? 1 : 0;
+    }
+    __syncthreads();
+    if (s_flag) {
+      __threadfence();
+      // fixed-order sum of the block partials
+      const int groups = blockDim.x / 2;
+      const int jv = threadIdx.x % 2, g0 = threadIdx.x / 2;
+      double s = 0.0;
+      for (int q = g0; q < (int)gridDim.x; q += groups) s += pb[(size_t)q * 2 + jv];
+      scratch[threadIdx.x] = s;
+      __syncthreads();
+      if (threadIdx.x < 2) {
+        double z = 0.0;
+        for (int gg = 0; gg < groups; ++gg) z += scratch[gg * 2 + threadIdx.x];
+        tot[threadIdx.x] = z;
+      }
+      __syncthreads();
+      if (threadIdx.x == 0) {
+        a.dot_out[(size_t)b * a.dot_stride] = make_double2(tot[0], tot[1]);
+        a.dot_counter[b] = 0;
+      }
+    }
+  }
 }
 
 // Replay the window on the 16-dim space of U (one block per state).
@@ -510,12 +633,21 @@ __global__ __launch_bounds__(256) void window_finalize_kernel(WinFinArgs a) {
 }
 
 template <typename S>
+void launch_fwd_pass(const FwdArgs<S>& a, int nblk, int batch, hipStream_t st) {
+  size_t shm = sizeof(S) << a.t;
+  hipLaunchKernelGGL(fwd_pass_kernel<S>, dim3(nblk, batch), dim3(256), shm, st, a);
+}
+template void launch_fwd_pass<float2>(const FwdArgs<float2>&, int, int, hipStream_t);
+template void launch_fwd_pass<double2>(const FwdArgs<double2>&, int, int, hipStream_t);
+
+template <typename S>
 void launch_window_pass(const WinPassArgs<S>& a, int nblk, int batch, hipStream_t st) {
-  int t = a.tbits < a.n ? a.tbits : a.n;
+  int t = a.t;
   size_t shm = 2 * (sizeof(S) << t);
   const size_t red = 4 * 512 * sizeof(double);
   if (shm < red) shm = red;
-  hipLaunchKernelGGL(window_pass_kernel<S>, dim3(nblk, batch), dim3(256), shm, st, a);
+  constexpr int wpe = sizeof(S) == 8 ? 4 : 2;
+  hipLaunchKernelGGL((window_pass_kernel<S, wpe>), dim3(nblk, batch), dim3(256), shm, st, a);
 }
 template void launch_window_pass<float2>(const WinPassArgs<float2>&, int, int, hipStream_t);
 template void launch_window_pass<double2>(const WinPassArgs<double2>&, int, int, hipStream_t);
