@@ -7,10 +7,15 @@ A step is one whole prepare_state (chi-truncation, layer extraction, 30
 sweeps, final fidelity) of one batch of such states on the GPU.
 
 Prints one JSON line (rank 0).  ``roofline`` describes the dominant kernel,
-the sweep gate step: algorithmic bytes per launch = (read phi, read T',
-write phi, write T') = 32 B per complex64 amplitude per state (24 B on the
-first step of a sweep, which reads the target and writes no T'), divided by
-the average launch duration measured with HIP events on the library's stream.
+the windowed sweep's streaming pass (window_pass_kernel): per launch it reads
+T' and phi and writes both back, 32 B per complex64 amplitude per state (24 B
+for the first window of a sweep, which reads the target and writes no T'),
+so algorithmic bytes per launch = B * 2^N * (32 (W - 1) + 24) / W averaged
+over the W windows of a sweep.  It is divided by the average launch duration,
+measured with HIP events recorded by the library on the stream the kernel is
+launched on, launch by launch, over the timed steps.  ``traffic`` is filled
+from rocprofv3 FETCH_SIZE / WRITE_SIZE counters when --traffic-kib gives them
+(KiB per dispatch, already calibrated against a plain copy).
 ``cpu_baseline`` times the FP64 NumPy oracle (a port of the reference
 algorithm) on one state: chi-truncation + all layers + one sweep, with the
 other sweeps scaled from the timed one.
@@ -27,6 +32,11 @@ import time
 import numpy as np
 
 HBM_PEAK_GBS = 8000.0
+# Calibrated HBM traffic of one window_pass_kernel dispatch on this workload,
+# from rocprofv3 --pmc FETCH_SIZE and WRITE_SIZE runs (FETCH_SIZE x 2 and
+# WRITE_SIZE x 1, the factors a plain 1 GiB copy reads on the same counters):
+# batch -> (read KiB, write KiB).  Summaries: profiles/window_pass_n20_*.md.
+MEASURED_TRAFFIC_KIB = {32: (526760.0, 529176.0)}
 WORKLOAD = {"num_qubits": 20, "bond_dimension": 64, "num_layers": 10, "num_sweeps": 30,
             "generator_bond": 96}
 
@@ -44,6 +54,16 @@ def _states(n: int, bond: int, count: int, seed0: int) -> np.ndarray:
         psi = psi.reshape(-1)
         rng_states.append((psi / np.linalg.norm(psi)).astype(np.complex64))
     return np.stack(rng_states)
+
+
+def _blas_threads() -> int:
+    """Threads the oracle's BLAS calls use (NumPy's own loops are serial)."""
+    try:
+        from threadpoolctl import threadpool_info
+        n = [i.get("num_threads", 1) for i in threadpool_info() if i.get("user_api") == "blas"]
+        return int(max(n)) if n else 1
+    except Exception:
+        return 1
 
 
 def cpu_baseline(sv: np.ndarray, wl: dict, gpu_fid_trace=None) -> dict:
@@ -65,7 +85,7 @@ def cpu_baseline(sv: np.ndarray, wl: dict, gpu_fid_trace=None) -> dict:
     f1 = abs(np.vdot(psi, O.circuit_state(n, gl)))
     per_state = (t1 - t0) + (t2 - t1) + S * (t4 - t3)
     out = {"value": 1.0 / per_state, "unit": "state-preps/s",
-           "cores": len(os.sched_getaffinity(0)), "kind": "port",
+           "cores": _blas_threads(), "kind": "port",
            "sample": (f"1 state, FP64 NumPy oracle: chi-truncation {t1 - t0:.1f}s + {L} layers "
                       f"{t2 - t1:.1f}s + 1 of {S} sweeps {t4 - t3:.1f}s timed; "
                       f"per-state time = chi + layers + {S} x sweep")}
@@ -81,8 +101,15 @@ def main() -> int:
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=3)
     ap.add_argument("--warmup", type=int, default=1)
-    ap.add_argument("--batch", type=int, default=16)
+    ap.add_argument("--batch", type=int, default=64)
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--traffic-kib", type=float, nargs=2, default=None,
+                    metavar=("FETCH", "WRITE"),
+                    help="calibrated rocprofv3 FETCH_SIZE and WRITE_SIZE of one "
+                         "window pass dispatch, KiB")
+    ap.add_argument("--copy-calibration", action="store_true",
+                    help="before the run, copy a 1 GiB buffer 3 times (a plain copy "
+                         "kernel of known size for calibrating HBM counters)")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py runs on one GPU only: this encoder shards across GPUs by state "
@@ -98,6 +125,13 @@ def main() -> int:
     B = args.batch
     svs = _states(n, wl["generator_bond"], B, seed0=1000)
     dev = _lib.require_gpu()
+    if args.copy_calibration:
+        src = torch.empty(1 << 30, dtype=torch.uint8, device=dev)
+        dst = torch.empty_like(src)
+        for _ in range(3):
+            dst.copy_(src)
+        torch.cuda.synchronize()
+        del src, dst
     sv_dev = engine.as_device_states(svs)
     ws = None
 
@@ -127,17 +161,22 @@ def main() -> int:
 
     fid = res.fidelity.abs().cpu().numpy()
     layers_used = res.layers_used.cpu().numpy()
-    # dominant kernel: sweep gate step
-    sweep_ms, sweep_launches = prof["sweep"]
-    avg_launch_s = sweep_ms / 1000.0 / max(sweep_launches, 1)
-    k_per_sweep = int(layers_used.max()) * n
-    amp_bytes = (32.0 * (k_per_sweep - 1) + 24.0) / k_per_sweep
+    # dominant kernel: the windowed sweep's streaming pass
+    wp_ms, wp_launches = prof["window_pass"]
+    avg_launch_s = wp_ms / 1000.0 / max(wp_launches, 1)
+    windows = max(1, round(wp_launches / (args.steps * S)))
+    amp_bytes = (32.0 * (windows - 1) + 24.0) / windows
     bytes_per_launch = B * (2 ** n) * amp_bytes
     achieved = bytes_per_launch / avg_launch_s / 1e9
+    traffic = None
+    kib = args.traffic_kib or MEASURED_TRAFFIC_KIB.get(B)
+    if kib:
+        traffic = int((kib[0] + kib[1]) * 1024)
     roofline = {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS,
-                "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None,
-                "kernel": "sweep_step_kernel<float2>",
+                "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": traffic,
+                "kernel": "window_pass_kernel<float2, 4>",
                 "avg_launch_us": round(avg_launch_s * 1e6, 2),
+                "launches": int(wp_launches), "windows_per_sweep": windows,
                 "bytes_per_launch": int(bytes_per_launch)}
     phases = {p: {"ms": round(v[0] / args.steps, 2), "launches": int(v[1] / args.steps)}
               for p, v in prof.items()}

@@ -66,8 +66,11 @@ int qmx_set_option(qmx_handle* h, int option, int value);
 #define QMX_PHASE_CHI 0      /* statevector -> chi-MPS chain            */
 #define QMX_PHASE_LAYERS 1   /* layer extraction loop                    */
 #define QMX_PHASE_REBUILD 2  /* circuit state C|0..0> before each sweep  */
-#define QMX_PHASE_SWEEP 3    /* sweep gate steps (sweep_step kernels)    */
-#define QMX_PHASE_COUNT 4
+#define QMX_PHASE_SWEEP 3    /* sweep gate steps (all sweep kernels)     */
+#define QMX_PHASE_WINDOW_PASS 4  /* the windowed sweep's streaming pass
+                                    kernel alone, timed launch by launch
+                                    (also counted in QMX_PHASE_SWEEP)      */
+#define QMX_PHASE_COUNT 5
 int qmx_profile_enable(qmx_handle* h, int enable);
 int qmx_profile_read(qmx_handle* h, double* ms, long long* launches);
 

@@ -26,7 +26,7 @@ QMX_ERR_BASIS = 6
 QMX_C64 = 0
 QMX_C128 = 1
 
-PHASES = ("chi", "layers", "rebuild", "sweep")
+PHASES = ("chi", "layers", "rebuild", "sweep", "window_pass")
 
 QMX_OPT_SWEEP_MODE = 1
 SWEEP_MODES = {"auto": 0, "step": 1, "window": 2}

@@ -668,7 +668,9 @@ int run_sweep(qmx_handle* h, int n, int batch, int lmax, const int* layers_used,
         pa.layers = layers_used;
         pa.skip = skip;
         pa.part = L.part;
+        PhaseTimer pt(h, st, QMX_PHASE_WINDOW_PASS);
         launch_window_pass(pa, wblk, batch, st);
+        pt.stop(1);
         WinFinArgs fa{};
         fa.n = n;
         fa.lmax = lmax;
@@ -899,6 +901,7 @@ int qmx_extract_layers(qmx_handle* h, int num_qubits, int batch, int num_layers,
 
 size_t qmx_sweep_workspace(int num_qubits, int batch, int storage_dtype) {
   if (num_qubits < 2 || num_qubits > kMaxQubits || batch < 1) return 0;
+  if (storage_dtype != QMX_C64 && storage_dtype != QMX_C128) return 0;
   Carver c(nullptr);
   if (storage_dtype == QMX_C64) sweep_layout<float2>(c, num_qubits, batch);
   else sweep_layout<double2>(c, num_qubits, batch);
@@ -966,7 +969,9 @@ EncodeLayout encode_layout(Carver& c, const qmx_encode_config& g) {
 
 size_t qmx_encode_workspace(const qmx_encode_config* cfg) {
   if (!cfg || cfg->num_qubits < 2 || cfg->num_qubits > kMaxQubits || cfg->batch < 1 ||
-      cfg->bond_dimension < 1)
+      cfg->bond_dimension < 1 || cfg->num_layers < 1 || cfg->num_sweeps < 0 ||
+      (cfg->sv_dtype != QMX_C64 && cfg->sv_dtype != QMX_C128) ||
+      (cfg->sweep_dtype != QMX_C64 && cfg->sweep_dtype != QMX_C128))
     return 0;
   Carver c(nullptr);
   encode_layout(c, *cfg);

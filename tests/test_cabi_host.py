@@ -1,0 +1,178 @@
+"""C-ABI and host logic on the CPU (no GPU calls).
+
+* libqmx.so loads and exports every function include/qmx.h declares, and the
+  ctypes binding declares exactly those;
+* workspace queries (pure host arithmetic) accept valid sizes and reject
+  invalid ones;
+* the Python API validates its arguments like the reference
+  (sequential.py:120-153, 594-595; mps.py:151-216; base.py:58-106) before any
+  device work, and the gate-array <-> UnitaryLayer conversion round-trips.
+"""
+
+from __future__ import annotations
+
+import ctypes
+import os
+import re
+
+import numpy as np
+import pytest
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+HEADER = os.path.join(ROOT, "include", "qmx.h")
+
+
+def _declared():
+    text = open(HEADER).read()
+    text = re.sub(r"/\*.*?\*/", "", text, flags=re.S)
+    return sorted(set(re.findall(r"^\s*(?:const\s+)?\w[\w\s\*]*?\b(qmx_\w+)\s*\(", text, flags=re.M)))
+
+
+def _lib():
+    from qmprs_amd import _lib
+    if not _lib.LIB_PATH.exists():
+        pytest.skip("libqmx.so not built")
+    return _lib, _lib.load()
+
+
+def test_header_declares_the_entry_points():
+    names = _declared()
+    for n in ("qmx_create", "qmx_destroy", "qmx_last_error", "qmx_chi_truncate",
+              "qmx_generate_layer", "qmx_apply_layer", "qmx_extract_layers", "qmx_sweep",
+              "qmx_encode", "qmx_encode_workspace"):
+        assert n in names
+
+
+def test_library_exports_every_declared_symbol():
+    _, lib = _lib()
+    for name in _declared():
+        assert hasattr(lib, name), name
+
+
+def test_binding_matches_header():
+    mod, _ = _lib()
+    assert sorted(mod.SIGNATURES) == _declared()
+
+
+def test_version_and_workspace_queries():
+    mod, lib = _lib()
+    assert lib.qmx_version() >= 1
+    assert lib.qmx_chi_truncate_workspace(20, 4, 64) > 0
+    assert lib.qmx_chi_truncate_workspace(0, 4, 64) == 0
+    assert lib.qmx_layer_workspace(12, 2) > 0
+    assert lib.qmx_extract_layers_workspace(12, 2) > 0
+    assert lib.qmx_sweep_workspace(20, 8, mod.QMX_C64) > 0
+    assert lib.qmx_sweep_workspace(20, 8, 7) == 0
+    cfg = mod.EncodeConfig(20, 8, 64, 10, 30, mod.QMX_C64, mod.QMX_C64, 1 - 1e-6, 1e-10)
+    big = lib.qmx_encode_workspace(ctypes.byref(cfg))
+    assert big > 0
+    cfg.batch = 16
+    assert lib.qmx_encode_workspace(ctypes.byref(cfg)) > big
+    cfg.num_layers = 0
+    assert lib.qmx_encode_workspace(ctypes.byref(cfg)) == 0
+
+
+def test_encode_config_layout_matches_c_struct():
+    mod, _ = _lib()
+    # 7 ints then 2 doubles: doubles start at offset 32 (8-byte aligned)
+    assert mod.EncodeConfig.fidelity_threshold.offset == 32
+    assert ctypes.sizeof(mod.EncodeConfig) == 48
+
+
+def test_null_handle_calls_are_rejected():
+    mod, lib = _lib()
+    assert lib.qmx_destroy(None) == mod.QMX_OK
+    assert lib.qmx_set_option(None, 1, 0) == mod.QMX_ERR_INVALID
+    assert lib.qmx_create(0, None) == mod.QMX_ERR_INVALID
+
+
+# ---------------------------------------------------------- host logic ---
+
+def test_fidelity_threshold_property():
+    from qmprs_amd.circuit import GateCircuit
+    from qmprs_amd.synthesis.mps_encoding import Sequential
+    enc = Sequential(GateCircuit)
+    assert enc.fidelity_threshold == 1 - 1e-6
+    enc.fidelity_threshold = 0.5
+    assert enc.fidelity_threshold == 0.5
+    for bad in (-0.1, 1.5, "x"):
+        with pytest.raises(ValueError):
+            enc.fidelity_threshold = bad
+
+
+def test_prepare_mps_argument_validation():
+    from qmprs_amd.circuit import GateCircuit
+    from qmprs_amd.synthesis.mps_encoding import Sequential
+    enc = Sequential(GateCircuit)
+    for kw in ({"num_layers": 0}, {"num_layers": 1.5}, {"num_layers": 2, "num_sweeps": -1}):
+        with pytest.raises(ValueError):
+            enc.prepare_mps(object(), **kw)
+
+
+def test_prepare_state_argument_validation():
+    from qmprs_amd.circuit import GateCircuit
+    from qmprs_amd.synthesis.mps_encoding import Sequential
+    enc = Sequential(GateCircuit)
+    sv = np.ones(8) / np.sqrt(8)
+    with pytest.raises(ValueError):
+        enc.prepare_state(sv, 4, index_type="snake")
+    with pytest.raises(ValueError):
+        enc.prepare_state(sv, 4, compression_percentage=10.0)
+
+
+def test_mps_argument_validation():
+    from qmprs_amd.primitives import MPS
+    for bad in (0, -3, 2.5, True):
+        with pytest.raises(ValueError):
+            MPS(statevector=np.ones(4), bond_dimension=bad)
+    with pytest.raises(ValueError):
+        MPS()
+    with pytest.raises(ValueError):
+        MPS(statevector=np.ones(4), mps=[np.ones((1, 2, 1))] * 2)
+    with pytest.raises(TypeError):
+        MPS(mps=5)
+    with pytest.raises(ValueError):
+        MPS(mps=[np.ones((1, 2, 1))])
+
+
+def test_statevector_length_checks():
+    from qmprs_amd import engine
+    assert engine.num_qubits_of(1024) == 10
+    for bad in (0, 1, 3, 1000):
+        with pytest.raises(ValueError):
+            engine.num_qubits_of(bad)
+
+
+def test_gate_arrays_round_trip():
+    """(L, N, 4, 4) extraction-order arrays <-> circuit-order UnitaryLayers,
+    with a block ending at every width-1 gate (mps.py:816-834)."""
+    from qmprs_amd import engine
+    rng = np.random.default_rng(0)
+    n, L = 6, 3
+    gates = rng.standard_normal((L, n, 4, 4)) + 1j * rng.standard_normal((L, n, 4, 4))
+    widths = np.full((L, n), 2, dtype=np.int32)
+    widths[:, n - 1] = 1
+    widths[1, 2] = 1
+    for l in range(L):
+        for s in range(n):
+            if widths[l, s] == 1:
+                gates[l, s, 2:, :] = 0
+                gates[l, s, :, 2:] = 0
+    layers = engine.gates_to_layers(gates, widths, L)
+    assert [(s, e) for s, e, _ in layers[-2]] == [(0, 2), (3, 5)]   # layer 1, reversed order
+    g2, w2 = engine.layers_to_arrays(layers, n)
+    np.testing.assert_array_equal(w2, widths)
+    np.testing.assert_allclose(g2, gates)
+
+
+def test_gate_circuit_rejects_bad_input():
+    from qmprs_amd.circuit import GateCircuit
+    c = GateCircuit(3)
+    with pytest.raises(ValueError):
+        c.unitary(np.eye(4), [0, 0])
+    with pytest.raises(ValueError):
+        c.unitary(np.eye(2), [0, 1])
+    with pytest.raises(ValueError):
+        c.unitary(np.eye(2), 3)
+    with pytest.raises(ValueError):
+        GateCircuit(0)

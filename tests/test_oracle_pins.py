@@ -1,0 +1,441 @@
+"""CPU pins of the FP64 oracle.
+
+The reference's own tests hold no golden vectors (thresholds only), and the
+reference cannot run here (quimb / quick-core are absent).  Each oracle
+function is therefore pinned by something independent of its own formula:
+a library routine (scipy.linalg.null_space, scipy.linalg.polar, numpy SVD),
+brute force on small inputs (Kronecker-product operators, explicit SVD of
+every cut), a closed form (Eckart-Young), or an invariant the mathematics
+fixes (isometries, projector identities, monotone overlaps).
+"""
+
+from __future__ import annotations
+
+import numpy as np
+import pytest
+import scipy.linalg
+
+from oracle import qmprs_oracle as O
+
+
+def _rng(seed):
+    return np.random.default_rng(seed)
+
+
+def _crandn(rng, *shape):
+    return (rng.standard_normal(shape) + 1j * rng.standard_normal(shape)) / np.sqrt(2.0)
+
+
+def _haar(rng, d):
+    q, r = np.linalg.qr(_crandn(rng, d, d))
+    return q * (np.diag(r) / np.abs(np.diag(r)))
+
+
+def _full_operator(n, gate, site, width):
+    """Brute force: I_{2^site} (x) G (x) I_rest with site 0 the MSB."""
+    return np.kron(np.kron(np.eye(2 ** site), gate), np.eye(2 ** (n - site - width)))
+
+
+def _cut_singular_values(psi, n, i):
+    return np.linalg.svd(psi.reshape(2 ** i, 2 ** (n - i)), compute_uv=False)
+
+
+# ---------------------------------------------------------------- inputs ---
+
+def test_ket_normalises_and_casts():
+    v = np.arange(8) + 1j
+    k = O.ket(v)
+    assert k.dtype == np.complex128
+    assert abs(np.linalg.norm(k) - 1) < 1e-15
+    np.testing.assert_allclose(k * np.linalg.norm(v), v, rtol=1e-14)
+
+
+def test_random_uniform_state_matches_reference_generator():
+    """rng.random(2^N) + 1j rng.random(2^N), normalised (the reference test's
+    generate_random_state with a seeded generator)."""
+    rng = np.random.default_rng(5)
+    sv = rng.random(2 ** 6) + 1j * rng.random(2 ** 6)
+    np.testing.assert_array_equal(O.random_uniform_state(6, 5), sv / np.linalg.norm(sv))
+
+
+@pytest.mark.parametrize("n,bond", [(6, 3), (8, 4), (10, 8)])
+def test_random_mps_state_has_generating_bond(n, bond):
+    psi = O.random_mps_state(n, bond, seed=3)
+    assert abs(np.linalg.norm(psi) - 1) < 1e-12
+    for i in range(1, n):
+        s = _cut_singular_values(psi, n, i)
+        rank = int((s > 1e-10 * s[0]).sum())
+        assert rank == min(bond, 2 ** i, 2 ** (n - i))
+
+
+# ------------------------------------------------------ (a) sv -> MPS ---
+
+@pytest.mark.parametrize("n,seed", [(2, 0), (5, 1), (8, 2), (10, 3)])
+def test_from_dense_reconstructs_and_is_right_isometric(n, seed):
+    psi = O.random_uniform_state(n, seed)
+    t = O.mps_from_dense(psi, n)
+    np.testing.assert_allclose(O.mps_to_dense(t), psi, atol=1e-12)
+    for i in range(1, n):
+        a = t[i].reshape(t[i].shape[0], -1)
+        np.testing.assert_allclose(a @ a.conj().T, np.eye(a.shape[0]), atol=1e-12)
+
+
+@pytest.mark.parametrize("n,bond,seed", [(8, 3, 0), (9, 5, 1)])
+def test_from_dense_bonds_equal_cut_ranks(n, bond, seed):
+    """Cutoff-only SVD chain: every bond equals the rank of that cut."""
+    psi = O.random_mps_state(n, bond, seed)
+    t = O.mps_from_dense(psi, n)
+    for i in range(1, n):
+        assert t[i].shape[0] == min(bond, 2 ** i, 2 ** (n - i))
+
+
+# ---------------------------------------------- (b) chi-truncation -------
+
+def test_keep_count_rsum2_examples():
+    """rsum2: smallest k whose discarded tail <= cutoff * total; max_bond cap;
+    at least one value kept."""
+    lam = np.array([0.7, 0.2, 0.1 - 1e-12, 1e-12])
+    assert O.keep_count(lam, None, 1e-10) == 3          # tail 1e-12 <= 1e-10
+    assert O.keep_count(lam, None, 0.0) == 4            # nothing discarded
+    assert O.keep_count(lam, 2, 1e-10) == 2
+    assert O.keep_count(lam, None, 0.15) == 2           # tail 0.1 <= 0.15 * 1
+    assert O.keep_count(lam, None, 0.35) == 1
+    assert O.keep_count(np.zeros(4), None, 1e-10) >= 1
+
+
+@pytest.mark.parametrize("n,seed", [(6, 0), (9, 1)])
+def test_chi_truncate_exact_when_chi_covers_rank(n, seed):
+    psi = O.random_uniform_state(n, seed)
+    t, _ = O.chi_truncate(psi, n, 2 ** (n // 2))
+    np.testing.assert_allclose(t, psi, atol=1e-12)
+
+
+def test_chi_truncate_one_cut_is_eckart_young():
+    """Two sites (one bond): the truncated state is the best rank-chi
+    approximation, with squared error = discarded squared singular values."""
+    rng = _rng(4)
+    for chi in (1,):
+        psi = _crandn(rng, 4)
+        psi /= np.linalg.norm(psi)
+        t, spectra = O.chi_truncate(psi, 2, chi)
+        u, s, vh = np.linalg.svd(psi.reshape(2, 2))
+        best = (u[:, :chi] * s[:chi]) @ vh[:chi]
+        np.testing.assert_allclose(t.reshape(2, 2), best, atol=1e-13)
+        assert abs(np.linalg.norm(psi - t) ** 2 - (s[chi:] ** 2).sum()) < 1e-13
+
+
+@pytest.mark.parametrize("n,chi,seed", [(8, 4, 0), (10, 6, 1), (11, 16, 2)])
+def test_chi_truncate_projector_identities(n, chi, seed):
+    """The chain applies nested orthogonal projections, so T = P psi:
+    <T|psi> = ||T||^2 and ||psi - T||^2 = 1 - ||T||^2; every cut of T has
+    rank <= chi; the rightmost truncated cut keeps the exact top spectrum."""
+    psi = O.random_uniform_state(n, seed)
+    t, spectra = O.chi_truncate(psi, n, chi)
+    nt2 = np.vdot(t, t).real
+    ov = np.vdot(t, psi)
+    assert abs(ov - nt2) < 1e-12
+    assert abs(np.linalg.norm(psi - t) ** 2 - (1 - nt2)) < 1e-12
+    for i in range(1, n):
+        s = _cut_singular_values(t, n, i)
+        assert int((s > 1e-9).sum()) <= chi
+    # the first cut the chain truncates (from the right) sees psi itself
+    i = max(j for j in range(1, n) if min(2 ** j, 2 ** (n - j)) > chi)
+    s = _cut_singular_values(psi, n, i)
+    s_t = _cut_singular_values(t, n, i)
+    # after all later (leftward) projections the kept values can only shrink
+    assert np.all(s_t[:chi] <= s[:chi] + 1e-12)
+
+
+@pytest.mark.parametrize("n,chi,seed", [(7, 3, 0), (9, 5, 1)])
+def test_chi_truncate_dense_chain_equals_mps_sweep(n, chi, seed):
+    """Dense restatement vs the quimb-like MPS restatement (from_dense,
+    left-canonise, right SVD-compress sweep)."""
+    psi = O.random_mps_state(n, 8, seed)
+    t_dense, _ = O.chi_truncate(psi, n, chi)
+    m = O.mps_from_dense(psi, n)
+    m = O.mps_left_canonize(m)
+    m = O.mps_right_compress(m, chi)
+    np.testing.assert_allclose(O.mps_to_dense(m), t_dense, atol=1e-10)
+
+
+@pytest.mark.parametrize("n,seed", [(6, 0), (8, 1)])
+def test_canonize_preserves_state_and_isometries(n, seed):
+    psi = O.random_uniform_state(n, seed)
+    t = O.mps_from_dense(psi, n)
+    left = O.mps_left_canonize(t)
+    np.testing.assert_allclose(O.mps_to_dense(left), psi, atol=1e-12)
+    for i in range(n - 1):
+        a = left[i].reshape(-1, left[i].shape[2])
+        np.testing.assert_allclose(a.conj().T @ a, np.eye(a.shape[1]), atol=1e-12)
+    right = O.mps_right_canonize(left)
+    np.testing.assert_allclose(O.mps_to_dense(right), psi, atol=1e-12)
+    for i in range(1, n):
+        a = right[i].reshape(right[i].shape[0], -1)
+        np.testing.assert_allclose(a @ a.conj().T, np.eye(a.shape[0]), atol=1e-12)
+
+
+def test_canonical_basis_depends_only_on_the_subspace():
+    rng = _rng(7)
+    v = np.linalg.qr(_crandn(rng, 8, 3))[0]
+    w1 = O.canonical_basis(v)
+    w2 = O.canonical_basis(v @ _haar(rng, 3))
+    np.testing.assert_allclose(w1, w2, atol=1e-12)
+    np.testing.assert_allclose(w1.conj().T @ w1, np.eye(3), atol=1e-12)
+    np.testing.assert_allclose(w1 @ w1.conj().T, v @ v.conj().T, atol=1e-12)
+
+
+# ---------------------------------------------------- (c) Ran layers -----
+
+@pytest.mark.parametrize("rows", [1, 2])
+def test_householder_null_space_equals_scipy(rows):
+    """scipy.linalg.null_space (mps.py:597,657,730) on orthonormal rows."""
+    rng = _rng(rows)
+    worst = 0.0
+    for _ in range(300):
+        a = np.linalg.qr(_crandn(rng, 4, rows))[0].T
+        worst = max(worst, np.abs(O.householder_null_space(a)
+                                  - scipy.linalg.null_space(a.conj())).max())
+    assert worst < 1e-13
+
+
+def test_householder_null_space_real_first_entry_cases():
+    """Sign rule of the reflector on a real-leading vector and on e_1."""
+    for a in (np.array([[1.0, 0, 0, 0]], dtype=complex),
+              np.array([[-0.6, 0.8j, 0, 0]], dtype=complex),
+              np.array([[0.5, 0.5, 0.5, 0.5]], dtype=complex)):
+        np.testing.assert_allclose(O.householder_null_space(a),
+                                   scipy.linalg.null_space(a.conj()), atol=1e-14)
+
+
+@pytest.mark.parametrize("n,seed", [(2, 0), (3, 1), (6, 2), (9, 3)])
+def test_layer_maps_zero_state_onto_bond2_mps(n, seed):
+    """Ran's construction: the layer's unitaries map |0..0> exactly onto the
+    bond-2 MPS they were read from; every gate is unitary."""
+    psi = O.random_uniform_state(n, seed)
+    tensors = O.bond2_mps(psi, n)
+    layer = O.generate_layer(tensors)
+    for _, _, gates in layer:
+        for g in gates:
+            np.testing.assert_allclose(g @ g.conj().T, np.eye(g.shape[0]), atol=1e-12)
+    got = O.circuit_state(n, O.layer_gate_list([layer]))
+    np.testing.assert_allclose(got, O.mps_to_dense(tensors), atol=1e-12)
+
+
+@pytest.mark.parametrize("n,seed", [(5, 0), (8, 1)])
+def test_first_layer_overlap_is_gauge_invariant(n, seed):
+    """|<0|U^dagger psi>| = |<psi2|psi>| / ||psi2|| for either bond gauge."""
+    psi = O.random_uniform_state(n, seed)
+    amps = []
+    for gauge in ("canonical", "lapack"):
+        layer = O.generate_layer(O.bond2_mps(psi, n, gauge=gauge))
+        amps.append(abs(O.apply_inverse_layer_dense(psi, n, layer)[0]))
+    t2 = O.truncation_chain(psi, n, 2)[0]
+    psi2 = O.mps_to_dense(t2)
+    expect = abs(np.vdot(psi2, psi)) / np.linalg.norm(psi2)
+    assert abs(amps[0] - expect) < 1e-12
+    assert abs(amps[1] - expect) < 1e-12
+
+
+@pytest.mark.parametrize("n,seed", [(5, 0), (7, 1)])
+def test_inverse_layer_dense_equals_mps_gate_split(n, seed):
+    psi = O.random_uniform_state(n, seed)
+    layer = O.generate_layer(O.bond2_mps(psi, n))
+    dense = O.apply_inverse_layer_dense(psi, n, layer)
+    t = O.mps_from_dense(psi, n)
+    t = O.apply_inverse_layer_mps(t, layer)
+    np.testing.assert_allclose(O.mps_to_dense(t), dense, atol=1e-11)
+
+
+def test_submps_indices_blocks():
+    assert O.submps_indices([2, 2, 2]) == [(0, 3)]
+    assert O.submps_indices([1, 1, 1]) == [(0, 0), (1, 1), (2, 2), (3, 3)]
+    assert O.submps_indices([2, 1, 2, 2, 1]) == [(0, 1), (2, 4), (5, 5)]
+
+
+def test_partial_entanglement_layer_blocks():
+    """H(0) CX(0,3) H(4) H(5) CX(5,7) (reference scenario): one layer is exact.
+    Qubit q is site 7 - q: the Bell pair (0, 3) spans sites 4..7 and (5, 7)
+    spans sites 0..2 (the chain carries it through the sites between), and
+    qubit 4 (site 3) is a block of its own."""
+    from qmprs_amd.circuit import GateCircuit
+    c = GateCircuit(8)
+    c.H(0); c.CX(0, 3); c.H(4); c.H(5); c.CX(5, 7)
+    psi = c.get_statevector()
+    res = O.prepare_state(psi, 32, num_layers=1)
+    assert res["fidelity"] > 1 - 1e-10
+    layer = res["layers"][0]
+    assert [(s, e) for s, e, _ in layer] == [(0, 2), (3, 3), (4, 7)]
+
+
+# ----------------------------------------------------- (d)/(e) dense ops --
+
+@pytest.mark.parametrize("width", [1, 2])
+def test_apply_gate_equals_kronecker_operator(width):
+    rng = _rng(11 + width)
+    n = 5
+    psi = _crandn(rng, 2 ** n)
+    g = _haar(rng, 2 ** width)
+    for site in range(n - width + 1):
+        np.testing.assert_allclose(O.apply_gate(psi, n, g, site, width),
+                                   _full_operator(n, g, site, width) @ psi, atol=1e-13)
+
+
+@pytest.mark.parametrize("width", [1, 2])
+def test_environment_is_overlap_derivative(width):
+    """E[o, i] = <T'| (|o><i| at the gate sites) |phi> by brute force."""
+    rng = _rng(21 + width)
+    n = 5
+    d = 2 ** width
+    tp, phi = _crandn(rng, 2 ** n), _crandn(rng, 2 ** n)
+    for site in range(n - width + 1):
+        e = O.environment(tp, phi, n, site, width)
+        for o in range(d):
+            for i in range(d):
+                unit = np.zeros((d, d))
+                unit[o, i] = 1.0
+                ref = np.vdot(tp, _full_operator(n, unit, site, width) @ phi)
+                assert abs(e[o, i] - ref) < 1e-12
+
+
+def test_polar_matches_scipy_and_jacobi_rule():
+    rng = _rng(31)
+    for d in (2, 4):
+        for _ in range(50):
+            e = _crandn(rng, d, d)
+            u = scipy.linalg.polar(e)[0]
+            np.testing.assert_allclose(O.polar_unitary(e), u, atol=1e-12)
+            np.testing.assert_allclose(O.polar_unitary_jacobi(e), u, atol=1e-12)
+
+
+@pytest.mark.parametrize("rank", [0, 1, 2, 3])
+def test_polar_rank_deficient_is_unitary_and_optimal(rank):
+    """For any E the polar factor maximises Re Tr(U^H E) = nuclear norm; with
+    rank-deficient E both completions stay unitary and optimal."""
+    rng = _rng(40 + rank)
+    e = _crandn(rng, 4, rank) @ _crandn(rng, rank, 4) if rank else np.zeros((4, 4), complex)
+    nuc = np.linalg.svd(e, compute_uv=False).sum()
+    for f in (O.polar_unitary, O.polar_unitary_jacobi):
+        u = f(e)
+        np.testing.assert_allclose(u @ u.conj().T, np.eye(4), atol=1e-12)
+        assert abs(np.trace(u.conj().T @ e).real - nuc) < 1e-12
+    best_random = max(np.trace(_haar(rng, 4).conj().T @ e).real for _ in range(200))
+    assert best_random <= nuc + 1e-12
+
+
+def test_sweep_local_updates_never_decrease_overlap():
+    """Each local update maximises Re <T'|G phi> over unitaries, so the
+    overlap <T|C|0> is non-decreasing step after step and sweep after sweep."""
+    n, chi = 7, 8
+    psi = O.random_uniform_state(n, 3)
+    target, _ = O.chi_truncate(psi, n, chi)
+    gl = O.layer_gate_list(O.get_unitary_layers(target, n, 3))
+    prev = abs(np.vdot(target, O.circuit_state(n, gl)))
+    for _ in range(4):
+        phi = O.circuit_state(n, gl)
+        tp = target.copy()
+        for idx in range(len(gl) - 1, -1, -1):
+            site, width, g = gl[idx]
+            before = np.vdot(tp, O.apply_gate(phi, n, np.eye(2 ** width), site, width))
+            phi = O.apply_gate(phi, n, g.conj().T, site, width)
+            env = O.environment(tp, phi, n, site, width)
+            g_new = np.conj(O.polar_unitary(env))
+            after = np.sum(g_new * env)
+            assert after.real >= abs(before) - 1e-12 or idx == len(gl) - 1
+            tp = O.apply_gate(tp, n, g_new.conj().T, site, width)
+            gl[idx] = (site, width, g_new)
+        cur = abs(np.vdot(target, O.circuit_state(n, gl)))
+        assert cur >= prev - 1e-12
+        prev = cur
+
+
+def test_sweep_overlap_trace_monotone_and_equals_state_overlap():
+    n = 6
+    psi = O.random_uniform_state(n, 8)
+    target, _ = O.chi_truncate(psi, n, 4)
+    gl = O.layer_gate_list(O.get_unitary_layers(target, n, 2))
+    trace = []
+    out = O.optimize_gate_list(target, n, gl, 5, trace)
+    assert np.all(np.diff(np.abs(trace)) >= -1e-12)
+    assert abs(abs(trace[-1]) - abs(np.vdot(target, O.circuit_state(n, out)))) < 1e-12
+
+
+# --------------------------------------------------- circuit emission ---
+
+def test_gate_circuit_conventions():
+    """quick/qiskit conventions: qubit q is bit q of the index; CX(c, t)."""
+    from qmprs_amd.circuit import GateCircuit
+    c = GateCircuit(3)
+    c.X(0)
+    sv = c.get_statevector()
+    assert abs(sv[1] - 1) < 1e-15
+    c = GateCircuit(2)
+    c.H(0)
+    c.CX(0, 1)
+    np.testing.assert_allclose(c.get_statevector(), np.array([1, 0, 0, 1]) / np.sqrt(2), atol=1e-15)
+    c = GateCircuit(3)
+    c.X(2)
+    c.CX(2, 0)
+    assert abs(c.get_statevector()[0b101] - 1) < 1e-15
+    assert c.get_depth() == 2
+
+
+@pytest.mark.parametrize("n,seed", [(4, 0), (7, 1)])
+def test_emission_reproduces_dense_circuit_state(n, seed):
+    """Gates emitted to qubits [N-2-i, N-1-i] / N-1-i (sequential.py:173-187)
+    give the same state as the MSB-ordered dense circuit."""
+    from qmprs_amd.circuit import GateCircuit
+    from qmprs_amd.primitives.mps import Tensor
+    from qmprs_amd.synthesis.mps_encoding.sequential import Sequential
+    psi = O.random_uniform_state(n, seed)
+    target, _ = O.chi_truncate(psi, n, 2 ** n)
+    layers = O.get_unitary_layers(target, n, 2)
+    c = GateCircuit(n)
+    for layer in layers:
+        Sequential._apply_unitary_layer_to_circuit(
+            c, [(s, e, [Tensor(g) for g in gates]) for s, e, gates in layer])
+    np.testing.assert_allclose(c.get_statevector(), O.circuit_state(n, O.layer_gate_list(layers)),
+                               atol=1e-12)
+
+
+# ------------------------------------ reference integration scenarios -----
+# tests/synthesis/mps_encoding/test_sequential_encoding.py, seeded.
+
+def test_scenario_prepare_state_many_layers():
+    psi = O.random_uniform_state(8, 100)
+    res = O.prepare_state(psi, 32, num_layers=32)
+    assert 1 - res["fidelity"] < 1e-2
+
+
+def test_scenario_partial_entanglement_with_sweep_depth():
+    from qmprs_amd.circuit import GateCircuit
+    from qmprs_amd.primitives.mps import Tensor
+    from qmprs_amd.synthesis.mps_encoding.sequential import Sequential
+    c = GateCircuit(4)
+    c.H(0); c.CX(0, 1); c.CX(1, 2); c.H(3)
+    psi = c.get_statevector()
+    res = O.prepare_state(psi, 32, num_layers=1, num_sweeps=1)
+    assert 1 - res["fidelity"] < 1e-2
+    out = GateCircuit(4)
+    for layer in res["layers"]:
+        Sequential._apply_unitary_layer_to_circuit(
+            out, [(s, e, [Tensor(g) for g in gates]) for s, e, gates in layer])
+    assert abs(abs(np.vdot(out.get_statevector(), psi)) - res["fidelity"]) < 1e-12
+    assert out.get_depth() <= 7
+
+
+def test_scenario_layer_improvement():
+    psi = O.random_uniform_state(8, 101)
+    fids = [O.prepare_state(psi, 64, num_layers=k)["fidelity"] for k in range(1, 10)]
+    assert np.all(np.diff(fids) >= 0)
+
+
+def test_scenario_sweep_improvement():
+    psi = O.random_uniform_state(8, 102)
+    target, _ = O.chi_truncate(O.ket(psi), 8, 64)
+    layers = O.get_unitary_layers(target, 8, 6)
+    gl = O.layer_gate_list(layers)
+    fids = []
+    for _ in range(7):
+        gl, _ = O.sweep_once(target, 8, gl)
+        fids.append(abs(np.vdot(psi, O.circuit_state(8, gl))))
+    assert np.all(np.diff(fids) >= 0)
