@@ -170,6 +170,59 @@ int qmx_encode(qmx_handle* h, const qmx_encode_config* cfg, const void* statevec
                void* gates, int* widths, int* layers_used, void* fidelity, const double* goal,
                int* sweeps_done, void* workspace, size_t workspace_bytes, void* stream);
 
+/* ---- Learned warm start (no reference counterpart) ------------------------
+ * A 3-layer ReLU MLP (BF16 weights, FP32 accumulation on MFMA) maps a feature
+ * row per two-qubit gate slot to 15 su(4) coefficients c; the proposal is
+ * G exp(i sum_k c_k l_k) (Gell-Mann basis, tr(l_a l_b) = 2 delta_ab).  It
+ * proposes starting gates for the sweeps of Sequential._optimize_unitary_layers
+ * (sequential.py:509-541); the plain sweeps remain the fallback and the judge.
+ *
+ * Parameters (FP32 master, BF16 shadow with the same offsets):
+ *   W1[in][width], b1[width], W2[width][width], b2[width], W3[width][out], b3[out]
+ * Matrices are row-major; x is BF16 [rows][in], y FP32 [rows][out].
+ * qmx_mlp_backward must follow qmx_mlp_forward on the same x, parameters and
+ * workspace (it reads the activations the forward pass stored there) and
+ * writes the gradient of sum(dy * y) for every parameter into grads. */
+size_t qmx_mlp_param_count(int in_dim, int width, int out_dim);
+size_t qmx_mlp_workspace(int rows, int in_dim, int width, int out_dim);
+int qmx_mlp_forward(qmx_handle* h, int rows, int in_dim, int width, int out_dim, const void* x_bf16,
+                    const float* params, const void* params_bf16, float* y, void* workspace,
+                    size_t workspace_bytes, void* stream);
+int qmx_mlp_backward(qmx_handle* h, int rows, int in_dim, int width, int out_dim, const void* x_bf16,
+                     const float* params, const void* params_bf16, const float* dy, float* grads,
+                     void* workspace, size_t workspace_bytes, void* stream);
+/* Adam with bias correction (step >= 1); refreshes the BF16 shadow if given. */
+int qmx_mlp_adam(qmx_handle* h, size_t count, float* params, void* params_bf16, const float* grads,
+                 float* m, float* v, float lr, float beta1, float beta2, float eps, int step,
+                 void* stream);
+int qmx_mlp_to_bf16(qmx_handle* h, size_t count, const float* src, void* dst, void* stream);
+/* Weighted MSE over the first `valid` of `ld` columns: loss (device float) =
+ * inv_norm * sum_r w_r sum_c (y - t)^2, dy = d loss / d y (w may be NULL). */
+size_t qmx_mse_workspace(void);
+int qmx_mse_loss(qmx_handle* h, int rows, int ld, int valid, const float* y, const float* t,
+                 const float* w, float inv_norm, float* dy, float* loss, void* workspace,
+                 size_t workspace_bytes, void* stream);
+
+/* Gate slot rows: row = (b * num_layers + layer) * (N - 1) + site for every
+ * site < N - 1; a row is live when layer < layers_used[b] and the slot holds a
+ * two-qubit gate (widths == 2).  Gates use the qmx gate layout above.
+ *   qmx_gate_features: BF16 rows of width ld >= 50:
+ *       [c(Ga^H Gb) (15), Re/Im of Gb row-major (32), layer / L, site / N, 1, 0 ...]
+ *       row_weight (device or NULL) = 1 for live rows, 0 otherwise
+ *   qmx_gate_log:      FP32 rows of width ld >= 15: c(Ga^H Gb), the traceless
+ *                      principal logarithm (eigenphases in (-pi, pi])
+ *   qmx_gate_expmap:   G <- G exp(i sum_k c_k l_k) on live rows of states with
+ *                      skip[b] == 0 (skip may be NULL) */
+int qmx_gate_features(qmx_handle* h, int num_qubits, int batch, int num_layers, const int* layers_used,
+                      const void* gates_a, const void* gates_b, const int* widths, void* features,
+                      int ld, float* row_weight, void* stream);
+int qmx_gate_log(qmx_handle* h, int num_qubits, int batch, int num_layers, const int* layers_used,
+                 const void* gates_a, const void* gates_b, const int* widths, float* coeffs, int ld,
+                 void* stream);
+int qmx_gate_expmap(qmx_handle* h, int num_qubits, int batch, int num_layers, const int* layers_used,
+                    const float* coeffs, int ld, void* gates, const int* widths, const int* skip,
+                    void* stream);
+
 #ifdef __cplusplus
 }
 #endif

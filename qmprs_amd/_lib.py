@@ -34,6 +34,7 @@ SWEEP_MODES = {"auto": 0, "step": 1, "window": 2}
 _VP = ctypes.c_void_p
 _INT = ctypes.c_int
 _DBL = ctypes.c_double
+_FLT = ctypes.c_float
 _SZ = ctypes.c_size_t
 
 
@@ -75,6 +76,17 @@ SIGNATURES = {
     "qmx_encode_workspace": (_SZ, [ctypes.POINTER(EncodeConfig)]),
     "qmx_encode": (_INT, [_VP, ctypes.POINTER(EncodeConfig), _VP, _VP, _VP, _VP, _VP, _VP, _VP, _VP,
                           _SZ, _VP]),
+    "qmx_mlp_param_count": (_SZ, [_INT, _INT, _INT]),
+    "qmx_mlp_workspace": (_SZ, [_INT, _INT, _INT, _INT]),
+    "qmx_mlp_forward": (_INT, [_VP, _INT, _INT, _INT, _INT, _VP, _VP, _VP, _VP, _VP, _SZ, _VP]),
+    "qmx_mlp_backward": (_INT, [_VP, _INT, _INT, _INT, _INT, _VP, _VP, _VP, _VP, _VP, _VP, _SZ, _VP]),
+    "qmx_mlp_adam": (_INT, [_VP, _SZ, _VP, _VP, _VP, _VP, _VP, _FLT, _FLT, _FLT, _FLT, _INT, _VP]),
+    "qmx_mlp_to_bf16": (_INT, [_VP, _SZ, _VP, _VP, _VP]),
+    "qmx_mse_workspace": (_SZ, []),
+    "qmx_mse_loss": (_INT, [_VP, _INT, _INT, _INT, _VP, _VP, _VP, _FLT, _VP, _VP, _VP, _SZ, _VP]),
+    "qmx_gate_features": (_INT, [_VP, _INT, _INT, _INT, _VP, _VP, _VP, _VP, _VP, _INT, _VP, _VP]),
+    "qmx_gate_log": (_INT, [_VP, _INT, _INT, _INT, _VP, _VP, _VP, _VP, _VP, _INT, _VP]),
+    "qmx_gate_expmap": (_INT, [_VP, _INT, _INT, _INT, _VP, _VP, _INT, _VP, _VP, _VP, _VP]),
 }
 
 _lib = None

@@ -729,6 +729,61 @@ int run_sweep(qmx_handle* h, int n, int batch, int lmax, const int* layers_used,
   return QMX_OK;
 }
 
+// ---- learned warm start: MLP parameter and activation layout -------------
+struct MlpDims {
+  int in, w, out;
+  size_t oW1() const { return 0; }
+  size_t ob1() const { return oW1() + (size_t)in * w; }
+  size_t oW2() const { return ob1() + w; }
+  size_t ob2() const { return oW2() + (size_t)w * w; }
+  size_t oW3() const { return ob2() + w; }
+  size_t ob3() const { return oW3() + (size_t)w * out; }
+  size_t count() const { return ob3() + out; }
+};
+
+bool mlp_dims_ok(int rows, int in, int w, int out) {
+  return rows >= 1 && in >= 1 && w >= 1 && out >= 1 && in <= 4096 && w <= 8192 && out <= 4096;
+}
+
+struct MlpLayout {
+  uint16_t *a1, *a2, *dyb, *dz2, *dz1;
+  float* part;
+};
+constexpr int kMseBlocks = 64;
+
+MlpLayout mlp_layout(Carver& c, int rows, const MlpDims& d) {
+  MlpLayout L{};
+  L.a1 = c.take<uint16_t>((size_t)rows * d.w);
+  L.a2 = c.take<uint16_t>((size_t)rows * d.w);
+  L.dyb = c.take<uint16_t>((size_t)rows * d.out);
+  L.dz2 = c.take<uint16_t>((size_t)rows * d.w);
+  L.dz1 = c.take<uint16_t>((size_t)rows * d.w);
+  L.part = c.take<float>(kMseBlocks);
+  return L;
+}
+
+MlpGemmArgs gemm(int M, int N, int K, const uint16_t* A, int lda, int ta, const uint16_t* B, int ldb, int tb) {
+  MlpGemmArgs g{};
+  g.M = M; g.N = N; g.K = K;
+  g.A = A; g.lda = lda; g.transA = ta;
+  g.B = B; g.ldb = ldb; g.transB = tb;
+  return g;
+}
+
+int gate_rows_check(qmx_handle* h, int n, int batch, int num_layers, GateRowArgs& r,
+                    const int* layers_used, const int* widths) {
+  QMX_CALL(check_common(h, n, batch));
+  if (num_layers < 1 || !layers_used || !widths) return fail(h, QMX_ERR_INVALID, "gate rows: bad arguments");
+  const long rows = (long)batch * num_layers * (n - 1);
+  if (rows > (1l << 30)) return fail(h, QMX_ERR_INVALID, "gate rows: too many gates");
+  r.n = n;
+  r.lmax = num_layers;
+  r.rows = (int)rows;
+  r.layers = layers_used;
+  r.widths = widths;
+  return QMX_OK;
+}
+
 }  // namespace
 
 // =================================================================== C-ABI ==
@@ -1020,4 +1075,168 @@ int qmx_encode(qmx_handle* h, const qmx_encode_config* cfg, const void* statevec
                    E.sweep_bytes, stream);
 }
 
+// ---------------------------------------------------- learned warm start --
+size_t qmx_mlp_param_count(int in_dim, int width, int out_dim) {
+  if (!mlp_dims_ok(1, in_dim, width, out_dim)) return 0;
+  return MlpDims{in_dim, width, out_dim}.count();
+}
+
+size_t qmx_mlp_workspace(int rows, int in_dim, int width, int out_dim) {
+  if (!mlp_dims_ok(rows, in_dim, width, out_dim)) return 0;
+  Carver c(nullptr);
+  mlp_layout(c, rows, MlpDims{in_dim, width, out_dim});
+  return c.off + 256;
+}
+
+int qmx_mlp_forward(qmx_handle* h, int rows, int in_dim, int width, int out_dim, const void* x_bf16,
+                    const float* params, const void* params_bf16, float* y, void* workspace,
+                    size_t workspace_bytes, void* stream) {
+  if (!h) return QMX_ERR_INVALID;
+  if (!mlp_dims_ok(rows, in_dim, width, out_dim) || !x_bf16 || !params || !params_bf16 || !y)
+    return fail(h, QMX_ERR_INVALID, "mlp_forward: bad arguments");
+  if (!workspace || workspace_bytes < qmx_mlp_workspace(rows, in_dim, width, out_dim))
+    return fail(h, QMX_ERR_WORKSPACE, "mlp_forward: workspace too small");
+  QMX_HIP(h, hipSetDevice(h->device));
+  hipStream_t st = static_cast<hipStream_t>(stream);
+  const MlpDims d{in_dim, width, out_dim};
+  Carver c(workspace);
+  MlpLayout L = mlp_layout(c, rows, d);
+  const uint16_t* x = static_cast<const uint16_t*>(x_bf16);
+  const uint16_t* pb = static_cast<const uint16_t*>(params_bf16);
+  MlpGemmArgs g = gemm(rows, width, in_dim, x, in_dim, 0, pb + d.oW1(), width, 0);
+  g.bias = params + d.ob1(); g.relu = 1; g.Cb = L.a1; g.ldcb = width;
+  launch_mlp_gemm(g, st);
+  g = gemm(rows, width, width, L.a1, width, 0, pb + d.oW2(), width, 0);
+  g.bias = params + d.ob2(); g.relu = 1; g.Cb = L.a2; g.ldcb = width;
+  launch_mlp_gemm(g, st);
+  g = gemm(rows, out_dim, width, L.a2, width, 0, pb + d.oW3(), out_dim, 0);
+  g.bias = params + d.ob3(); g.C = y; g.ldc = out_dim;
+  launch_mlp_gemm(g, st);
+  QMX_LAUNCHED(h, "mlp forward");
+  return QMX_OK;
+}
+
+int qmx_mlp_backward(qmx_handle* h, int rows, int in_dim, int width, int out_dim, const void* x_bf16,
+                     const float* params, const void* params_bf16, const float* dy, float* grads,
+                     void* workspace, size_t workspace_bytes, void* stream) {
+  (void)params;
+  if (!h) return QMX_ERR_INVALID;
+  if (!mlp_dims_ok(rows, in_dim, width, out_dim) || !x_bf16 || !params_bf16 || !dy || !grads)
+    return fail(h, QMX_ERR_INVALID, "mlp_backward: bad arguments");
+  if (!workspace || workspace_bytes < qmx_mlp_workspace(rows, in_dim, width, out_dim))
+    return fail(h, QMX_ERR_WORKSPACE, "mlp_backward: workspace too small");
+  QMX_HIP(h, hipSetDevice(h->device));
+  hipStream_t st = static_cast<hipStream_t>(stream);
+  const MlpDims d{in_dim, width, out_dim};
+  Carver c(workspace);
+  MlpLayout L = mlp_layout(c, rows, d);
+  const uint16_t* x = static_cast<const uint16_t*>(x_bf16);
+  const uint16_t* pb = static_cast<const uint16_t*>(params_bf16);
+  launch_to_bf16(dy, L.dyb, (size_t)rows * out_dim, st);
+  // layer 3
+  MlpGemmArgs g = gemm(width, out_dim, rows, L.a2, width, 1, L.dyb, out_dim, 0);
+  g.C = grads + d.oW3(); g.ldc = out_dim;
+  launch_mlp_gemm(g, st);
+  launch_colsum_bf16(L.dyb, rows, out_dim, out_dim, grads + d.ob3(), st);
+  g = gemm(rows, width, out_dim, L.dyb, out_dim, 0, pb + d.oW3(), out_dim, 1);
+  g.mask = L.a2; g.ldm = width; g.Cb = L.dz2; g.ldcb = width;
+  launch_mlp_gemm(g, st);
+  // layer 2
+  g = gemm(width, width, rows, L.a1, width, 1, L.dz2, width, 0);
+  g.C = grads + d.oW2(); g.ldc = width;
+  launch_mlp_gemm(g, st);
+  launch_colsum_bf16(L.dz2, rows, width, width, grads + d.ob2(), st);
+  g = gemm(rows, width, width, L.dz2, width, 0, pb + d.oW2(), width, 1);
+  g.mask = L.a1; g.ldm = width; g.Cb = L.dz1; g.ldcb = width;
+  launch_mlp_gemm(g, st);
+  // layer 1
+  g = gemm(in_dim, width, rows, x, in_dim, 1, L.dz1, width, 0);
+  g.C = grads + d.oW1(); g.ldc = width;
+  launch_mlp_gemm(g, st);
+  launch_colsum_bf16(L.dz1, rows, width, width, grads + d.ob1(), st);
+  QMX_LAUNCHED(h, "mlp backward");
+  return QMX_OK;
+}
+
+int qmx_mlp_adam(qmx_handle* h, size_t count, float* params, void* params_bf16, const float* grads, float* m,
+                 float* v, float lr, float beta1, float beta2, float eps, int step, void* stream) {
+  if (!h) return QMX_ERR_INVALID;
+  if (count == 0 || !params || !grads || !m || !v || step < 1)
+    return fail(h, QMX_ERR_INVALID, "mlp_adam: bad arguments");
+  QMX_HIP(h, hipSetDevice(h->device));
+  launch_adam(params, static_cast<uint16_t*>(params_bf16), grads, m, v, count, lr, beta1, beta2, eps, step,
+              static_cast<hipStream_t>(stream));
+  QMX_LAUNCHED(h, "mlp adam");
+  return QMX_OK;
+}
+
+int qmx_mlp_to_bf16(qmx_handle* h, size_t count, const float* src, void* dst, void* stream) {
+  if (!h) return QMX_ERR_INVALID;
+  if (count == 0 || !src || !dst) return fail(h, QMX_ERR_INVALID, "to_bf16: bad arguments");
+  QMX_HIP(h, hipSetDevice(h->device));
+  launch_to_bf16(src, static_cast<uint16_t*>(dst), count, static_cast<hipStream_t>(stream));
+  QMX_LAUNCHED(h, "to_bf16");
+  return QMX_OK;
+}
+
+size_t qmx_mse_workspace(void) { return kMseBlocks * sizeof(float) + 256; }
+
+int qmx_mse_loss(qmx_handle* h, int rows, int ld, int valid, const float* y, const float* t, const float* w,
+                 float inv_norm, float* dy, float* loss, void* workspace, size_t workspace_bytes,
+                 void* stream) {
+  if (!h) return QMX_ERR_INVALID;
+  if (rows < 1 || ld < 1 || valid < 1 || valid > ld || !y || !t || !dy || !loss)
+    return fail(h, QMX_ERR_INVALID, "mse: bad arguments");
+  if (!workspace || workspace_bytes < qmx_mse_workspace())
+    return fail(h, QMX_ERR_WORKSPACE, "mse: workspace too small");
+  QMX_HIP(h, hipSetDevice(h->device));
+  launch_mse(y, t, w, rows, ld, valid, inv_norm, dy, static_cast<float*>(workspace), kMseBlocks, loss,
+             static_cast<hipStream_t>(stream));
+  QMX_LAUNCHED(h, "mse");
+  return QMX_OK;
+}
+
+int qmx_gate_features(qmx_handle* h, int num_qubits, int batch, int num_layers, const int* layers_used,
+                      const void* gates_a, const void* gates_b, const int* widths, void* features,
+                      int ld, float* row_weight, void* stream) {
+  if (!h) return QMX_ERR_INVALID;
+  GateRowArgs r{};
+  QMX_CALL(gate_rows_check(h, num_qubits, batch, num_layers, r, layers_used, widths));
+  if (!gates_a || !gates_b || !features || ld < kGateFeatures)
+    return fail(h, QMX_ERR_INVALID, "gate_features: bad arguments");
+  QMX_HIP(h, hipSetDevice(h->device));
+  launch_gate_features(r, static_cast<const double2*>(gates_a), static_cast<const double2*>(gates_b),
+                       static_cast<uint16_t*>(features), ld, row_weight, static_cast<hipStream_t>(stream));
+  QMX_LAUNCHED(h, "gate features");
+  return QMX_OK;
+}
+
+int qmx_gate_log(qmx_handle* h, int num_qubits, int batch, int num_layers, const int* layers_used,
+                 const void* gates_a, const void* gates_b, const int* widths, float* coeffs, int ld,
+                 void* stream) {
+  if (!h) return QMX_ERR_INVALID;
+  GateRowArgs r{};
+  QMX_CALL(gate_rows_check(h, num_qubits, batch, num_layers, r, layers_used, widths));
+  if (!gates_a || !gates_b || !coeffs || ld < 15) return fail(h, QMX_ERR_INVALID, "gate_log: bad arguments");
+  QMX_HIP(h, hipSetDevice(h->device));
+  launch_gate_log(r, static_cast<const double2*>(gates_a), static_cast<const double2*>(gates_b), coeffs, ld,
+                  static_cast<hipStream_t>(stream));
+  QMX_LAUNCHED(h, "gate log");
+  return QMX_OK;
+}
+
+int qmx_gate_expmap(qmx_handle* h, int num_qubits, int batch, int num_layers, const int* layers_used,
+                    const float* coeffs, int ld, void* gates, const int* widths, const int* skip,
+                    void* stream) {
+  if (!h) return QMX_ERR_INVALID;
+  GateRowArgs r{};
+  QMX_CALL(gate_rows_check(h, num_qubits, batch, num_layers, r, layers_used, widths));
+  if (!coeffs || !gates || ld < 15) return fail(h, QMX_ERR_INVALID, "gate_expmap: bad arguments");
+  QMX_HIP(h, hipSetDevice(h->device));
+  launch_gate_expmap(r, coeffs, ld, static_cast<double2*>(gates), skip, static_cast<hipStream_t>(stream));
+  QMX_LAUNCHED(h, "gate expmap");
+  return QMX_OK;
+}
+
 }  // extern "C"
+

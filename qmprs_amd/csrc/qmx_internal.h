@@ -17,7 +17,6 @@ constexpr double kGsTol = 1e-7;      // canonical-gauge Gram-Schmidt threshold
 constexpr double kPhaseTol = 1e-12;  // Ran two-site phase fix skipped below this
 
 constexpr int kWinTileBits = 11;     // window pass tile: 2^11 amplitudes
-constexpr int kWinThreads = 512;     // window pass block size
 constexpr int kMaxWindow = 8;        // gates per window
 constexpr int kWinMinQubits = 10;    // windowed sweep: 4 waves x k-steps of 4 rest indices
 
@@ -176,6 +175,44 @@ struct Amp0Args {
 inline __host__ __device__ int chain_buf_is_a(int n, int j) { return ((n - j) & 1) == 1; }
 
 // launchers (qmx_kernels.hip)
+// BF16 GEMM with fused epilogue (qmx_mlp.hip)
+struct MlpGemmArgs {
+  int M, N, K;
+  const uint16_t* A;
+  int lda, transA;
+  const uint16_t* B;
+  int ldb, transB;
+  const float* bias;       // [N] or nullptr
+  int relu;
+  const uint16_t* mask;    // keep where mask > 0 (BF16 [M x ldm]) or nullptr
+  int ldm;
+  float* C;                // FP32 out or nullptr
+  int ldc;
+  uint16_t* Cb;            // BF16 out or nullptr
+  int ldcb;
+};
+
+// gate slots (b, layer, site < n - 1) as rows of a feature matrix; a row
+// holds kGateFeatures values (see gate_features_kernel)
+constexpr int kGateFeatures = 50;
+struct GateRowArgs {
+  int n, lmax, rows;
+  const int* layers;
+  const int* widths;
+};
+
+void launch_mlp_gemm(const MlpGemmArgs&, hipStream_t);
+void launch_colsum_bf16(const uint16_t* dz, int M, int N, int ld, float* db, hipStream_t);
+void launch_to_bf16(const float* x, uint16_t* y, size_t n, hipStream_t);
+void launch_adam(float* p, uint16_t* pb, const float* g, float* m, float* v, size_t n, float lr, float b1,
+                 float b2, float eps, int step, hipStream_t);
+void launch_mse(const float* y, const float* t, const float* w, int rows, int ld, int valid, float inv_norm,
+                float* dy, float* part, int nblk, float* loss, hipStream_t);
+void launch_gate_features(const GateRowArgs&, const double2* ga, const double2* gb, uint16_t* feat, int ld,
+                          float* weight, hipStream_t);
+void launch_gate_log(const GateRowArgs&, const double2* ga, const double2* gb, float* out, int ld, hipStream_t);
+void launch_gate_expmap(const GateRowArgs&, const float* coef, int ld, double2* g, const int* skip, hipStream_t);
+
 template <typename S> void launch_apply_pass(const PassArgs<S>&, int nblk, int batch, hipStream_t);
 template <typename S> void launch_sweep_step(const SweepArgs<S>&, int nblk, int batch, hipStream_t);
 template <typename S>
