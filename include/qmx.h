@@ -170,6 +170,16 @@ int qmx_encode(qmx_handle* h, const qmx_encode_config* cfg, const void* statevec
                void* gates, int* widths, int* layers_used, void* fidelity, const double* goal,
                int* sweeps_done, void* workspace, size_t workspace_bytes, void* stream);
 
+/* The part of qmx_encode before the sweeps (chi-truncation, normalisation,
+ * layer extraction), for callers that drive the sweeps themselves with
+ * qmx_sweep: qmx_encode equals qmx_encode_prepare followed by
+ * qmx_sweep(target_out, psi_ref = psi_out).  Workspace: qmx_encode_workspace.
+ *   psi_out     device, B x 2^N complex128: the normalised input
+ *   target_out  device, B x 2^N of cfg->sweep_dtype: the chi-MPS state */
+int qmx_encode_prepare(qmx_handle* h, const qmx_encode_config* cfg, const void* statevectors,
+                       void* gates, int* widths, int* layers_used, void* psi_out, void* target_out,
+                       void* workspace, size_t workspace_bytes, void* stream);
+
 /* ---- Learned warm start (no reference counterpart) ------------------------
  * A 3-layer ReLU MLP (BF16 weights, FP32 accumulation on MFMA) maps a feature
  * row per two-qubit gate slot to 15 su(4) coefficients c; the proposal is

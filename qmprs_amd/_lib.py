@@ -76,6 +76,8 @@ SIGNATURES = {
     "qmx_encode_workspace": (_SZ, [ctypes.POINTER(EncodeConfig)]),
     "qmx_encode": (_INT, [_VP, ctypes.POINTER(EncodeConfig), _VP, _VP, _VP, _VP, _VP, _VP, _VP, _VP,
                           _SZ, _VP]),
+    "qmx_encode_prepare": (_INT, [_VP, ctypes.POINTER(EncodeConfig), _VP, _VP, _VP, _VP, _VP, _VP, _VP,
+                                  _SZ, _VP]),
     "qmx_mlp_param_count": (_SZ, [_INT, _INT, _INT]),
     "qmx_mlp_workspace": (_SZ, [_INT, _INT, _INT, _INT]),
     "qmx_mlp_forward": (_INT, [_VP, _INT, _INT, _INT, _INT, _VP, _VP, _VP, _VP, _VP, _SZ, _VP]),

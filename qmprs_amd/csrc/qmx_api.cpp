@@ -1033,9 +1033,8 @@ size_t qmx_encode_workspace(const qmx_encode_config* cfg) {
   return c.off + 256;
 }
 
-int qmx_encode(qmx_handle* h, const qmx_encode_config* cfg, const void* statevectors,
-               void* gates, int* widths, int* layers_used, void* fidelity, const double* goal,
-               int* sweeps_done, void* workspace, size_t workspace_bytes, void* stream) {
+namespace {
+int encode_check(qmx_handle* h, const qmx_encode_config* cfg, size_t workspace_bytes) {
   if (!h || !cfg) return QMX_ERR_INVALID;
   const qmx_encode_config g = *cfg;
   QMX_CALL(check_common(h, g.num_qubits, g.batch));
@@ -1047,13 +1046,14 @@ int qmx_encode(qmx_handle* h, const qmx_encode_config* cfg, const void* statevec
   if ((g.sv_dtype != QMX_C64 && g.sv_dtype != QMX_C128) ||
       (g.sweep_dtype != QMX_C64 && g.sweep_dtype != QMX_C128))
     return fail(h, QMX_ERR_INVALID, "dtype");
-  if (!statevectors || !gates || !widths || !layers_used || !fidelity || !workspace)
-    return fail(h, QMX_ERR_INVALID, "null pointer");
   if (workspace_bytes < qmx_encode_workspace(cfg)) return fail(h, QMX_ERR_WORKSPACE, "workspace too small");
-  QMX_HIP(h, hipSetDevice(h->device));
-  hipStream_t st = static_cast<hipStream_t>(stream);
-  Carver c(workspace);
-  EncodeLayout E = encode_layout(c, g);
+  return QMX_OK;
+}
+
+// chi-truncation, normalisation, layer extraction, conversion of the target
+// to the sweep storage type: everything of prepare_state before the sweeps
+int encode_prepare(qmx_handle* h, const qmx_encode_config& g, const void* statevectors, void* gates,
+                   int* widths, int* layers_used, const EncodeLayout& E, hipStream_t st) {
   const int n = g.num_qubits, B = g.batch;
   QMX_CALL(run_chi(h, n, B, g.bond_dimension, g.cutoff, statevectors, g.sv_dtype, E.psi, E.target,
                    nullptr, E.chi, st));
@@ -1070,9 +1070,46 @@ int qmx_encode(qmx_handle* h, const qmx_encode_config* cfg, const void* statevec
   if (g.sweep_dtype == QMX_C64)
     launch_scale_convert(static_cast<const double2*>(E.target), static_cast<float2*>(E.target_s), n,
                          B, nullptr, st);
-  return qmx_sweep(h, n, B, g.num_layers, layers_used, g.num_sweeps, E.target_s, g.sweep_dtype,
-                   gates, widths, E.psi, fidelity, nullptr, goal, sweeps_done, nullptr, E.sweep_ws,
-                   E.sweep_bytes, stream);
+  QMX_LAUNCHED(h, "target conversion");
+  return QMX_OK;
+}
+}  // namespace
+
+int qmx_encode(qmx_handle* h, const qmx_encode_config* cfg, const void* statevectors,
+               void* gates, int* widths, int* layers_used, void* fidelity, const double* goal,
+               int* sweeps_done, void* workspace, size_t workspace_bytes, void* stream) {
+  QMX_CALL(encode_check(h, cfg, workspace_bytes));
+  if (!statevectors || !gates || !widths || !layers_used || !fidelity || !workspace)
+    return fail(h, QMX_ERR_INVALID, "null pointer");
+  const qmx_encode_config g = *cfg;
+  QMX_HIP(h, hipSetDevice(h->device));
+  hipStream_t st = static_cast<hipStream_t>(stream);
+  Carver c(workspace);
+  EncodeLayout E = encode_layout(c, g);
+  QMX_CALL(encode_prepare(h, g, statevectors, gates, widths, layers_used, E, st));
+  return qmx_sweep(h, g.num_qubits, g.batch, g.num_layers, layers_used, g.num_sweeps, E.target_s,
+                   g.sweep_dtype, gates, widths, E.psi, fidelity, nullptr, goal, sweeps_done, nullptr,
+                   E.sweep_ws, E.sweep_bytes, stream);
+}
+
+int qmx_encode_prepare(qmx_handle* h, const qmx_encode_config* cfg, const void* statevectors, void* gates,
+                       int* widths, int* layers_used, void* psi_out, void* target_out, void* workspace,
+                       size_t workspace_bytes, void* stream) {
+  QMX_CALL(encode_check(h, cfg, workspace_bytes));
+  if (!statevectors || !gates || !widths || !layers_used || !psi_out || !target_out || !workspace)
+    return fail(h, QMX_ERR_INVALID, "null pointer");
+  const qmx_encode_config g = *cfg;
+  QMX_HIP(h, hipSetDevice(h->device));
+  hipStream_t st = static_cast<hipStream_t>(stream);
+  Carver c(workspace);
+  EncodeLayout E = encode_layout(c, g);
+  QMX_CALL(encode_prepare(h, g, statevectors, gates, widths, layers_used, E, st));
+  const size_t amps = state_len(g.num_qubits) * g.batch;
+  QMX_HIP(h, hipMemcpyAsync(psi_out, E.psi, amps * sizeof(double2), hipMemcpyDeviceToDevice, st));
+  QMX_HIP(h, hipMemcpyAsync(target_out, E.target_s,
+                            amps * (g.sweep_dtype == QMX_C64 ? sizeof(float2) : sizeof(double2)),
+                            hipMemcpyDeviceToDevice, st));
+  return QMX_OK;
 }
 
 // ---------------------------------------------------- learned warm start --

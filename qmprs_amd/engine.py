@@ -188,6 +188,41 @@ def encode(statevectors, bond_dimension: int, num_layers: int = 1, num_sweeps: i
     return EncodeResult(gates, widths, used, fid, done)
 
 
+@dataclass
+class Prepared:
+    psi: torch.Tensor          # (B, 2^N) complex128 normalised input
+    target: torch.Tensor       # (B, 2^N) sweep dtype: the chi-MPS state
+    gates: torch.Tensor        # (B, L, N, 4, 4) complex128 Ran gates
+    widths: torch.Tensor       # (B, L, N) int32
+    layers_used: torch.Tensor  # (B,) int32
+
+
+def encode_prepare(statevectors, bond_dimension: int, num_layers: int = 1,
+                   fidelity_threshold: float = DEFAULT_THRESHOLD, cutoff: float = CUTOFF,
+                   sweep_dtype=torch.complex64, workspace: torch.Tensor | None = None) -> Prepared:
+    """prepare_state up to the sweeps (qmx_encode_prepare); ``sweep`` on the
+    result with psi_ref=psi reproduces ``encode`` exactly."""
+    sv = as_device_states(statevectors)
+    b, length = sv.shape
+    n = num_qubits_of(length)
+    lib, h = _lib.load(), _lib.handle()
+    cfg = _lib.EncodeConfig(n, b, int(bond_dimension), int(num_layers), 0, _DTYPES[sv.dtype],
+                            _DTYPES[sweep_dtype], float(fidelity_threshold), float(cutoff))
+    need = lib.qmx_encode_workspace(ctypes.byref(cfg))
+    if workspace is None or workspace.numel() < need:
+        workspace = _lib.workspace(need, sv.device)
+    gates = torch.zeros((b, num_layers, n, 4, 4), dtype=torch.complex128, device=sv.device)
+    widths = torch.ones((b, num_layers, n), dtype=torch.int32, device=sv.device)
+    used = torch.zeros(b, dtype=torch.int32, device=sv.device)
+    psi = torch.empty((b, length), dtype=torch.complex128, device=sv.device)
+    target = torch.empty((b, length), dtype=sweep_dtype, device=sv.device)
+    st = lib.qmx_encode_prepare(h, ctypes.byref(cfg), _lib.ptr(sv), _lib.ptr(gates), _lib.ptr(widths),
+                                _lib.ptr(used), _lib.ptr(psi), _lib.ptr(target), _lib.ptr(workspace),
+                                workspace.numel(), _lib.stream_ptr())
+    _lib.check(st, h)
+    return Prepared(psi, target, gates, widths, used)
+
+
 def gates_to_layers(gates: np.ndarray, widths: np.ndarray, layers_used: int):
     """Device gate arrays of one state -> list[UnitaryLayer] in circuit order.
 

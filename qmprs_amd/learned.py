@@ -177,12 +177,9 @@ def plain_trajectory(statevectors, bond_dimension: int, num_layers: int, num_swe
                      sweep_dtype=torch.complex64):
     """Plain method on a batch: returns (psi, target, widths, used, G0, G1, GS,
     fidelity trace (B, S + 1))."""
-    sv = engine.as_device_states(statevectors)
-    psi, target, _ = engine.chi_truncate(sv, bond_dimension)
-    work = (target / torch.linalg.vector_norm(target, dim=1, keepdim=True)).contiguous()
-    gates, widths, used, _, _ = engine.extract_layers(work, num_layers)
+    p = engine.encode_prepare(statevectors, bond_dimension, num_layers, sweep_dtype=sweep_dtype)
+    psi, tgt, gates, widths, used = p.psi, p.target, p.gates, p.widths, p.layers_used
     g0 = gates.clone()
-    tgt = target.to(sweep_dtype).contiguous()
     r1 = engine.sweep(tgt, gates, widths, used, 1, psi_ref=psi)
     g1 = gates.clone()
     rs = engine.sweep(tgt, gates, widths, used, num_sweeps - 1, psi_ref=psi)
@@ -257,11 +254,8 @@ def encode_warm(statevectors, bond_dimension: int, num_layers: int, num_sweeps: 
     after num_sweeps in total."""
     if num_sweeps < 1:
         raise ValueError("the warm start needs num_sweeps >= 1")
-    sv = engine.as_device_states(statevectors)
-    psi, target, _ = engine.chi_truncate(sv, bond_dimension)
-    work = (target / torch.linalg.vector_norm(target, dim=1, keepdim=True)).contiguous()
-    gates, widths, used, _, _ = engine.extract_layers(work, num_layers)
-    tgt = target.to(sweep_dtype).contiguous()
+    p = engine.encode_prepare(statevectors, bond_dimension, num_layers, sweep_dtype=sweep_dtype)
+    psi, tgt, gates, widths, used = p.psi, p.target, p.gates, p.widths, p.layers_used
     g0 = gates.clone()
     r1 = engine.sweep(tgt, gates, widths, used, 1, psi_ref=psi)
     f1 = r1.fidelity[:, -1].abs()

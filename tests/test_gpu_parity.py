@@ -288,3 +288,20 @@ def test_window_sweep_equals_step_sweep(gpu, dtype, n, layers, sweeps, seed):
     assert sd < (1e-10 if dtype == torch.complex128 else 1e-5)
     np.testing.assert_allclose(out["window"][2], out["step"][2], atol=1e-12 if dtype == torch.complex128 else 1e-6)
     np.testing.assert_allclose(out["window"][3], out["step"][3], rtol=1e-10 if dtype == torch.complex128 else 1e-6)
+
+
+def test_prepare_then_sweeps_equals_encode(gpu):
+    """qmx_encode == qmx_encode_prepare + qmx_sweep, and S sweeps in one call
+    == 1 + (S - 1) sweeps in two calls, bit for bit (every sweep restarts from
+    the target and the current gates)."""
+    eng = _engine()
+    n, chi, L, S = 12, 16, 4, 5
+    svs = np.stack([O.random_mps_state(n, 24, 40 + s) for s in range(3)])
+    ref = eng.encode(svs, chi, L, S, sweep_dtype=torch.complex64)
+    p = eng.encode_prepare(svs, chi, L, sweep_dtype=torch.complex64)
+    r1 = eng.sweep(p.target, p.gates, p.widths, p.layers_used, 1, psi_ref=p.psi)
+    r2 = eng.sweep(p.target, p.gates, p.widths, p.layers_used, S - 1, psi_ref=p.psi)
+    assert torch.equal(p.layers_used, ref.layers_used)
+    assert torch.equal(p.gates, ref.gates)
+    assert torch.equal(r2.fidelity[:, -1], ref.fidelity[:, -1])
+    assert torch.equal(r1.fidelity[:, 0], ref.fidelity[:, 0])
