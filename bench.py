@@ -35,8 +35,8 @@ HBM_PEAK_GBS = 8000.0
 # Calibrated HBM traffic of one window_pass_kernel dispatch on this workload,
 # from rocprofv3 --pmc FETCH_SIZE and WRITE_SIZE runs (FETCH_SIZE x 2 and
 # WRITE_SIZE x 1, the factors a plain 1 GiB copy reads on the same counters):
-# batch -> (read KiB, write KiB).  Summaries: profiles/window_pass_n20_*.md.
-MEASURED_TRAFFIC_KIB = {32: (526760.0, 529176.0)}
+# states per launch -> (read KiB, write KiB).  Summaries: profiles/window_pass_n20_*.md.
+MEASURED_TRAFFIC_KIB = {32: (527304.0, 541359.0)}
 WORKLOAD = {"num_qubits": 20, "bond_dimension": 64, "num_layers": 10, "num_sweeps": 30,
             "generator_bond": 96}
 
@@ -215,15 +215,19 @@ def main() -> int:
 
     fid = res.fidelity.abs().cpu().numpy()
     layers_used = res.layers_used.cpu().numpy()
-    # dominant kernel: the windowed sweep's streaming pass
+    # dominant kernel: the windowed sweep's streaming pass.  The library runs
+    # a batch as two half-batches whose passes alternate (never overlap), so
+    # a launch covers B / jobs states.
     wp_ms, wp_launches = prof["window_pass"]
     avg_launch_s = wp_ms / 1000.0 / max(wp_launches, 1)
-    windows = max(1, round(wp_launches / (args.steps * S)))
+    jobs = 2 if B >= 2 else 1
+    windows = max(1, round(wp_launches / (args.steps * S * jobs)))
     amp_bytes = (32.0 * (windows - 1) + 24.0) / windows
-    bytes_per_launch = B * (2 ** n) * amp_bytes
-    achieved = bytes_per_launch / avg_launch_s / 1e9
+    total_bytes = args.steps * S * windows * B * (2 ** n) * amp_bytes
+    bytes_per_launch = total_bytes / max(wp_launches, 1)
+    achieved = total_bytes / (wp_ms / 1000.0) / 1e9
     traffic = None
-    kib = args.traffic_kib or MEASURED_TRAFFIC_KIB.get(B)
+    kib = args.traffic_kib or MEASURED_TRAFFIC_KIB.get(B // jobs)
     if kib:
         traffic = int((kib[0] + kib[1]) * 1024)
     roofline = {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS,
@@ -231,6 +235,7 @@ def main() -> int:
                 "kernel": "window_pass_kernel<float2, 4>",
                 "avg_launch_us": round(avg_launch_s * 1e6, 2),
                 "launches": int(wp_launches), "windows_per_sweep": windows,
+                "states_per_launch": B // jobs,
                 "bytes_per_launch": int(bytes_per_launch)}
     phases = {p: {"ms": round(v[0] / args.steps, 2), "launches": int(v[1] / args.steps)}
               for p, v in prof.items()}

@@ -56,6 +56,15 @@ const char* qmx_last_error(const qmx_handle* h);
 #define QMX_SWEEP_AUTO 0
 #define QMX_SWEEP_STEP 1
 #define QMX_SWEEP_WINDOW 2
+/* QMX_OPT_SPLIT_SWEEPS: 1 (default) runs a windowed sweep of a batch of two
+ * or more states as two half-batches on two internal streams whose window
+ * passes alternate, so that each half's window finalize overlaps the other
+ * half's pass; 0 runs the whole batch on the caller's stream.  Results agree
+ * to rounding (states are independent).
+ * QMX_OPT_WINDOW_BLOCKS_PER_CU: grid of the windowed pass, in blocks per
+ * compute unit over the whole batch (1..16, default 16). */
+#define QMX_OPT_SPLIT_SWEEPS 2
+#define QMX_OPT_WINDOW_BLOCKS_PER_CU 3
 int qmx_set_option(qmx_handle* h, int option, int value);
 
 /* Phase timers (no reference counterpart: instrumentation for benchmarks).

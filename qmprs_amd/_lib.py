@@ -175,6 +175,21 @@ def profile_read() -> dict:
     return {p: (ms[i], ln[i]) for i, p in enumerate(PHASES)}
 
 
+QMX_OPT_SPLIT_SWEEPS = 2
+QMX_OPT_WINDOW_BLOCKS_PER_CU = 3
+
+
+def set_option(option: int, value: int) -> None:
+    h = handle()
+    check(load().qmx_set_option(h, option, int(value)), h)
+
+
+def set_split_sweeps(enable: bool) -> None:
+    """Run batches of >= 2 states as two interleaved half-batches (default on)."""
+    h = handle()
+    check(load().qmx_set_option(h, QMX_OPT_SPLIT_SWEEPS, 1 if enable else 0), h)
+
+
 def set_sweep_mode(mode: str) -> None:
     """'auto' (windowed passes for N >= 8), 'step' (one pass per gate) or 'window'."""
     h = handle()

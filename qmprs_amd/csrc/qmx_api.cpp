@@ -31,6 +31,12 @@ struct qmx_handle {
   int sweep_mode = QMX_SWEEP_AUTO;
   std::vector<ProfRec> recs;
   std::vector<hipEvent_t> pool;
+  bool split_sweeps = true;             // QMX_OPT_SPLIT_SWEEPS
+  int win_blocks_per_cu = 16;           // QMX_OPT_WINDOW_BLOCKS_PER_CU
+  hipStream_t aux[2] = {nullptr, nullptr};
+  hipEvent_t pass_ev[2] = {nullptr, nullptr};
+  hipEvent_t join_ev[2] = {nullptr, nullptr};
+  hipEvent_t call_ev = nullptr;
 };
 
 namespace {
@@ -233,6 +239,16 @@ struct LayerLayout {
 };
 constexpr int kChain2MaxBlocks = 256;
 constexpr int kWinMaxBlocks = 256;
+
+// window-pass blocks per state: about win_blocks_per_cu resident blocks per CU
+// over the whole batch
+int window_blocks(const qmx_handle* h, int n, int batch) {
+  const long ntiles = 1l << (n - std::min(n, kWinTileBits));
+  long per = ((long)h->win_blocks_per_cu * h->cus + batch - 1) / batch;
+  per = std::max(1l, std::min(per, ntiles));
+  return (int)std::min<long>(per, kWinMaxBlocks);
+}
+
 
 struct Window {
   int s0, s1;
@@ -599,134 +615,282 @@ int run_extract(qmx_handle* h, int n, int batch, int lmax, double thr, double cu
   return device_error_status(h, L.error, st);
 }
 
+// One batch of states swept on one stream.  qmx_sweep runs one job, or two
+// jobs (the two halves of the batch) on the handle's two auxiliary streams;
+// their window passes alternate, so each window's latency-bound finalize
+// overlaps the other half's bandwidth-bound pass.
 template <typename S>
-int run_sweep(qmx_handle* h, int n, int batch, int lmax, const int* layers_used, int nsweeps,
-              const S* target, double2* gates, const int* widths, const double2* psi_ref,
-              double2* fidelity, double* overlap, const double* goal, int* sweeps_done,
-              S* circuit_out, const SweepLayout<S>& L, hipStream_t st) {
+struct SweepJob {
+  int n, batch, lmax, nsweeps;
+  const int* layers_used;
+  const S* target;
+  double2* gates;
+  const int* widths;
+  const double2* psi_ref;
+  double2* fidelity;
+  double* overlap;
+  const double* goal;
+  int* sweeps_done;
+  S* circuit_out;
+  SweepLayout<S> L;
+  hipStream_t st;
+  const int* skip = nullptr;
+  bool use_goal = false;
+  bool active = true;
+};
+
+template <typename S>
+int job_begin(qmx_handle* h, SweepJob<S>& J) {
+  const hipStream_t st = J.st;
+  launch_fill_int(J.L.counter, 0, J.batch, st);
+  launch_fill_int(J.L.dot_counter, 0, J.batch, st);
+  launch_fill_int(J.L.skip, 0, J.batch, st);
+  if (J.fidelity)
+    launch_fill_int(reinterpret_cast<int*>(J.fidelity), 0, J.batch * (J.nsweeps + 1) * 4, st);
+  if (J.sweeps_done) launch_fill_int(J.sweeps_done, J.goal ? 0 : J.nsweeps, J.batch, st);
+  J.use_goal = J.goal && J.psi_ref && J.fidelity;
+  J.skip = J.use_goal ? J.L.skip : nullptr;
+  QMX_LAUNCHED(h, "sweep set-up");
+  return QMX_OK;
+}
+
+// C|0..0> (before sweep s, or the final state when s == nsweeps) and the
+// fidelity entry s
+template <typename S>
+int job_rebuild(qmx_handle* h, SweepJob<S>& J, int s, bool final_state) {
+  const int n = J.n;
+  const int* skip = final_state ? nullptr : J.skip;
+  launch_basis0(J.L.phi, n, J.batch, skip, J.st);
+  const double2* ref = J.fidelity ? J.psi_ref : nullptr;
+  double2* dot = J.fidelity ? J.fidelity + s : nullptr;
+  if (n >= 5 && h->sweep_mode != QMX_SWEEP_STEP) {
+    QMX_CALL(run_fwd<S>(h, n, J.batch, J.lmax, plan_fwd(n, J.lmax * n), J.L.phi, J.gates, J.widths,
+                        J.layers_used, skip, ref, J.L.dot_part, J.L.dot_counter, dot, J.nsweeps + 1, J.st));
+  } else {
+    QMX_CALL(run_passes<S>(h, n, J.batch, J.lmax, 1, 0, forward_circuit_passes(n, J.lmax), J.L.phi,
+                           J.gates, J.widths, J.layers_used, skip, ref, J.L.dot_part, J.L.dot_counter,
+                           dot, J.nsweeps + 1, h->cus, J.st));
+  }
+  return QMX_OK;
+}
+
+template <typename S>
+int job_goal(qmx_handle* h, SweepJob<S>& J, int s) {
+  launch_fill_int(J.L.active, 0, 1, J.st);
+  launch_goal_check(J.fidelity + s, J.nsweeps + 1, J.goal, J.L.skip, J.L.active, J.sweeps_done, J.batch,
+                    J.st);
+  int active = 0;
+  QMX_HIP(h, hipMemcpyAsync(&active, J.L.active, sizeof(int), hipMemcpyDeviceToHost, J.st));
+  QMX_HIP(h, hipStreamSynchronize(J.st));
+  J.active = active != 0;
+  return QMX_OK;
+}
+
+template <typename S>
+void job_window_pass(qmx_handle* h, SweepJob<S>& J, const std::vector<Window>& wins, size_t wi) {
+  const int n = J.n;
+  const Window& w = wins[wi];
+  const int wblk = window_blocks(h, n, J.batch);
+  WinPassArgs<S> pa{};
+  pa.n = n;
+  pa.lmax = J.lmax;
+  pa.t = std::min(n, kWinTileBits);
+  pa.s0 = w.s0;
+  pa.s1 = w.s1;
+  pa.p0 = wi ? wins[wi - 1].s0 : 0;
+  pa.p1 = wi ? wins[wi - 1].s1 : 0;
+  const uint64_t pmask = wi ? wins[wi - 1].umask : w.umask;
+  const uint64_t pbits = tile_bits(n, pa.t, w.umask | pmask);
+  pa.qmask = ((1ull << n) - 1ull) & ~pbits;
+  pa.gu = make_group(n, pbits, w.umask);
+  pa.gp = make_group(n, pbits, pmask);
+  pa.target = J.target;
+  pa.tprime = J.L.tprime;
+  pa.phi = J.L.phi;
+  pa.gates = J.gates;
+  pa.widths = J.widths;
+  pa.layers = J.layers_used;
+  pa.skip = J.skip;
+  pa.part = J.L.part;
+  PhaseTimer pt(h, J.st, QMX_PHASE_WINDOW_PASS);
+  launch_window_pass(pa, wblk, J.batch, J.st);
+  pt.stop(1);
+}
+
+template <typename S>
+void job_window_finalize(qmx_handle* h, SweepJob<S>& J, const std::vector<Window>& wins, size_t wi, int s) {
+  const int n = J.n;
+  const Window& w = wins[wi];
+  WinFinArgs fa{};
+  fa.n = n;
+  fa.lmax = J.lmax;
+  fa.nblk = window_blocks(h, n, J.batch);
+  fa.s0 = w.s0;
+  fa.s1 = w.s1;
+  fa.umask = w.umask;
+  fa.gates = J.gates;
+  fa.widths = J.widths;
+  fa.layers = J.layers_used;
+  fa.skip = J.skip;
+  fa.part = J.L.part;
+  fa.overlap = J.overlap ? J.overlap + s : nullptr;
+  fa.overlap_stride = J.nsweeps;
+  launch_window_finalize(fa, J.batch, J.st);
+}
+
+template <typename S>
+void job_step_sweep(qmx_handle* h, SweepJob<S>& J, int s) {
+  const int n = J.n;
+  const int kmax = J.lmax * n;
+  const int nblk = blocks_for(1l << (n - std::min(n, kSweepTileBits)), J.batch, h->cus);
+  for (int k = 0; k < kmax; ++k) {
+    SweepArgs<S> a{};
+    a.n = n;
+    a.lmax = J.lmax;
+    a.tbits = kSweepTileBits;
+    a.step = k;
+    a.target = J.target;
+    a.tprime = J.L.tprime;
+    a.phi = J.L.phi;
+    a.gates = J.gates;
+    a.widths = J.widths;
+    a.layers = J.layers_used;
+    a.skip = J.skip;
+    a.part = J.L.part;
+    a.counter = J.L.counter;
+    a.overlap = J.overlap ? J.overlap + s : nullptr;
+    a.overlap_stride = J.nsweeps;
+    launch_sweep_step(a, nblk, J.batch, J.st);
+  }
+}
+
+// record `ev` on `from` and make `to` wait for it
+int stream_join(qmx_handle* h, hipEvent_t ev, hipStream_t from, hipStream_t to) {
+  QMX_HIP(h, hipEventRecord(ev, from));
+  QMX_HIP(h, hipStreamWaitEvent(to, ev, 0));
+  return QMX_OK;
+}
+
+template <typename S>
+int run_sweep_jobs(qmx_handle* h, std::vector<SweepJob<S>>& jobs) {
+  const int n = jobs[0].n, lmax = jobs[0].lmax, nsweeps = jobs[0].nsweeps;
   const int kmax = lmax * n;
-  launch_fill_int(L.counter, 0, batch, st);
-  launch_fill_int(L.dot_counter, 0, batch, st);
-  launch_fill_int(L.skip, 0, batch, st);
-  if (fidelity) launch_fill_int(reinterpret_cast<int*>(fidelity), 0, batch * (nsweeps + 1) * 4, st);
-  if (sweeps_done) launch_fill_int(sweeps_done, goal ? 0 : nsweeps, batch, st);
-  const bool use_goal = goal && psi_ref && fidelity;
-  const int* skip = use_goal ? L.skip : nullptr;
-  auto rebuild = forward_circuit_passes(n, lmax);
-  const int ts = std::min(n, kSweepTileBits);
-  const int nblk = blocks_for(1l << (n - ts), batch, h->cus);
-  const bool windowed = (h->sweep_mode == QMX_SWEEP_WINDOW ||
-                         (h->sweep_mode == QMX_SWEEP_AUTO)) && n >= kWinMinQubits;
+  const bool windowed = (h->sweep_mode == QMX_SWEEP_WINDOW || h->sweep_mode == QMX_SWEEP_AUTO) &&
+                        n >= kWinMinQubits;
   const auto wins = plan_windows(n, kmax);
-  const bool use_fwd = n >= 5 && h->sweep_mode != QMX_SWEEP_STEP;
-  const auto fwd = plan_fwd(n, kmax);
-  const int wblk = std::min(kWinMaxBlocks,
-                            blocks_for(1l << (n - std::min(n, kWinTileBits)), batch, h->cus));
+  const size_t nj = jobs.size();
+  const hipStream_t s0 = jobs[0].st;
+  for (auto& J : jobs) QMX_CALL(job_begin(h, J));
+  const long long rebuild_launches =
+      (n >= 5 && h->sweep_mode != QMX_SWEEP_STEP) ? (long long)plan_fwd(n, kmax).size()
+                                                   : (long long)forward_circuit_passes(n, lmax).size();
   for (int s = 0; s < nsweeps; ++s) {
-    PhaseTimer rt(h, st, QMX_PHASE_REBUILD);
-    launch_basis0(L.phi, n, batch, skip, st);
-    if (use_fwd) {
-      QMX_CALL(run_fwd<S>(h, n, batch, lmax, fwd, L.phi, gates, widths, layers_used, skip,
-                          fidelity ? psi_ref : nullptr, L.dot_part, L.dot_counter,
-                          fidelity ? fidelity + s : nullptr, nsweeps + 1, st));
-    } else {
-      QMX_CALL(run_passes<S>(h, n, batch, lmax, 1, 0, rebuild, L.phi, gates, widths, layers_used,
-                             skip, fidelity ? psi_ref : nullptr, L.dot_part, L.dot_counter,
-                             fidelity ? fidelity + s : nullptr, nsweeps + 1, h->cus, st));
+    PhaseTimer rt(h, s0, QMX_PHASE_REBUILD);
+    for (auto& J : jobs)
+      if (J.active) QMX_CALL(job_rebuild(h, J, s, false));
+    for (size_t j = 1; j < nj; ++j) QMX_CALL(stream_join(h, h->join_ev[j], jobs[j].st, s0));
+    rt.stop(rebuild_launches);
+    bool any = false;
+    for (auto& J : jobs) {
+      if (J.use_goal && J.active) QMX_CALL(job_goal(h, J, s));
+      any = any || J.active;
     }
-    rt.stop(static_cast<long long>(use_fwd ? fwd.size() : rebuild.size()));
-    if (use_goal) {
-      launch_fill_int(L.active, 0, 1, st);
-      launch_goal_check(fidelity + s, nsweeps + 1, goal, L.skip, L.active, sweeps_done, batch, st);
-      int active = 0;
-      QMX_HIP(h, hipMemcpyAsync(&active, L.active, sizeof(int), hipMemcpyDeviceToHost, st));
-      QMX_HIP(h, hipStreamSynchronize(st));
-      if (active == 0) break;
-    }
-    PhaseTimer swt(h, st, QMX_PHASE_SWEEP);
+    if (!any) break;
+    PhaseTimer swt(h, s0, QMX_PHASE_SWEEP);
     if (windowed) {
       for (size_t wi = 0; wi < wins.size(); ++wi) {
-        const Window& w = wins[wi];
-        WinPassArgs<S> pa{};
-        pa.n = n;
-        pa.lmax = lmax;
-        pa.t = std::min(n, kWinTileBits);
-        pa.s0 = w.s0;
-        pa.s1 = w.s1;
-        pa.p0 = wi ? wins[wi - 1].s0 : 0;
-        pa.p1 = wi ? wins[wi - 1].s1 : 0;
-        const uint64_t pmask = wi ? wins[wi - 1].umask : w.umask;
-        const uint64_t pbits = tile_bits(n, pa.t, w.umask | pmask);
-        pa.qmask = ((1ull << n) - 1ull) & ~pbits;
-        pa.gu = make_group(n, pbits, w.umask);
-        pa.gp = make_group(n, pbits, pmask);
-        pa.target = target;
-        pa.tprime = L.tprime;
-        pa.phi = L.phi;
-        pa.gates = gates;
-        pa.widths = widths;
-        pa.layers = layers_used;
-        pa.skip = skip;
-        pa.part = L.part;
-        PhaseTimer pt(h, st, QMX_PHASE_WINDOW_PASS);
-        launch_window_pass(pa, wblk, batch, st);
-        pt.stop(1);
-        WinFinArgs fa{};
-        fa.n = n;
-        fa.lmax = lmax;
-        fa.nblk = wblk;
-        fa.s0 = w.s0;
-        fa.s1 = w.s1;
-        fa.umask = w.umask;
-        fa.gates = gates;
-        fa.widths = widths;
-        fa.layers = layers_used;
-        fa.skip = skip;
-        fa.part = L.part;
-        fa.overlap = overlap ? overlap + s : nullptr;
-        fa.overlap_stride = nsweeps;
-        launch_window_finalize(fa, batch, st);
+        for (size_t j = 0; j < nj; ++j) {
+          SweepJob<S>& J = jobs[j];
+          if (!J.active) continue;
+          // passes of the two jobs never overlap: wait for the other job's last pass
+          if (nj > 1) QMX_HIP(h, hipStreamWaitEvent(J.st, h->pass_ev[1 - j], 0));
+          job_window_pass(h, J, wins, wi);
+          if (nj > 1) QMX_HIP(h, hipEventRecord(h->pass_ev[j], J.st));
+          job_window_finalize(h, J, wins, wi, s);
+        }
       }
-      swt.stop(static_cast<long long>(wins.size()));
       QMX_LAUNCHED(h, "windowed sweep");
-      continue;
+    } else {
+      for (auto& J : jobs)
+        if (J.active) job_step_sweep(h, J, s);
+      QMX_LAUNCHED(h, "sweep");
     }
-    for (int k = 0; k < kmax; ++k) {
-      SweepArgs<S> a{};
-      a.n = n;
-      a.lmax = lmax;
-      a.tbits = kSweepTileBits;
-      a.step = k;
-      a.target = target;
-      a.tprime = L.tprime;
-      a.phi = L.phi;
-      a.gates = gates;
-      a.widths = widths;
-      a.layers = layers_used;
-      a.skip = skip;
-      a.part = L.part;
-      a.counter = L.counter;
-      a.overlap = overlap ? overlap + s : nullptr;
-      a.overlap_stride = nsweeps;
-      launch_sweep_step(a, nblk, batch, st);
-    }
-    swt.stop(kmax);
-    QMX_LAUNCHED(h, "sweep");
+    for (size_t j = 1; j < nj; ++j) QMX_CALL(stream_join(h, h->join_ev[j], jobs[j].st, s0));
+    swt.stop(windowed ? (long long)wins.size() : (long long)kmax);
   }
   // final circuit state and fidelity
-  launch_basis0(L.phi, n, batch, nullptr, st);
-  if (use_fwd) {
-    QMX_CALL(run_fwd<S>(h, n, batch, lmax, fwd, L.phi, gates, widths, layers_used, nullptr,
-                        fidelity ? psi_ref : nullptr, L.dot_part, L.dot_counter,
-                        fidelity ? fidelity + nsweeps : nullptr, nsweeps + 1, st));
-  } else {
-    QMX_CALL(run_passes<S>(h, n, batch, lmax, 1, 0, rebuild, L.phi, gates, widths, layers_used,
-                           nullptr, fidelity ? psi_ref : nullptr, L.dot_part, L.dot_counter,
-                           fidelity ? fidelity + nsweeps : nullptr, nsweeps + 1, h->cus, st));
+  for (auto& J : jobs) {
+    QMX_CALL(job_rebuild(h, J, nsweeps, true));
+    if (J.circuit_out) launch_scale_convert<S, S>(J.L.phi, J.circuit_out, n, J.batch, nullptr, J.st);
   }
-  if (circuit_out) launch_scale_convert<S, S>(L.phi, circuit_out, n, batch, nullptr, st);
   QMX_LAUNCHED(h, "final circuit state");
   return QMX_OK;
+}
+
+bool sweep_windowed(const qmx_handle* h, int n) {
+  return (h->sweep_mode == QMX_SWEEP_WINDOW || h->sweep_mode == QMX_SWEEP_AUTO) && n >= kWinMinQubits;
+}
+
+template <typename S>
+int sweep_dispatch(qmx_handle* h, int n, int batch, int lmax, const int* layers_used, int nsweeps,
+                   const S* target, double2* gates, const int* widths, const double2* psi_ref,
+                   double2* fidelity, double* overlap, const double* goal, int* sweeps_done,
+                   S* circuit_out, Carver& c, hipStream_t st) {
+  const bool split = h->split_sweeps && batch >= 2 && sweep_windowed(h, n);
+  const size_t len = state_len(n);
+  auto job = [&](int b0, int nb, hipStream_t js) {
+    SweepJob<S> J{};
+    J.n = n;
+    J.batch = nb;
+    J.lmax = lmax;
+    J.nsweeps = nsweeps;
+    J.layers_used = layers_used + b0;
+    J.target = target + (size_t)b0 * len;
+    J.gates = gates + (size_t)b0 * lmax * n * 16;
+    J.widths = widths + (size_t)b0 * lmax * n;
+    J.psi_ref = psi_ref ? psi_ref + (size_t)b0 * len : nullptr;
+    J.fidelity = fidelity ? fidelity + (size_t)b0 * (nsweeps + 1) : nullptr;
+    J.overlap = overlap ? overlap + (size_t)b0 * nsweeps : nullptr;
+    J.goal = goal ? goal + b0 : nullptr;
+    J.sweeps_done = sweeps_done ? sweeps_done + b0 : nullptr;
+    J.circuit_out = circuit_out ? circuit_out + (size_t)b0 * len : nullptr;
+    J.L = sweep_layout<S>(c, n, nb);
+    J.st = js;
+    return J;
+  };
+  std::vector<SweepJob<S>> jobs;
+  if (!split) {
+    jobs.push_back(job(0, batch, st));
+    return run_sweep_jobs<S>(h, jobs);
+  }
+  for (int i = 0; i < 2; ++i) {
+    if (!h->aux[i]) QMX_HIP(h, hipStreamCreateWithFlags(&h->aux[i], hipStreamNonBlocking));
+    if (!h->pass_ev[i]) QMX_HIP(h, hipEventCreateWithFlags(&h->pass_ev[i], hipEventDisableTiming));
+    if (!h->join_ev[i]) QMX_HIP(h, hipEventCreateWithFlags(&h->join_ev[i], hipEventDisableTiming));
+  }
+  if (!h->call_ev) QMX_HIP(h, hipEventCreateWithFlags(&h->call_ev, hipEventDisableTiming));
+  const int b1 = (batch + 1) / 2;
+  jobs.push_back(job(0, b1, h->aux[0]));
+  jobs.push_back(job(b1, batch - b1, h->aux[1]));
+  // the auxiliary streams start after the caller's earlier work ...
+  QMX_HIP(h, hipEventRecord(h->call_ev, st));
+  for (int i = 0; i < 2; ++i) QMX_HIP(h, hipStreamWaitEvent(h->aux[i], h->call_ev, 0));
+  const int rc = run_sweep_jobs<S>(h, jobs);
+  // ... and the caller's later work waits for both
+  for (int i = 0; i < 2; ++i) QMX_CALL(stream_join(h, h->join_ev[i], h->aux[i], st));
+  return rc;
+}
+
+template <typename S>
+size_t sweep_bytes(int n, int batch) {
+  Carver one(nullptr);
+  sweep_layout<S>(one, n, batch);
+  Carver two(nullptr);
+  if (batch >= 2) {
+    sweep_layout<S>(two, n, (batch + 1) / 2);
+    sweep_layout<S>(two, n, batch / 2);
+  }
+  return std::max(one.off, two.off) + 256;
 }
 
 // ---- learned warm start: MLP parameter and activation layout -------------
@@ -811,6 +975,12 @@ int qmx_destroy(qmx_handle* h) {
   if (!h) return QMX_OK;
   for (auto& r : h->recs) { hipEventDestroy(r.start); hipEventDestroy(r.end); }
   for (auto e : h->pool) hipEventDestroy(e);
+  for (int i = 0; i < 2; ++i) {
+    if (h->aux[i]) hipStreamDestroy(h->aux[i]);
+    if (h->pass_ev[i]) hipEventDestroy(h->pass_ev[i]);
+    if (h->join_ev[i]) hipEventDestroy(h->join_ev[i]);
+  }
+  if (h->call_ev) hipEventDestroy(h->call_ev);
   delete h;
   return QMX_OK;
 }
@@ -822,6 +992,16 @@ int qmx_set_option(qmx_handle* h, int option, int value) {
   if (option == QMX_OPT_SWEEP_MODE) {
     if (value < QMX_SWEEP_AUTO || value > QMX_SWEEP_WINDOW) return fail(h, QMX_ERR_INVALID, "sweep mode");
     h->sweep_mode = value;
+    return QMX_OK;
+  }
+  if (option == QMX_OPT_WINDOW_BLOCKS_PER_CU) {
+    if (value < 1 || value > 16) return fail(h, QMX_ERR_INVALID, "window blocks per CU: 1..16");
+    h->win_blocks_per_cu = value;
+    return QMX_OK;
+  }
+  if (option == QMX_OPT_SPLIT_SWEEPS) {
+    if (value != 0 && value != 1) return fail(h, QMX_ERR_INVALID, "split sweeps: 0 or 1");
+    h->split_sweeps = value != 0;
     return QMX_OK;
   }
   return fail(h, QMX_ERR_INVALID, "unknown option");
@@ -957,10 +1137,8 @@ int qmx_extract_layers(qmx_handle* h, int num_qubits, int batch, int num_layers,
 size_t qmx_sweep_workspace(int num_qubits, int batch, int storage_dtype) {
   if (num_qubits < 2 || num_qubits > kMaxQubits || batch < 1) return 0;
   if (storage_dtype != QMX_C64 && storage_dtype != QMX_C128) return 0;
-  Carver c(nullptr);
-  if (storage_dtype == QMX_C64) sweep_layout<float2>(c, num_qubits, batch);
-  else sweep_layout<double2>(c, num_qubits, batch);
-  return c.off + 256;
+  return storage_dtype == QMX_C64 ? sweep_bytes<float2>(num_qubits, batch)
+                                  : sweep_bytes<double2>(num_qubits, batch);
 }
 
 int qmx_sweep(qmx_handle* h, int num_qubits, int batch, int num_layers, const int* layers_used,
@@ -978,20 +1156,15 @@ int qmx_sweep(qmx_handle* h, int num_qubits, int batch, int num_layers, const in
   QMX_HIP(h, hipSetDevice(h->device));
   hipStream_t st = static_cast<hipStream_t>(stream);
   Carver c(workspace);
-  if (storage_dtype == QMX_C64) {
-    SweepLayout<float2> L = sweep_layout<float2>(c, num_qubits, batch);
-    return run_sweep<float2>(h, num_qubits, batch, num_layers, layers_used, num_sweeps,
-                             static_cast<const float2*>(target), static_cast<double2*>(gates),
-                             widths, static_cast<const double2*>(psi_ref),
-                             static_cast<double2*>(fidelity), overlap, goal, sweeps_done,
-                             static_cast<float2*>(circuit_out), L, st);
-  }
-  SweepLayout<double2> L = sweep_layout<double2>(c, num_qubits, batch);
-  return run_sweep<double2>(h, num_qubits, batch, num_layers, layers_used, num_sweeps,
-                            static_cast<const double2*>(target), static_cast<double2*>(gates),
-                            widths, static_cast<const double2*>(psi_ref),
-                            static_cast<double2*>(fidelity), overlap, goal, sweeps_done,
-                            static_cast<double2*>(circuit_out), L, st);
+  if (storage_dtype == QMX_C64)
+    return sweep_dispatch<float2>(h, num_qubits, batch, num_layers, layers_used, num_sweeps,
+                                  static_cast<const float2*>(target), static_cast<double2*>(gates), widths,
+                                  static_cast<const double2*>(psi_ref), static_cast<double2*>(fidelity),
+                                  overlap, goal, sweeps_done, static_cast<float2*>(circuit_out), c, st);
+  return sweep_dispatch<double2>(h, num_qubits, batch, num_layers, layers_used, num_sweeps,
+                                 static_cast<const double2*>(target), static_cast<double2*>(gates), widths,
+                                 static_cast<const double2*>(psi_ref), static_cast<double2*>(fidelity),
+                                 overlap, goal, sweeps_done, static_cast<double2*>(circuit_out), c, st);
 }
 
 namespace {

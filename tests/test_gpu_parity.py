@@ -305,3 +305,23 @@ def test_prepare_then_sweeps_equals_encode(gpu):
     assert torch.equal(p.gates, ref.gates)
     assert torch.equal(r2.fidelity[:, -1], ref.fidelity[:, -1])
     assert torch.equal(r1.fidelity[:, 0], ref.fidelity[:, 0])
+
+
+@pytest.mark.parametrize("batch", [2, 5])
+def test_split_sweeps_equal_single_stream(gpu, batch):
+    """Two interleaved half-batches on two streams give the single-stream
+    results (up to the order of the per-block partial sums)."""
+    from qmprs_amd import _lib
+    eng = _engine()
+    n, chi, L, S = 11, 16, 3, 3
+    svs = np.stack([O.random_mps_state(n, 24, 60 + s) for s in range(batch)])
+    _lib.set_split_sweeps(False)
+    try:
+        one = eng.encode(svs, chi, L, S, sweep_dtype=torch.complex128)
+    finally:
+        _lib.set_split_sweeps(True)
+    two = eng.encode(svs, chi, L, S, sweep_dtype=torch.complex128)
+    torch.cuda.synchronize()
+    assert torch.equal(one.layers_used, two.layers_used)
+    assert (one.gates - two.gates).abs().max().item() < 1e-10
+    assert (one.fidelity - two.fidelity).abs().max().item() < 1e-12
