@@ -146,6 +146,57 @@ def warm_start_leg(svs, plain_res, wl, args) -> dict:
     }
 
 
+# ragged batch: 256 states, N ~ U{12..20}, generating bond D ~ U{16, 32, 64},
+# bond_dimension = D, 8 layers, 30 sweeps, complex64
+RAGGED = {"states": 256, "num_qubits": (12, 20), "bonds": (16, 32, 64), "num_layers": 8,
+          "num_sweeps": 30, "seed": 7}
+
+
+def ragged_main(args) -> int:
+    import torch
+    from qmprs_amd import _lib, engine
+    rg = RAGGED
+    rng = np.random.default_rng(rg["seed"])
+    ns = rng.integers(rg["num_qubits"][0], rg["num_qubits"][1] + 1, rg["states"])
+    ds = rng.choice(rg["bonds"], rg["states"])
+    svs = [_states(int(n), int(d), 1, seed0=10000 + i)[0] for i, (n, d) in enumerate(zip(ns, ds))]
+    _lib.require_gpu()
+    dev_svs = [engine.as_device_states(v) for v in svs]
+    L, S = rg["num_layers"], rg["num_sweeps"]
+
+    def run(conc):
+        return engine.encode_ragged(dev_svs, [int(d) for d in ds], L, S, max_concurrency=conc)
+
+    out = {}
+    for conc in (1, 4):
+        for _ in range(args.warmup):
+            run(conc)
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        for _ in range(args.steps):
+            res = run(conc)
+        torch.cuda.synchronize()
+        out[conc] = (time.perf_counter() - t0) / args.steps
+    fid = np.array([abs(r.fidelity[-1].item()) for r in res])
+    buckets = len(set(zip(ns.tolist(), ds.tolist())))
+    value = rg["states"] / out[4]
+    line = {
+        "metric": "state-preps/sec, ragged batch of 256 states (N 12..20, chi 16/32/64)",
+        "value": round(value, 3), "unit": "state-preps/s", "n_gpus": 1, "steps": args.steps,
+        "warmup": args.warmup, "ms_per_step": round(1000 * out[4], 2), "higher_is_better": True,
+        "scaling": "weak", "vs_baseline": None,
+        "dtype": "complex64 sweep storage, fp64 arithmetic and reductions",
+        "data": "synthetic: seeded random-MPS statevectors (complex Gaussian, generating bond = chi)",
+        "config": {"workload": "ragged batch: 256 states, N ~ U{12..20}, bond_dimension = generating "
+                               "bond ~ U{16,32,64}, num_layers=8, num_sweeps=30", "buckets": buckets,
+                   "max_concurrency": 4},
+        "sequential_buckets": {"value": round(rg["states"] / out[1], 3), "ms_per_step": round(1000 * out[1], 2)},
+        "fidelity": {"mean": float(fid.mean()), "min": float(fid.min()), "max": float(fid.max())},
+    }
+    print(json.dumps(line))
+    return 0
+
+
 def main() -> int:
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -164,6 +215,9 @@ def main() -> int:
     ap.add_argument("--copy-calibration", action="store_true",
                     help="before the run, copy a 1 GiB buffer 3 times (a plain copy "
                          "kernel of known size for calibrating HBM counters)")
+    ap.add_argument("--workload", choices=("single-size", "ragged"), default="single-size",
+                    help="single-size: the N=20, chi=64 batch (default); ragged: 256 states of "
+                         "mixed N and chi through the ragged scheduler")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py runs on one GPU only: this encoder shards across GPUs by state "
@@ -173,6 +227,8 @@ def main() -> int:
     import torch
     sys.path.insert(0, os.path.dirname(os.path.abspath(__file__)))
     from qmprs_amd import _lib, engine
+    if args.workload == "ragged":
+        return ragged_main(args)
 
     wl = WORKLOAD
     n, chi, L, S = wl["num_qubits"], wl["bond_dimension"], wl["num_layers"], wl["num_sweeps"]

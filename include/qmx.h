@@ -65,6 +65,9 @@ const char* qmx_last_error(const qmx_handle* h);
  * compute unit over the whole batch (1..16, default 16). */
 #define QMX_OPT_SPLIT_SWEEPS 2
 #define QMX_OPT_WINDOW_BLOCKS_PER_CU 3
+/* QMX_OPT_FWD_BLOCKS_PER_CU: grid of the forward-circuit rebuild passes, in
+ * blocks per compute unit over the whole batch (1..16, default 2). */
+#define QMX_OPT_FWD_BLOCKS_PER_CU 4
 int qmx_set_option(qmx_handle* h, int option, int value);
 
 /* Phase timers (no reference counterpart: instrumentation for benchmarks).
@@ -188,6 +191,35 @@ int qmx_encode(qmx_handle* h, const qmx_encode_config* cfg, const void* statevec
 int qmx_encode_prepare(qmx_handle* h, const qmx_encode_config* cfg, const void* statevectors,
                        void* gates, int* widths, int* layers_used, void* psi_out, void* target_out,
                        void* workspace, size_t workspace_bytes, void* stream);
+
+/* Ragged batches (no reference counterpart: the reference encodes one state
+ * per prepare_state call, base.py:58-106).  Each state brings its own qubit
+ * count, bond dimension and device buffers; states with equal (N, chi) are
+ * encoded together as one batch (qmx_encode), and the buckets, largest
+ * estimated cost first, run on up to max_concurrency worker threads with
+ * their own handles and streams, so they overlap on the GPU.  All states
+ * share num_layers, num_sweeps and the dtypes.  Synchronises (returns when
+ * every state is done).
+ *   gates        device, num_layers x N x 16 complex128 (qmx gate layout)
+ *   widths       device, num_layers x N ints
+ *   layers_used  device, 1 int
+ *   fidelity     device, num_sweeps + 1 complex128 (see qmx_sweep) */
+typedef struct qmx_ragged_state {
+  int num_qubits;
+  int bond_dimension;
+  const void* statevector;
+  void* gates;
+  int* widths;
+  int* layers_used;
+  void* fidelity;
+} qmx_ragged_state;
+
+size_t qmx_encode_ragged_workspace(int count, const qmx_ragged_state* states, int num_layers,
+                                   int num_sweeps, int sv_dtype, int sweep_dtype);
+int qmx_encode_ragged(qmx_handle* h, int count, const qmx_ragged_state* states, int num_layers,
+                      int num_sweeps, int sv_dtype, int sweep_dtype, double fidelity_threshold,
+                      double cutoff, int max_concurrency, void* workspace, size_t workspace_bytes,
+                      void* stream);
 
 /* ---- Learned warm start (no reference counterpart) ------------------------
  * A 3-layer ReLU MLP (BF16 weights, FP32 accumulation on MFMA) maps a feature

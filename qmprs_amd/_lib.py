@@ -53,6 +53,19 @@ class EncodeConfig(ctypes.Structure):
     ]
 
 
+class RaggedState(ctypes.Structure):
+    """Mirror of ``qmx_ragged_state``."""
+    _fields_ = [
+        ("num_qubits", _INT),
+        ("bond_dimension", _INT),
+        ("statevector", _VP),
+        ("gates", _VP),
+        ("widths", _VP),
+        ("layers_used", _VP),
+        ("fidelity", _VP),
+    ]
+
+
 # name -> (restype, argtypes)
 SIGNATURES = {
     "qmx_version": (_INT, []),
@@ -78,6 +91,9 @@ SIGNATURES = {
                           _SZ, _VP]),
     "qmx_encode_prepare": (_INT, [_VP, ctypes.POINTER(EncodeConfig), _VP, _VP, _VP, _VP, _VP, _VP, _VP,
                                   _SZ, _VP]),
+    "qmx_encode_ragged_workspace": (_SZ, [_INT, ctypes.POINTER(RaggedState), _INT, _INT, _INT, _INT]),
+    "qmx_encode_ragged": (_INT, [_VP, _INT, ctypes.POINTER(RaggedState), _INT, _INT, _INT, _INT, _DBL, _DBL,
+                                 _INT, _VP, _SZ, _VP]),
     "qmx_mlp_param_count": (_SZ, [_INT, _INT, _INT]),
     "qmx_mlp_workspace": (_SZ, [_INT, _INT, _INT, _INT]),
     "qmx_mlp_forward": (_INT, [_VP, _INT, _INT, _INT, _INT, _VP, _VP, _VP, _VP, _VP, _SZ, _VP]),
@@ -177,6 +193,7 @@ def profile_read() -> dict:
 
 QMX_OPT_SPLIT_SWEEPS = 2
 QMX_OPT_WINDOW_BLOCKS_PER_CU = 3
+QMX_OPT_FWD_BLOCKS_PER_CU = 4
 
 
 def set_option(option: int, value: int) -> None:

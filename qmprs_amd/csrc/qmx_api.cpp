@@ -33,6 +33,7 @@ struct qmx_handle {
   std::vector<hipEvent_t> pool;
   bool split_sweeps = true;             // QMX_OPT_SPLIT_SWEEPS
   int win_blocks_per_cu = 16;           // QMX_OPT_WINDOW_BLOCKS_PER_CU
+  int fwd_blocks_per_cu = 2;            // QMX_OPT_FWD_BLOCKS_PER_CU
   hipStream_t aux[2] = {nullptr, nullptr};
   hipEvent_t pass_ev[2] = {nullptr, nullptr};
   hipEvent_t join_ev[2] = {nullptr, nullptr};
@@ -549,7 +550,8 @@ int run_fwd(qmx_handle* h, int n, int batch, int lmax, const std::vector<FwdPass
             const double2* ref, double* dot_part, int* dot_counter, double2* dot_out,
             int dot_stride, hipStream_t st) {
   const int t = std::min(n, kFwdTileBits);
-  const int nblk = blocks_for(1l << (n - t), batch, h->cus);
+  const long ntiles = 1l << (n - t);
+  const int nblk = (int)std::max(1l, std::min(ntiles, ((long)h->fwd_blocks_per_cu * h->cus + batch - 1) / batch));
   for (size_t pi = 0; pi < passes.size(); ++pi) {
     const FwdPass& fp = passes[pi];
     uint64_t um = 0;
@@ -950,6 +952,11 @@ int gate_rows_check(qmx_handle* h, int n, int batch, int num_layers, GateRowArgs
 
 }  // namespace
 
+void qmx_set_error_text(qmx_handle* h, const char* msg) {
+  if (h) h->err = msg ? msg : "";
+}
+int qmx_handle_device(const qmx_handle* h) { return h ? h->device : 0; }
+
 // =================================================================== C-ABI ==
 extern "C" {
 
@@ -997,6 +1004,11 @@ int qmx_set_option(qmx_handle* h, int option, int value) {
   if (option == QMX_OPT_WINDOW_BLOCKS_PER_CU) {
     if (value < 1 || value > 16) return fail(h, QMX_ERR_INVALID, "window blocks per CU: 1..16");
     h->win_blocks_per_cu = value;
+    return QMX_OK;
+  }
+  if (option == QMX_OPT_FWD_BLOCKS_PER_CU) {
+    if (value < 1 || value > 16) return fail(h, QMX_ERR_INVALID, "forward blocks per CU: 1..16");
+    h->fwd_blocks_per_cu = value;
     return QMX_OK;
   }
   if (option == QMX_OPT_SPLIT_SWEEPS) {

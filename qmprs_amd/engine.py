@@ -223,6 +223,53 @@ def encode_prepare(statevectors, bond_dimension: int, num_layers: int = 1,
     return Prepared(psi, target, gates, widths, used)
 
 
+@dataclass
+class RaggedResult:
+    gates: torch.Tensor        # (L, N, 4, 4) complex128
+    widths: torch.Tensor       # (L, N) int32
+    layers_used: int
+    fidelity: torch.Tensor     # (S + 1,) complex128
+
+
+def encode_ragged(statevectors, bond_dimensions, num_layers: int = 1, num_sweeps: int = 0,
+                  fidelity_threshold: float = DEFAULT_THRESHOLD, cutoff: float = CUTOFF,
+                  sweep_dtype=torch.complex64, max_concurrency: int = 4) -> list[RaggedResult]:
+    """prepare_state for states of different sizes in one call (qmx_encode_ragged).
+
+    ``statevectors``: sequence of 1-D arrays / tensors (any power-of-two
+    lengths); ``bond_dimensions``: one int per state, or one int for all."""
+    dev = _lib.require_gpu()
+    svs = [as_device_states(v) for v in statevectors]
+    dtype = torch.complex128 if any(v.dtype == torch.complex128 for v in svs) else torch.complex64
+    svs = [v.to(dtype).contiguous() for v in svs]
+    if isinstance(bond_dimensions, int):
+        bond_dimensions = [bond_dimensions] * len(svs)
+    if len(bond_dimensions) != len(svs):
+        raise ValueError("one bond dimension per state")
+    outs = []
+    arr = (_lib.RaggedState * len(svs))()
+    for i, (v, chi) in enumerate(zip(svs, bond_dimensions)):
+        n = num_qubits_of(v.shape[1])
+        g = torch.zeros((num_layers, n, 4, 4), dtype=torch.complex128, device=dev)
+        w = torch.ones((num_layers, n), dtype=torch.int32, device=dev)
+        u = torch.zeros(1, dtype=torch.int32, device=dev)
+        f = torch.zeros(num_sweeps + 1, dtype=torch.complex128, device=dev)
+        arr[i] = _lib.RaggedState(n, int(chi), v.data_ptr(), g.data_ptr(), w.data_ptr(), u.data_ptr(),
+                                  f.data_ptr())
+        outs.append((g, w, u, f))
+    lib, h = _lib.load(), _lib.handle()
+    sv_dt, sw_dt = _DTYPES[dtype], _DTYPES[sweep_dtype]
+    need = lib.qmx_encode_ragged_workspace(len(svs), arr, num_layers, num_sweeps, sv_dt, sw_dt)
+    if need == 0:
+        raise ValueError("invalid ragged batch")
+    ws = _lib.workspace(need, dev)
+    st = lib.qmx_encode_ragged(h, len(svs), arr, num_layers, num_sweeps, sv_dt, sw_dt,
+                               float(fidelity_threshold), float(cutoff), int(max_concurrency),
+                               _lib.ptr(ws), ws.numel(), _lib.stream_ptr())
+    _lib.check(st, h)
+    return [RaggedResult(g, w, int(u.item()), f) for g, w, u, f in outs]
+
+
 def gates_to_layers(gates: np.ndarray, widths: np.ndarray, layers_used: int):
     """Device gate arrays of one state -> list[UnitaryLayer] in circuit order.
 

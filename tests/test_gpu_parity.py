@@ -325,3 +325,31 @@ def test_split_sweeps_equal_single_stream(gpu, batch):
     assert torch.equal(one.layers_used, two.layers_used)
     assert (one.gates - two.gates).abs().max().item() < 1e-10
     assert (one.fidelity - two.fidelity).abs().max().item() < 1e-12
+
+
+def test_encode_ragged_equals_bucket_encodes(gpu):
+    """qmx_encode_ragged: states of different (N, chi), shuffled, give what
+    qmx_encode gives on each (N, chi) bucket (bit for bit: same batches)."""
+    eng = _engine()
+    spec = [(9, 8), (11, 16), (9, 8), (12, 8), (11, 16), (10, 4), (9, 8)]
+    svs = [O.random_mps_state(n, 2 * chi, 70 + i).astype(np.complex64) for i, (n, chi) in enumerate(spec)]
+    L, S = 3, 2
+    res = eng.encode_ragged(svs, [c for _, c in spec], L, S, max_concurrency=3)
+    buckets = {}
+    for i, (n, chi) in enumerate(spec):
+        buckets.setdefault((n, chi), []).append(i)
+    for (n, chi), idx in buckets.items():
+        ref = eng.encode(np.stack([svs[i] for i in idx]), chi, L, S)
+        for k, i in enumerate(idx):
+            assert res[i].layers_used == int(ref.layers_used[k])
+            assert torch.equal(res[i].gates, ref.gates[k])
+            assert torch.equal(res[i].widths, ref.widths[k])
+            assert torch.equal(res[i].fidelity, ref.fidelity[k])
+
+
+def test_encode_ragged_rejects_bad_input(gpu):
+    eng = _engine()
+    with pytest.raises(ValueError):
+        eng.encode_ragged([np.ones(8, np.complex64)], [4, 4], 2, 1)
+    with pytest.raises(ValueError):
+        eng.encode_ragged([np.ones(8, np.complex64)], [0], 2, 1)
