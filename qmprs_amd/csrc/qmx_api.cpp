@@ -194,6 +194,8 @@ struct ChiLayout {
   int* kk;
   int* ncur;
   int* error;
+  double* jlog;      // LDS Jacobi rotation log (nmax <= 128)
+  int* jstate;
   int i_first, nmax, nchunks_max, nblk_norm;
 };
 
@@ -220,6 +222,10 @@ ChiLayout chi_layout(Carver& c, int n, int batch, int chi, bool need_psi) {
     L.v = c.take<double2>(nm2 * batch);
     L.w = c.take<double2>(nm2 * batch * (i_first + 1));
     L.part = c.take<double2>(nm2 * batch * L.nchunks_max);
+    if (jacobi_lds_fits(L.nmax)) {
+      L.jlog = c.take<double>(jacobi_lds_log_doubles(L.nmax) * batch);
+      L.jstate = c.take<int>(2 * static_cast<size_t>(batch));
+    }
   }
   L.nn = c.take<int>(static_cast<size_t>(batch) * (n + 1));
   L.kk = c.take<int>(static_cast<size_t>(batch) * (n + 1));
@@ -239,6 +245,7 @@ struct LayerLayout {
   int* error;
 };
 constexpr int kChain2MaxBlocks = 256;
+constexpr int kJacobiSweeps = 30;   // sweep launches of the LDS Jacobi per bond
 constexpr int kWinMaxBlocks = 256;
 
 // window-pass blocks per state: about win_blocks_per_cu resident blocks per CU
@@ -426,7 +433,10 @@ int run_chi(qmx_handle* h, int n, int batch, int chi, double cutoff, const void*
     int nchunks = std::max(1, std::min(L.nchunks_max, rows / 256));
     launch_gram(y, len, L.ncur, i, nmax, L.part, nchunks, batch, st);
     launch_gram_reduce(L.part, nchunks, L.ncur, nmax, L.a, nm2, batch, st);
-    launch_jacobi(L.a, L.v, nm2, L.ncur, nmax, batch, L.error, st);
+    if (L.jlog && jacobi_lds_setup(nmax) == hipSuccess)
+      launch_jacobi_lds(L.a, L.v, nm2, L.ncur, nmax, batch, L.jlog, L.jstate, kJacobiSweeps, L.error, st);
+    else
+      launch_jacobi(L.a, L.v, nm2, L.ncur, nmax, batch, L.error, st);
     double2* wi = L.w + static_cast<size_t>(i) * batch * nm2;
     launch_select(L.a, L.v, wi, nm2, L.ncur, nmax, chi, cutoff, L.kk + i, n + 1,
                   spectra ? spectra + static_cast<size_t>(i) * 2 * chi : nullptr,

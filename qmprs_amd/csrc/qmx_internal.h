@@ -234,6 +234,12 @@ void launch_gram(const double2* y, size_t y_stride, const int* ncols_dev, int ro
                  int ncols_max, double2* part, int nchunks, int batch, hipStream_t);
 void launch_gram_reduce(const double2* part, int nchunks, const int* ncols_dev, int ncols_max,
                         double2* a, size_t a_stride, int batch, hipStream_t);
+// LDS-resident Jacobi for matrices up to 128 x 128 (qmx_linalg.hip)
+size_t jacobi_lds_log_doubles(int ncols_max);
+bool jacobi_lds_fits(int ncols_max);
+hipError_t jacobi_lds_setup(int ncols_max);
+void launch_jacobi_lds(double2* a, double2* v, size_t stride, const int* ncols_dev, int ncols_max,
+                       int batch, double* log, int* state, int max_sweeps, int* error, hipStream_t st);
 void launch_jacobi(double2* a, double2* v, size_t stride, const int* ncols_dev, int ncols_max,
                    int batch, int* error, hipStream_t);
 void launch_select(const double2* a, const double2* v, double2* w, size_t stride,

@@ -173,6 +173,191 @@ __global__ __launch_bounds__(512) void jacobi_kernel(double2* a_all, double2* v_
   if (!converged && tid == 0 && error) atomicOr(error, kErrEigNoConv);
 }
 
+// ------------------------------------------------ LDS-resident Jacobi -------
+// For n <= kJacobiLdsMax the upper triangle of A (n (n + 1) / 2 complex128,
+// 132 KiB at n = 128) lives in LDS for a whole sweep: one block per state,
+// the same parallel (circle) ordering and rotations as jacobi_kernel.  Each
+// sweep logs its rotations (c, s, e^{-i phi} per pair and round); a second
+// kernel applies the log to the rows of V, 16 rows per block held in LDS.
+// The host launches sweep / update pairs; a state whose sweep rotated nothing
+// is converged and every later launch returns at once for it.
+constexpr int kJacobiLdsMax = 128;
+constexpr int kJacobiRowsPerBlock = 16;
+
+QMX_DEV int tri_index(int i, int j, int n) { return i * (2 * n - i + 1) / 2 + (j - i); }
+
+__global__ __launch_bounds__(1024) void jacobi_lds_sweep_kernel(double2* a_all, size_t stride,
+                                                                const int* ncols_dev, int ld,
+                                                                double4* log_all, size_t log_stride,
+                                                                int* state, int sweep) {
+  extern __shared__ double2 tri[];
+  __shared__ int pp[kJacobiLdsMax / 2], qq[kJacobiLdsMax / 2];
+  __shared__ double jc[kJacobiLdsMax / 2], js[kJacobiLdsMax / 2];
+  __shared__ double2 je[kJacobiLdsMax / 2];
+  __shared__ int any_rot;
+  __shared__ double s_scale;
+  const int b = blockIdx.x;
+  // state[b]: 0 = running, 1 = converged; state[batch + b] = sweeps run
+  if (state[b]) return;
+  const int n = ncols_dev[b];
+  if (n < 2) {
+    if (threadIdx.x == 0) state[b] = 1;
+    return;
+  }
+  double2* a = a_all + (size_t)b * stride;
+  double4* lg = log_all + (size_t)b * log_stride;
+  const int tid = threadIdx.x, nt = blockDim.x;
+  const int ntri = n * (n + 1) / 2;
+  for (int e = tid; e < ntri; e += nt) {
+    // row i: the largest i with tri_index(i, i) <= e
+    int i = (int)((2.0 * n + 1.0 - sqrt((2.0 * n + 1.0) * (2.0 * n + 1.0) - 8.0 * e)) * 0.5);
+    while (i > 0 && tri_index(i, i, n) > e) --i;
+    while (i + 1 < n && tri_index(i + 1, i + 1, n) <= e) ++i;
+    const int j = i + (e - tri_index(i, i, n));
+    tri[e] = a[(size_t)i * ld + j];
+  }
+  if (tid == 0) {
+    any_rot = 0;
+    double tr = 0.0;
+    for (int i = 0; i < n; ++i) tr += fabs(a[(size_t)i * ld + i].x);
+    s_scale = tr;
+  }
+  __syncthreads();
+  const int np = n / 2;
+  const double abs_tol = 1e-17 * s_scale + 1e-300;
+  for (int r = 0; r < n - 1; ++r) {
+    if (tid < np) {
+      const int m = tid;
+      int p = (m == 0) ? n - 1 : (r + m) % (n - 1);
+      int q = (r + n - 1 - m) % (n - 1);
+      if (m == 0) q = r % (n - 1);
+      if (p > q) { int t = p; p = q; q = t; }
+      const double al = tri[tri_index(p, p, n)].x, be = tri[tri_index(q, q, n)].x;
+      const double2 g = tri[tri_index(p, q, n)];
+      const double ag = cabs_(g);
+      double c = 1.0, sn = 0.0;
+      double2 em = make_double2(1.0, 0.0);
+      if (ag > abs_tol && ag > 1e-15 * sqrt(fabs(al * be))) {
+        const double tau = (be - al) / (2.0 * ag);
+        const double t = (tau >= 0 ? 1.0 : -1.0) / (fabs(tau) + sqrt(1.0 + tau * tau));
+        c = 1.0 / sqrt(1.0 + t * t);
+        sn = t * c;
+        em = make_double2(g.x / ag, -g.y / ag);   // e^{-i phi}
+        any_rot = 1;
+      }
+      pp[m] = p; qq[m] = q; jc[m] = c; js[m] = sn; je[m] = em;
+      lg[(size_t)r * np + m] = make_double4(c, sn, em.x, em.y);
+    }
+    __syncthreads();
+    // blocks (P, Q), P <= Q: B' = J_P^H B J_Q with B = A[{p0, p1}][{q0, q1}];
+    // (Q, P) is its conjugate transpose and is never stored separately
+    for (int e = tid; e < np * np; e += nt) {
+      const int P = e / np, Q = e % np;
+      if (P > Q) continue;
+      const int p0 = pp[P], p1 = qq[P], q0 = pp[Q], q1 = qq[Q];
+      auto get = [&](int i, int j) {
+        return i <= j ? tri[tri_index(i, j, n)] : cconj(tri[tri_index(j, i, n)]);
+      };
+      auto put = [&](int i, int j, double2 v) {
+        if (i <= j) tri[tri_index(i, j, n)] = v;
+        else tri[tri_index(j, i, n)] = cconj(v);
+      };
+      const double2 J00 = make_double2(jc[Q], 0.0), J01 = make_double2(js[Q], 0.0);
+      const double2 J10 = cscale(je[Q], -js[Q]), J11 = cscale(je[Q], jc[Q]);
+      const double2 K00 = make_double2(jc[P], 0.0), K01 = make_double2(js[P], 0.0);
+      const double2 K10 = cscale(je[P], -js[P]), K11 = cscale(je[P], jc[P]);
+      const double2 a00 = get(p0, q0), a01 = get(p0, q1), a10 = get(p1, q0), a11 = get(p1, q1);
+      const double2 b00 = cadd(cmul(a00, J00), cmul(a01, J10));
+      const double2 b01 = cadd(cmul(a00, J01), cmul(a01, J11));
+      const double2 b10 = cadd(cmul(a10, J00), cmul(a11, J10));
+      const double2 b11 = cadd(cmul(a10, J01), cmul(a11, J11));
+      double2 c00 = cadd(cmulc(K00, b00), cmulc(K10, b10));
+      double2 c01 = cadd(cmulc(K00, b01), cmulc(K10, b11));
+      double2 c10 = cadd(cmulc(K01, b00), cmulc(K11, b10));
+      double2 c11 = cadd(cmulc(K01, b01), cmulc(K11, b11));
+      if (P == Q) {
+        c00.y = 0.0; c11.y = 0.0;
+        c01 = cz();
+        put(p0, q0, c00); put(p0, q1, c01); put(p1, q1, c11);
+      } else {
+        put(p0, q0, c00); put(p0, q1, c01); put(p1, q0, c10); put(p1, q1, c11);
+      }
+    }
+    __syncthreads();
+  }
+  // write the triangle back (the lower part stays stale: readers use i <= j)
+  for (int e = tid; e < ntri; e += nt) {
+    int i = (int)((2.0 * n + 1.0 - sqrt((2.0 * n + 1.0) * (2.0 * n + 1.0) - 8.0 * e)) * 0.5);
+    while (i > 0 && tri_index(i, i, n) > e) --i;
+    while (i + 1 < n && tri_index(i + 1, i + 1, n) <= e) ++i;
+    const int j = i + (e - tri_index(i, i, n));
+    a[(size_t)i * ld + j] = tri[e];
+  }
+  if (tid == 0) {
+    state[gridDim.x + b] = sweep + 1;
+    // a sweep that rotated nothing leaves the log at identity: converged
+    if (!any_rot) state[b] = 1;
+  }
+}
+
+// V <- V J over one sweep's log; rows [16 blockIdx.x, +16) of state blockIdx.y
+__global__ __launch_bounds__(1024) void jacobi_lds_rows_kernel(double2* v_all, size_t stride,
+                                                               const int* ncols_dev, int ld,
+                                                               const double4* log_all, size_t log_stride,
+                                                               const int* state, int sweep) {
+  __shared__ double2 rows[kJacobiRowsPerBlock][kJacobiLdsMax];
+  const int b = blockIdx.y;
+  // the sweep ran for this state iff its sweep counter reached sweep + 1
+  if (state[gridDim.y + b] != sweep + 1) return;
+  const int n = ncols_dev[b];
+  const int r0 = blockIdx.x * kJacobiRowsPerBlock;
+  if (n < 2 || r0 >= n) return;
+  double2* v = v_all + (size_t)b * stride;
+  const double4* lg = log_all + (size_t)b * log_stride;
+  const int nr = min(kJacobiRowsPerBlock, n - r0);
+  const int np = n / 2;
+  for (int e = threadIdx.x; e < nr * n; e += blockDim.x) rows[e / n][e % n] = v[(size_t)(r0 + e / n) * ld + e % n];
+  __syncthreads();
+  for (int r = 0; r < n - 1; ++r) {
+    for (int e = threadIdx.x; e < nr * np; e += blockDim.x) {
+      const int i = e / np, m = e % np;
+      int p = (m == 0) ? n - 1 : (r + m) % (n - 1);
+      int q = (r + n - 1 - m) % (n - 1);
+      if (m == 0) q = r % (n - 1);
+      if (p > q) { int t = p; p = q; q = t; }
+      const double4 R = lg[(size_t)r * np + m];
+      const double2 em = make_double2(R.z, R.w);
+      const double2 J00 = make_double2(R.x, 0.0), J01 = make_double2(R.y, 0.0);
+      const double2 J10 = cscale(em, -R.y), J11 = cscale(em, R.x);
+      const double2 x = rows[i][p], y = rows[i][q];
+      rows[i][p] = cadd(cmul(x, J00), cmul(y, J10));
+      rows[i][q] = cadd(cmul(x, J01), cmul(y, J11));
+    }
+    __syncthreads();
+  }
+  for (int e = threadIdx.x; e < nr * n; e += blockDim.x) v[(size_t)(r0 + e / n) * ld + e % n] = rows[e / n][e % n];
+}
+
+__global__ void jacobi_lds_init_kernel(double2* v_all, size_t stride, const int* ncols_dev, int ld, int* state,
+                                       int batch) {
+  const int b = blockIdx.y;
+  const int n = ncols_dev[b];
+  double2* v = v_all + (size_t)b * stride;
+  for (int e = blockIdx.x * blockDim.x + threadIdx.x; e < n * n; e += gridDim.x * blockDim.x) {
+    const int i = e / n, j = e % n;
+    v[(size_t)i * ld + j] = make_double2(i == j ? 1.0 : 0.0, 0.0);
+  }
+  if (blockIdx.x == 0 && threadIdx.x == 0) {
+    state[b] = 0;
+    state[batch + b] = 0;
+  }
+}
+
+__global__ void jacobi_lds_check_kernel(const int* state, int batch, int* error) {
+  const int b = blockIdx.x * blockDim.x + threadIdx.x;
+  if (b < batch && !state[b] && error) atomicOr(error, kErrEigNoConv);
+}
+
 // ---------------------------------------------------------------- select ----
 // Sort eigenvalues (diag of A) in decreasing order, pick k (rsum2 cutoff and
 // max bond chi; k = n when n <= chi), and write the sorted eigenvectors into
@@ -317,6 +502,36 @@ void launch_jacobi(double2* a, double2* v, size_t stride, const int* ncols_dev, 
                    int batch, int* error, hipStream_t st) {
   hipLaunchKernelGGL(jacobi_kernel, dim3(batch), dim3(512), 0, st, a, v, stride, ncols_dev,
                      ncols_max, error);
+}
+
+size_t jacobi_lds_log_doubles(int ncols_max) {
+  return ncols_max <= kJacobiLdsMax ? (size_t)(ncols_max - 1) * (ncols_max / 2) * 4 : 0;
+}
+
+bool jacobi_lds_fits(int ncols_max) { return ncols_max >= 2 && ncols_max <= kJacobiLdsMax; }
+
+void launch_jacobi_lds(double2* a, double2* v, size_t stride, const int* ncols_dev, int ncols_max,
+                       int batch, double* log, int* state, int max_sweeps, int* error, hipStream_t st) {
+  const size_t tri_bytes = sizeof(double2) * (size_t)ncols_max * (ncols_max + 1) / 2;
+  const size_t log_stride = (size_t)(ncols_max - 1) * (ncols_max / 2);
+  double4* lg = reinterpret_cast<double4*>(log);
+  hipLaunchKernelGGL(jacobi_lds_init_kernel, dim3((ncols_max * ncols_max + 255) / 256, batch), dim3(256), 0,
+                     st, v, stride, ncols_dev, ncols_max, state, batch);
+  const int row_blocks = (ncols_max + kJacobiRowsPerBlock - 1) / kJacobiRowsPerBlock;
+  for (int s = 0; s < max_sweeps; ++s) {
+    hipLaunchKernelGGL(jacobi_lds_sweep_kernel, dim3(batch), dim3(1024), tri_bytes, st, a, stride,
+                       ncols_dev, ncols_max, lg, log_stride, state, s);
+    hipLaunchKernelGGL(jacobi_lds_rows_kernel, dim3(row_blocks, batch), dim3(1024), 0, st, v, stride,
+                       ncols_dev, ncols_max, lg, log_stride, state, s);
+  }
+  hipLaunchKernelGGL(jacobi_lds_check_kernel, dim3((batch + 255) / 256), dim3(256), 0, st, state, batch,
+                     error);
+}
+
+hipError_t jacobi_lds_setup(int ncols_max) {
+  const size_t tri_bytes = sizeof(double2) * (size_t)ncols_max * (ncols_max + 1) / 2;
+  return hipFuncSetAttribute(reinterpret_cast<const void*>(jacobi_lds_sweep_kernel),
+                             hipFuncAttributeMaxDynamicSharedMemorySize, (int)tri_bytes);
 }
 
 void launch_select(const double2* a, const double2* v, double2* w, size_t stride,
