@@ -158,6 +158,17 @@ int qmx_sweep(qmx_handle* h, int num_qubits, int batch, int num_layers, const in
               const double* goal, int* sweeps_done, void* circuit_out, void* workspace,
               size_t workspace_bytes, void* stream);
 
+/* C|0..0> of a batch of gate lists on the GPU: the state a circuit backend's
+ * get_statevector() returns for the emitted circuit (used by the reference's
+ * tests, test_sequential_encoding.py:64-67), and the forward layer
+ * application of mps.py:893-931 from |0..0>.  Workspace: qmx_sweep_workspace.
+ *   state_out  device, B x 2^N of storage_dtype
+ *   psi_ref / fidelity  optional (both or neither): fidelity[b] = <psi_ref|C|0> */
+int qmx_circuit_state(qmx_handle* h, int num_qubits, int batch, int num_layers, const int* layers_used,
+                      const void* gates, const int* widths, int storage_dtype, const void* psi_ref,
+                      void* fidelity, void* state_out, void* workspace, size_t workspace_bytes,
+                      void* stream);
+
 /* Whole Sequential.prepare_state (base.py:58-106 -> sequential.py:543-600)
  * for a batch of same-size states: chi-truncation, layer extraction, sweeps
  * and the final fidelity |<psi|C|0..0>|.  Gate emission into a circuit

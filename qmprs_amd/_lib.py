@@ -86,6 +86,7 @@ SIGNATURES = {
     "qmx_sweep_workspace": (_SZ, [_INT, _INT, _INT]),
     "qmx_sweep": (_INT, [_VP, _INT, _INT, _INT, _VP, _INT, _VP, _INT, _VP, _VP, _VP, _VP, _VP, _VP,
                          _VP, _VP, _VP, _SZ, _VP]),
+    "qmx_circuit_state": (_INT, [_VP, _INT, _INT, _INT, _VP, _VP, _VP, _INT, _VP, _VP, _VP, _VP, _SZ, _VP]),
     "qmx_encode_workspace": (_SZ, [ctypes.POINTER(EncodeConfig)]),
     "qmx_encode": (_INT, [_VP, ctypes.POINTER(EncodeConfig), _VP, _VP, _VP, _VP, _VP, _VP, _VP, _VP,
                           _SZ, _VP]),

@@ -583,14 +583,17 @@ int run_fwd(qmx_handle* h, int n, int batch, int lmax, const std::vector<FwdPass
     a.widths = widths;
     a.layers = layers;
     a.skip = skip;
-    const bool last = (pi + 1 == passes.size());
-    a.ref = last ? ref : nullptr;
+    // the overlap with ref is a separate coalesced pass (below): in the last
+    // circuit pass the gate bits are the lowest ones, where a fused read of
+    // the complex128 reference would be strided
+    a.ref = nullptr;
     a.dot_part = dot_part;
     a.dot_counter = dot_counter;
     a.dot_out = dot_out;
     a.dot_stride = dot_stride;
     launch_fwd_pass(a, nblk, batch, st);
   }
+  if (ref) launch_overlap<S>(psi, ref, n, batch, dot_part, dot_counter, dot_out, dot_stride, skip, st);
   QMX_LAUNCHED(h, "forward passes");
   return QMX_OK;
 }
@@ -1187,6 +1190,18 @@ int qmx_sweep(qmx_handle* h, int num_qubits, int batch, int num_layers, const in
                                  static_cast<const double2*>(target), static_cast<double2*>(gates), widths,
                                  static_cast<const double2*>(psi_ref), static_cast<double2*>(fidelity),
                                  overlap, goal, sweeps_done, static_cast<double2*>(circuit_out), c, st);
+}
+
+int qmx_circuit_state(qmx_handle* h, int num_qubits, int batch, int num_layers, const int* layers_used,
+                      const void* gates, const int* widths, int storage_dtype, const void* psi_ref,
+                      void* fidelity, void* state_out, void* workspace, size_t workspace_bytes,
+                      void* stream) {
+  if (!state_out) return h ? fail(h, QMX_ERR_INVALID, "state_out is null") : QMX_ERR_INVALID;
+  // zero sweeps: only the final C|0..0> and its overlap are computed; the
+  // target argument is never read
+  return qmx_sweep(h, num_qubits, batch, num_layers, layers_used, 0, state_out, storage_dtype,
+                   const_cast<void*>(gates), widths, psi_ref, fidelity, nullptr, nullptr, nullptr,
+                   state_out, workspace, workspace_bytes, stream);
 }
 
 namespace {

@@ -221,6 +221,9 @@ void launch_norm2(const S* x, int n, int batch, double* part, int* counter, doub
 template <typename S, typename D>
 void launch_scale_convert(const S* x, D* y, int n, int batch, const double* norm2, hipStream_t);
 void launch_fill_int(int* p, int v, int count, hipStream_t);
+template <typename S>
+void launch_overlap(const S* x, const double2* ref, int n, int batch, double* part, int* counter,
+                    double2* out, int out_stride, const int* skip, hipStream_t);
 template <typename S> void launch_basis0(S* psi, int n, int batch, const int* skip, hipStream_t);
 void launch_chain2_step(const Chain2Args&, int nblk, int batch, hipStream_t);
 template <typename S> void launch_window_pass(const WinPassArgs<S>&, int nblk, int batch, hipStream_t);

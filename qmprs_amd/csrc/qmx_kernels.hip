@@ -10,6 +10,8 @@
 // LDS.  The tile bit set P is the union U of the gate bits plus the lowest
 // bits outside U, so a tile is a few long contiguous runs in HBM (coalesced
 // loads) and every gate of the pass acts inside it.  Tiles partition the state.
+#include <algorithm>
+
 #include "qmx_device.h"
 #include "qmx_internal.h"
 
@@ -232,6 +234,51 @@ __global__ __launch_bounds__(256) void norm2_kernel(const S* __restrict__ x, int
     reduce_partials<1>(pb, gridDim.x, scratch, tot);
     if (threadIdx.x == 0) {
       out[b] = tot[0];
+      counter[b] = 0;
+    }
+  }
+}
+
+// out[b * out_stride] = <ref_b | x_b> = sum_i conj(ref[i]) x[i], one coalesced
+// pass (FP64 accumulation, fixed-order reduction of the block partials)
+template <typename S>
+__global__ __launch_bounds__(256) void overlap_kernel(const S* __restrict__ x, const double2* __restrict__ ref,
+                                                      int n, double* part, int* counter, double2* out,
+                                                      int out_stride, const int* skip) {
+  __shared__ double red[4 * 2];
+  __shared__ double scratch[256];
+  __shared__ double tot[2];
+  __shared__ int s_flag;
+  const int b = blockIdx.y;
+  if (skip && skip[b]) return;
+  const size_t len = (size_t)1 << n;
+  const S* xb = x + (size_t)b * len;
+  const double2* rb = ref + (size_t)b * len;
+  // block blockIdx.x takes a contiguous chunk; 4 independent loads in flight
+  // per lane and 4 accumulator pairs
+  const size_t chunk = len / gridDim.x;
+  const size_t beg = (size_t)blockIdx.x * chunk;
+  double v[2] = {0.0, 0.0};
+  double acc[4][2] = {{0.0, 0.0}, {0.0, 0.0}, {0.0, 0.0}, {0.0, 0.0}};
+  for (size_t i = beg + threadIdx.x; i < beg + chunk; i += 4 * (size_t)blockDim.x) {
+#pragma unroll
+    for (int u = 0; u < 4; ++u) {
+      const size_t k = i + (size_t)u * blockDim.x;
+      if (k < beg + chunk) {
+        const double2 r = rb[k], p = to_c(xb[k]);
+        acc[u][0] = fma(r.x, p.x, fma(r.y, p.y, acc[u][0]));
+        acc[u][1] = fma(r.x, p.y, fma(-r.y, p.x, acc[u][1]));
+      }
+    }
+  }
+  v[0] = (acc[0][0] + acc[1][0]) + (acc[2][0] + acc[3][0]);
+  v[1] = (acc[0][1] + acc[1][1]) + (acc[2][1] + acc[3][1]);
+  block_sum<2>(v, red);
+  double* pb = part + (size_t)b * gridDim.x * 2;
+  if (publish_partial<2>(red, pb, counter + b, gridDim.x, &s_flag)) {
+    reduce_partials<2>(pb, gridDim.x, scratch, tot);
+    if (threadIdx.x == 0) {
+      out[(size_t)b * out_stride] = make_double2(tot[0], tot[1]);
       counter[b] = 0;
     }
   }
@@ -752,6 +799,21 @@ void launch_norm2(const S* x, int n, int batch, double* part, int* counter, doub
                   int nblk, hipStream_t st) {
   hipLaunchKernelGGL(norm2_kernel<S>, dim3(nblk, batch), dim3(256), 0, st, x, n, part, counter, out);
 }
+template <typename S>
+void launch_overlap(const S* x, const double2* ref, int n, int batch, double* part, int* counter,
+                    double2* out, int out_stride, const int* skip, hipStream_t st) {
+  // whole chunks of >= 4096 amplitudes (len is a power of two), about 1024
+  // blocks over the batch
+  const size_t len = (size_t)1 << n;
+  size_t nblk = 1;
+  while (nblk * 2 * batch <= 1024 && (len / (nblk * 2)) >= 4096) nblk *= 2;
+  hipLaunchKernelGGL(overlap_kernel<S>, dim3((unsigned)nblk, batch), dim3(256), 0, st, x, ref, n, part, counter,
+                     out, out_stride, skip);
+}
+template void launch_overlap<float2>(const float2*, const double2*, int, int, double*, int*, double2*, int,
+                                     const int*, hipStream_t);
+template void launch_overlap<double2>(const double2*, const double2*, int, int, double*, int*, double2*, int,
+                                      const int*, hipStream_t);
 template void launch_norm2<float2>(const float2*, int, int, double*, int*, double*, int, hipStream_t);
 template void launch_norm2<double2>(const double2*, int, int, double*, int*, double*, int, hipStream_t);
 

@@ -152,6 +152,24 @@ def sweep(target: torch.Tensor, gates: torch.Tensor, widths: torch.Tensor, layer
     return SweepResult(fid, ov[:, :num_sweeps], done, circ)
 
 
+def circuit_state(gates: torch.Tensor, widths: torch.Tensor, layers_used: torch.Tensor,
+                  dtype=torch.complex128, psi_ref: torch.Tensor | None = None):
+    """C|0..0> of each state's gate list on the GPU (qmx_circuit_state);
+    with psi_ref also returns <psi_ref|C|0> per state."""
+    b, nl, n = widths.shape
+    lib, h = _lib.load(), _lib.handle()
+    dev = gates.device
+    dt = _DTYPES[dtype]
+    out = torch.empty((b, 1 << n), dtype=dtype, device=dev)
+    fid = torch.zeros((b, 1), dtype=torch.complex128, device=dev) if psi_ref is not None else None
+    ws = _lib.workspace(lib.qmx_sweep_workspace(n, b, dt), dev)
+    st = lib.qmx_circuit_state(h, n, b, nl, _lib.ptr(layers_used), _lib.ptr(gates), _lib.ptr(widths), dt,
+                               _lib.ptr(psi_ref), _lib.ptr(fid), _lib.ptr(out), _lib.ptr(ws), ws.numel(),
+                               _lib.stream_ptr())
+    _lib.check(st, h)
+    return (out, fid[:, 0]) if fid is not None else out
+
+
 @dataclass
 class EncodeResult:
     gates: torch.Tensor          # (B, L, N, 4, 4) complex128, layer = extraction order

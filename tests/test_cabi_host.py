@@ -176,3 +176,44 @@ def test_gate_circuit_rejects_bad_input():
         c.unitary(np.eye(2), 3)
     with pytest.raises(ValueError):
         GateCircuit(0)
+
+
+def test_wire_format_round_trip(tmp_path):
+    """npz gate-list format: save -> load keeps every gate, block and mapping,
+    and the emitted circuit reproduces the dense circuit state."""
+    from oracle import qmprs_oracle as O
+    from qmprs_amd.circuit import GateCircuit
+    from qmprs_amd.primitives.mps import Tensor
+    from qmprs_amd.synthesis.mps_encoding import wire
+    n = 7
+    psi = O.random_uniform_state(n, 3)
+    target, _ = O.chi_truncate(psi, n, 8)
+    raw = O.get_unitary_layers(target, n, 3)
+    layers = [[(s, e, [Tensor(g) for g in gs]) for s, e, gs in layer] for layer in raw]
+    path = tmp_path / "layers.npz"
+    wire.save(path, layers, n)
+    back, n2 = wire.load(path)
+    assert n2 == n and len(back) == len(layers)
+    for la, lb in zip(layers, back):
+        assert [(s, e) for s, e, _ in la] == [(s, e) for s, e, _ in lb]
+        for (_, _, ma), (_, _, mb) in zip(la, lb):
+            for a, b in zip(ma, mb):
+                np.testing.assert_array_equal(a.data, b.data)
+    arrays = wire.to_arrays(layers, n)
+    two = arrays["widths"] == 2
+    np.testing.assert_array_equal(arrays["qubits"][two, 1], n - 1 - arrays["sites"][two])
+    c = wire.emit(back, n, GateCircuit)
+    np.testing.assert_allclose(c.get_statevector(), O.circuit_state(n, O.layer_gate_list(raw)), atol=1e-12)
+    bad = dict(arrays)
+    bad["version"] = np.int32(99)
+    with pytest.raises(ValueError):
+        wire.from_arrays(bad)
+
+
+def test_qiskit_adapter_is_optional():
+    from qmprs_amd.synthesis.mps_encoding import wire
+    try:
+        import qiskit  # noqa: F401
+    except ImportError:
+        with pytest.raises(ImportError):
+            wire.to_qiskit([], 2)

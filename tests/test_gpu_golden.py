@@ -168,3 +168,19 @@ def test_c1_through_the_python_api(gpu):
     sv = circuit.get_statevector()
     assert np.linalg.norm(sv - g["state"]) < 1e-5
     assert abs(_fid(circuit, O.ket(g["statevector"])) - g["fidelity"][-1]) < 1e-6
+
+
+@pytest.mark.parametrize("path", FIXTURES[:2], ids=[os.path.basename(p)[:-4] for p in FIXTURES[:2]])
+def test_circuit_state_readback(gpu, path):
+    """qmx_circuit_state on the fixture's swept gates equals the fixture's
+    C|0..0> and fidelity (the get_statevector() check without a simulator)."""
+    from qmprs_amd import engine
+    g = _load(path)
+    used = int(g["layers_used"])
+    gates = torch.from_numpy(g["swept_gates"]).cuda().unsqueeze(0).contiguous()
+    widths = torch.from_numpy(g["widths"]).cuda().unsqueeze(0).contiguous()
+    lu = torch.tensor([used], dtype=torch.int32, device="cuda")
+    psi = torch.from_numpy(O.ket(g["statevector"])).cuda().unsqueeze(0).contiguous()
+    state, fid = engine.circuit_state(gates, widths, lu, psi_ref=psi)
+    np.testing.assert_allclose(state[0].cpu().numpy(), g["state"], atol=1e-12)
+    assert abs(abs(fid[0].item()) - g["fidelity"][-1]) < 1e-12
