@@ -66,7 +66,7 @@ const char* qmx_last_error(const qmx_handle* h);
 #define QMX_OPT_SPLIT_SWEEPS 2
 #define QMX_OPT_WINDOW_BLOCKS_PER_CU 3
 /* QMX_OPT_FWD_BLOCKS_PER_CU: grid of the forward-circuit rebuild passes, in
- * blocks per compute unit over the whole batch (1..16, default 2). */
+ * blocks per compute unit over the whole batch (1..16, default 4). */
 #define QMX_OPT_FWD_BLOCKS_PER_CU 4
 int qmx_set_option(qmx_handle* h, int option, int value);
 

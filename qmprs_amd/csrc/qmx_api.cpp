@@ -33,7 +33,7 @@ struct qmx_handle {
   std::vector<hipEvent_t> pool;
   bool split_sweeps = true;             // QMX_OPT_SPLIT_SWEEPS
   int win_blocks_per_cu = 16;           // QMX_OPT_WINDOW_BLOCKS_PER_CU
-  int fwd_blocks_per_cu = 2;            // QMX_OPT_FWD_BLOCKS_PER_CU
+  int fwd_blocks_per_cu = 4;            // QMX_OPT_FWD_BLOCKS_PER_CU
   hipStream_t aux[2] = {nullptr, nullptr};
   hipEvent_t pass_ev[2] = {nullptr, nullptr};
   hipEvent_t join_ev[2] = {nullptr, nullptr};

@@ -281,15 +281,15 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(WPE))) void
     xol[side][x] = (int)(side ? xoffs(a.gu.ux, x) : xoffs(a.gp.ux, x));
     if (threadIdx.x < 72) rdep[x] = group_roffl(a.gu, 4 * x);
   }
-  if (threadIdx.x < np + nc) {
-    const int j = threadIdx.x;
+  // one gate entry per thread ((np + nc) * 16 <= 256)
+  if (threadIdx.x < (np + nc) * 16) {
+    const int j = threadIdx.x >> 4, e = threadIdx.x & 15;
     const int s = j < np ? a.p0 + j : a.s0 + (j - np);
     int layer, site;
     step_gate(n, s, layer, site);
-    size_t gi = ((size_t)b * a.lmax + layer) * n + site;
-    int w = wsane(n, site, a.widths[gi]);
-    for (int e = 0; e < 16; ++e) gmat[j][e] = T::mk(a.gates[gi * 16 + e]);
-    gcode[j] = gate_code(j < np ? a.gp.mask : a.gu.mask, n, site, w);
+    const size_t gi = ((size_t)b * a.lmax + layer) * n + site;
+    gmat[j][e] = T::mk(a.gates[gi * 16 + e]);
+    if (e == 0) gcode[j] = gate_code(j < np ? a.gp.mask : a.gu.mask, n, site, wsane(n, site, a.widths[gi]));
   }
   __syncthreads();
   const int t = a.t;
@@ -409,20 +409,19 @@ __global__ __launch_bounds__(256) void fwd_pass_kernel(FwdArgs<S> a) {
   const int n = a.n;
   if (a.skip && a.skip[b]) return;
   const int lb = a.layers[b];
-  if (threadIdx.x < kMaxStages * kMaxWindow) {
-    const int st = threadIdx.x / kMaxWindow, j = threadIdx.x % kMaxWindow;
+  // one gate entry per thread (kMaxStages * kMaxWindow * 16 <= 2 * 256)
+  for (int t = threadIdx.x; t < kMaxStages * kMaxWindow * 16; t += blockDim.x) {
+    const int e = t & 15, gslot = t >> 4;
+    const int st = gslot / kMaxWindow, j = gslot % kMaxWindow;
     if (st < a.nstage) {
       const int f = a.f0[st] + j;
       if (f < a.f1[st] && f < lb * n) {
         const int layer = lb - 1 - f / n, site = f % n;
-        size_t gi = ((size_t)b * a.lmax + layer) * n + site;
-        int w = wsane(n, site, a.widths[gi]);
+        const size_t gi = ((size_t)b * a.lmax + layer) * n + site;
         // grp_apply applies the adjoint of what it is given: stage G^dagger
-        for (int e = 0; e < 16; ++e) {
-          const double2 gv = a.gates[gi * 16 + (e & 3) * 4 + (e >> 2)];
-          gmat[st][j][e] = T::mk(make_double2(gv.x, -gv.y));
-        }
-        gcode[st][j] = gate_code(a.gs[st].mask, n, site, w);
+        const double2 gv = a.gates[gi * 16 + (e & 3) * 4 + (e >> 2)];
+        gmat[st][j][e] = T::mk(make_double2(gv.x, -gv.y));
+        if (e == 0) gcode[st][j] = gate_code(a.gs[st].mask, n, site, wsane(n, site, a.widths[gi]));
       }
     }
   }
