@@ -174,9 +174,11 @@ std::vector<std::vector<int>> plan_passes(int n, const std::vector<int>& pos,
   return passes;
 }
 
-// The Jacobi kernel handles matrices up to 512 x 512, i.e. chi <= 256 whenever
-// the chain truncates (2^(N-1) > chi).
-bool chi_supported(int n, int chi) { return chi <= 256 || (1l << (n - 1)) <= chi; }
+// The Jacobi kernels handle Gram matrices up to 1024 x 1024, i.e. chi <= 512
+// whenever the chain truncates (2^(N-1) > chi).
+bool chi_supported(int n, int chi) { return chi <= 512 || (1l << (n - 1)) <= chi; }
+
+constexpr int kJacobiBlockMax = 512;  // one-block global Jacobi up to 512 x 512
 
 // ------------------------------------------------------------ layouts -------
 struct ChiLayout {
@@ -196,6 +198,7 @@ struct ChiLayout {
   int* error;
   double* jlog;      // LDS Jacobi rotation log (nmax <= 128)
   int* jstate;
+  char* jgrid;       // grid Jacobi state (nmax > 512)
   int i_first, nmax, nchunks_max, nblk_norm;
 };
 
@@ -226,6 +229,7 @@ ChiLayout chi_layout(Carver& c, int n, int batch, int chi, bool need_psi) {
       L.jlog = c.take<double>(jacobi_lds_log_doubles(L.nmax) * batch);
       L.jstate = c.take<int>(2 * static_cast<size_t>(batch));
     }
+    if (L.nmax > kJacobiBlockMax) L.jgrid = c.take<char>(jacobi_grid_bytes(L.nmax, batch));
   }
   L.nn = c.take<int>(static_cast<size_t>(batch) * (n + 1));
   L.kk = c.take<int>(static_cast<size_t>(batch) * (n + 1));
@@ -245,7 +249,7 @@ struct LayerLayout {
   int* error;
 };
 constexpr int kChain2MaxBlocks = 256;
-constexpr int kJacobiSweeps = 30;   // sweep launches of the LDS Jacobi per bond
+constexpr int kJacobiSweeps = 30;   // sweep launches of the LDS / grid Jacobi per bond
 constexpr int kWinMaxBlocks = 256;
 
 // window-pass blocks per state: about win_blocks_per_cu resident blocks per CU
@@ -433,10 +437,16 @@ int run_chi(qmx_handle* h, int n, int batch, int chi, double cutoff, const void*
     int nchunks = std::max(1, std::min(L.nchunks_max, rows / 256));
     launch_gram(y, len, L.ncur, i, nmax, L.part, nchunks, batch, st);
     launch_gram_reduce(L.part, nchunks, L.ncur, nmax, L.a, nm2, batch, st);
-    if (L.jlog && jacobi_lds_setup(nmax) == hipSuccess)
+    if (L.jgrid) {
+      std::vector<int> conv(batch);
+      if (launch_jacobi_grid(L.a, L.v, nm2, L.ncur, nmax, batch, L.jgrid, conv.data(), kJacobiSweeps,
+                             L.error, st) != 0)
+        return fail(h, QMX_ERR_HIP, "grid Jacobi");
+    } else if (L.jlog && jacobi_lds_setup(nmax) == hipSuccess) {
       launch_jacobi_lds(L.a, L.v, nm2, L.ncur, nmax, batch, L.jlog, L.jstate, kJacobiSweeps, L.error, st);
-    else
+    } else {
       launch_jacobi(L.a, L.v, nm2, L.ncur, nmax, batch, L.error, st);
+    }
     double2* wi = L.w + static_cast<size_t>(i) * batch * nm2;
     launch_select(L.a, L.v, wi, nm2, L.ncur, nmax, chi, cutoff, L.kk + i, n + 1,
                   spectra ? spectra + static_cast<size_t>(i) * 2 * chi : nullptr,
@@ -478,7 +488,9 @@ int run_generate_layer(qmx_handle* h, int n, int batch, int lmax, int layer, dou
     a.counter = L.counter;
     a.error = L.error;
     long rows = 1l << i;
-    int nblk = static_cast<int>(std::max(1l, std::min<long>(kChain2MaxBlocks, rows / 1024)));
+    // about 1024 blocks over the batch, each thread >= 4 rows
+    const long per = std::max(1l, (1024l + batch - 1) / batch);
+    int nblk = static_cast<int>(std::max(1l, std::min<long>({(long)kChain2MaxBlocks, rows / 1024, per})));
     launch_chain2_step(a, nblk, batch, st);
   }
   RanArgs r{};
@@ -1071,7 +1083,7 @@ int qmx_chi_truncate(qmx_handle* h, int num_qubits, int batch, int bond_dimensio
   if (!statevectors || !target_out || !workspace) return fail(h, QMX_ERR_INVALID, "null pointer");
   if (sv_dtype != QMX_C64 && sv_dtype != QMX_C128) return fail(h, QMX_ERR_INVALID, "sv_dtype");
   if (!chi_supported(num_qubits, bond_dimension))
-    return fail(h, QMX_ERR_INVALID, "bond_dimension above 256 with truncation is not supported");
+    return fail(h, QMX_ERR_INVALID, "bond_dimension above 512 with truncation is not supported");
   if (workspace_bytes < qmx_chi_truncate_workspace(num_qubits, batch, bond_dimension))
     return fail(h, QMX_ERR_WORKSPACE, "workspace too small");
   QMX_HIP(h, hipSetDevice(h->device));
@@ -1251,7 +1263,7 @@ int encode_check(qmx_handle* h, const qmx_encode_config* cfg, size_t workspace_b
   if (g.bond_dimension < 1) return fail(h, QMX_ERR_INVALID, "`bond_dimension` must be an integer greater than 0.");
   if (g.num_layers < 1) return fail(h, QMX_ERR_INVALID, "The number of layers must be a positive integer.");
   if (!chi_supported(g.num_qubits, g.bond_dimension))
-    return fail(h, QMX_ERR_INVALID, "bond_dimension above 256 with truncation is not supported");
+    return fail(h, QMX_ERR_INVALID, "bond_dimension above 512 with truncation is not supported");
   if (g.num_sweeps < 0) return fail(h, QMX_ERR_INVALID, "num_sweeps must be >= 0");
   if ((g.sv_dtype != QMX_C64 && g.sv_dtype != QMX_C128) ||
       (g.sweep_dtype != QMX_C64 && g.sweep_dtype != QMX_C128))

@@ -575,9 +575,18 @@ __global__ __launch_bounds__(256) void chain2_step_kernel(Chain2Args a) {
   }
   const int ncols = 2 * rcur;                   // columns of Y_i
   const size_t rows = (size_t)1 << i;
-  double e[32];
+  const int nin = 2 * rin;                      // columns of Y_{i+1}
+  // W_{i+1} columns c < rcur over rows j < nin (zero elsewhere)
+  double2 wp[4][2];
 #pragma unroll
-  for (int j = 0; j < 32; ++j) e[j] = 0.0;
+  for (int j = 0; j < 4; ++j)
+#pragma unroll
+    for (int c = 0; c < 2; ++c)
+      wp[j][c] = (wproj && j < nin && c < rcur) ? wproj[j * 4 + c] : cz();
+  // upper triangle of the Gram matrix (p <= q), 10 complex entries
+  double e[20];
+#pragma unroll
+  for (int j = 0; j < 20; ++j) e[j] = 0.0;
   for (size_t row = (size_t)blockIdx.x * blockDim.x + threadIdx.x; row < rows;
        row += (size_t)gridDim.x * blockDim.x) {
     double2 y[4];
@@ -586,44 +595,62 @@ __global__ __launch_bounds__(256) void chain2_step_kernel(Chain2Args a) {
       y[1] = xin[2 * row + 1];
       y[2] = cz(); y[3] = cz();
     } else {
-      // rows 2row, 2row+1 of X_{i+1}; each from 2 rows of X_{i+1}'s parent
-      const int nin = 2 * rin;   // columns of Y_{i+1}
+      // rows 2row, 2row+1 of X_{i+1} = Y_{i+1} W_{i+1}
+      double2 s[2][2];
+#pragma unroll
       for (int h = 0; h < 2; ++h) {
-        size_t xr = 2 * row + h;            // row of X_{i+1}
+        const size_t xr = 2 * row + h;
         double2 yin[4];
-        for (int c = 0; c < nin; ++c) yin[c] = xin[xr * nin + c];  // Y_{i+1} row xr
-        for (int c = 0; c < rcur; ++c) {
-          double2 s = cz();
-          for (int j = 0; j < nin; ++j) s = cfma(yin[j], wproj[j * 4 + c], s);
-          y[h * rcur + c] = s;
-          xout[xr * rcur + c] = s;
+        yin[0] = xin[xr * nin];
+        yin[1] = xin[xr * nin + 1];
+        if (nin == 4) {
+          yin[2] = xin[xr * nin + 2];
+          yin[3] = xin[xr * nin + 3];
+        } else {
+          yin[2] = cz(); yin[3] = cz();
         }
+#pragma unroll
+        for (int c = 0; c < 2; ++c) {
+          double2 acc = cz();
+#pragma unroll
+          for (int j = 0; j < 4; ++j) acc = cfma(yin[j], wp[j][c], acc);
+          s[h][c] = acc;
+        }
+        xout[xr * rcur] = s[h][0];
+        if (rcur == 2) xout[xr * rcur + 1] = s[h][1];
       }
-      for (int c = ncols; c < 4; ++c) y[c] = cz();
+      // Y_i row: columns (p, c) = p * rcur + c
+      const bool two = (rcur == 2);
+      y[0] = s[0][0];
+      y[1] = two ? s[0][1] : s[1][0];
+      y[2] = two ? s[1][0] : cz();
+      y[3] = two ? s[1][1] : cz();
     }
+    int t = 0;
 #pragma unroll
     for (int p = 0; p < 4; ++p)
 #pragma unroll
-      for (int q = 0; q < 4; ++q) {
-        double2 c = cmulc(y[p], y[q]);
-        e[2 * (p * 4 + q)] += c.x;
-        e[2 * (p * 4 + q) + 1] += c.y;
+      for (int q = p; q < 4; ++q) {
+        const double2 c = cmulc(y[p], y[q]);
+        e[2 * t] += c.x;
+        e[2 * t + 1] += c.y;
+        ++t;
       }
   }
-  block_sum<32>(e, red);
-  double* pb = a.part + (size_t)b * gridDim.x * 32;
-  if (publish_partial<32>(red, pb, a.counter + b, gridDim.x, &s_flag)) {
-    reduce_partials<32>(pb, gridDim.x, scratch, tot);
+  block_sum<20>(e, red);
+  double* pb = a.part + (size_t)b * gridDim.x * 20;
+  if (publish_partial<20>(red, pb, a.counter + b, gridDim.x, &s_flag)) {
+    reduce_partials<20>(pb, gridDim.x, scratch, tot);
     if (threadIdx.x == 0) {
       double2 rho[16], v[16], w[16];
       double lam[4];
-      for (int j = 0; j < 16; ++j) rho[j] = make_double2(tot[2 * j], tot[2 * j + 1]);
-      for (int p = 0; p < 4; ++p)   // Hermitian part
+      int t = 0;
+      for (int p = 0; p < 4; ++p)   // Hermitian matrix from its upper triangle
         for (int q = p; q < 4; ++q) {
-          double2 x = rho[p * 4 + q], y = rho[q * 4 + p];
-          double2 h = make_double2(0.5 * (x.x + y.x), 0.5 * (x.y - y.y));
+          const double2 h = make_double2(tot[2 * t], p == q ? 0.0 : tot[2 * t + 1]);
           rho[p * 4 + q] = h;
           rho[q * 4 + p] = cconj(h);
+          ++t;
         }
       herm_eig4(rho, v, lam);   // rho is zero-padded to 4 x 4
       int k = keep_count(lam, 4, 2, a.cutoff);

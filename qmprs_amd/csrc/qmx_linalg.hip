@@ -9,6 +9,8 @@
 // and the state is contracted back with Z_{i+1} = Z_i W_i^H (gemm, conj_b).
 // Shapes that depend on the cutoff live on the device (k_i, n_i per state);
 // launches are sized by host upper bounds and idle blocks exit early.
+#include <algorithm>
+
 #include "qmx_device.h"
 #include "qmx_internal.h"
 
@@ -76,7 +78,7 @@ __global__ void gram_reduce_kernel(const double2* part, int nchunks, const int* 
 // method; all n/2 rotations of a round are disjoint, so the update
 // A <- J^H A J is done blockwise: each 2x2 block (P, Q) of A becomes
 // J_P^H A_PQ J_Q, independently of every other block.  V accumulates J.
-constexpr int kJacobiMaxPairs = 256;
+constexpr int kJacobiMaxPairs = 512;
 
 __global__ __launch_bounds__(512) void jacobi_kernel(double2* a_all, double2* v_all, size_t stride,
                                                      const int* ncols_dev, int ld, int* error) {
@@ -251,16 +253,23 @@ __global__ __launch_bounds__(1024) void jacobi_lds_sweep_kernel(double2* a_all, 
     __syncthreads();
     // blocks (P, Q), P <= Q: B' = J_P^H B J_Q with B = A[{p0, p1}][{q0, q1}];
     // (Q, P) is its conjugate transpose and is never stored separately
-    for (int e = tid; e < np * np; e += nt) {
-      const int P = e / np, Q = e % np;
-      if (P > Q) continue;
+    const int nblocks = np * (np + 1) / 2;
+    for (int e = tid; e < nblocks; e += nt) {
+      // e -> (P, Q), P <= Q, row-major over the upper triangle of pair blocks
+      int P = (int)((2.0 * np + 1.0 - sqrt((2.0 * np + 1.0) * (2.0 * np + 1.0) - 8.0 * e)) * 0.5);
+      while (P > 0 && tri_index(P, P, np) > e) --P;
+      while (P + 1 < np && tri_index(P + 1, P + 1, np) <= e) ++P;
+      const int Q = P + (e - tri_index(P, P, np));
       const int p0 = pp[P], p1 = qq[P], q0 = pp[Q], q1 = qq[Q];
+      // one LDS access per entry: (i, j) with i > j is the conjugate of (j, i)
       auto get = [&](int i, int j) {
-        return i <= j ? tri[tri_index(i, j, n)] : cconj(tri[tri_index(j, i, n)]);
+        const bool lo = i > j;
+        const double2 v = tri[tri_index(lo ? j : i, lo ? i : j, n)];
+        return make_double2(v.x, lo ? -v.y : v.y);
       };
       auto put = [&](int i, int j, double2 v) {
-        if (i <= j) tri[tri_index(i, j, n)] = v;
-        else tri[tri_index(j, i, n)] = cconj(v);
+        const bool lo = i > j;
+        tri[tri_index(lo ? j : i, lo ? i : j, n)] = make_double2(v.x, lo ? -v.y : v.y);
       };
       const double2 J00 = make_double2(jc[Q], 0.0), J01 = make_double2(js[Q], 0.0);
       const double2 J10 = cscale(je[Q], -js[Q]), J11 = cscale(je[Q], jc[Q]);
@@ -356,6 +365,132 @@ __global__ void jacobi_lds_init_kernel(double2* v_all, size_t stride, const int*
 __global__ void jacobi_lds_check_kernel(const int* state, int batch, int* error) {
   const int b = blockIdx.x * blockDim.x + threadIdx.x;
   if (b < batch && !state[b] && error) atomicOr(error, kErrEigNoConv);
+}
+
+// ------------------------------------------------------ grid Jacobi -------
+// For matrices too large for one workgroup (n > 512, chi = 512): the same
+// parallel ordering and rotations as jacobi_kernel, one round per pair of
+// launches -- rotation parameters (one thread per pair), then the 2x2 block
+// updates of A and the column-pair updates of V spread over the grid.  A
+// sweep that rotates nothing marks its state converged; later launches return
+// at once for it.
+
+QMX_DEV void circle_pair(int n, int r, int m, int& p, int& q) {
+  p = (m == 0) ? n - 1 : (r + m) % (n - 1);
+  q = (r + n - 1 - m) % (n - 1);
+  if (m == 0) q = r % (n - 1);
+  if (p > q) { const int t = p; p = q; q = t; }
+}
+
+__global__ void jgrid_init_kernel(const double2* a_all, double2* v_all, size_t stride, const int* ncols_dev,
+                                  int ld, double* scale, int* conv, int* rot) {
+  const int b = blockIdx.y;
+  const int n = ncols_dev[b];
+  const double2* a = a_all + (size_t)b * stride;
+  double2* v = v_all + (size_t)b * stride;
+  for (size_t e = blockIdx.x * (size_t)blockDim.x + threadIdx.x; e < (size_t)n * n;
+       e += (size_t)gridDim.x * blockDim.x) {
+    const int i = (int)(e / n), j = (int)(e % n);
+    v[(size_t)i * ld + j] = make_double2(i == j ? 1.0 : 0.0, 0.0);
+  }
+  if (blockIdx.x == 0 && threadIdx.x == 0) {
+    double tr = 0.0;
+    for (int i = 0; i < n; ++i) tr += fabs(a[(size_t)i * ld + i].x);
+    scale[b] = tr;
+    conv[b] = n < 2 ? 1 : 0;
+    rot[b] = 0;
+  }
+}
+
+__global__ void jgrid_params_kernel(const double2* a_all, size_t stride, const int* ncols_dev, int ld, int r,
+                                    double4* prm, int max_pairs, const double* scale, const int* conv,
+                                    int* rot) {
+  const int b = blockIdx.y;
+  if (conv[b]) return;
+  const int n = ncols_dev[b], np = n / 2;
+  const int m = blockIdx.x * blockDim.x + threadIdx.x;
+  if (m >= np) return;
+  const double2* a = a_all + (size_t)b * stride;
+  int p, q;
+  circle_pair(n, r, m, p, q);
+  const double al = a[(size_t)p * ld + p].x, be = a[(size_t)q * ld + q].x;
+  const double2 g = a[(size_t)p * ld + q];
+  const double ag = cabs_(g);
+  double c = 1.0, sn = 0.0;
+  double2 em = make_double2(1.0, 0.0);
+  if (ag > 1e-17 * scale[b] + 1e-300 && ag > 1e-15 * sqrt(fabs(al * be))) {
+    const double tau = (be - al) / (2.0 * ag);
+    const double t = (tau >= 0 ? 1.0 : -1.0) / (fabs(tau) + sqrt(1.0 + tau * tau));
+    c = 1.0 / sqrt(1.0 + t * t);
+    sn = t * c;
+    em = make_double2(g.x / ag, -g.y / ag);   // e^{-i phi}
+    rot[b] = 1;
+  }
+  prm[(size_t)b * max_pairs + m] = make_double4(c, sn, em.x, em.y);
+}
+
+__global__ void jgrid_update_kernel(double2* a_all, double2* v_all, size_t stride, const int* ncols_dev, int ld,
+                                    int r, const double4* prm, int max_pairs, const int* conv) {
+  const int b = blockIdx.y;
+  if (conv[b]) return;
+  const int n = ncols_dev[b], np = n / 2;
+  const size_t e = blockIdx.x * (size_t)blockDim.x + threadIdx.x;
+  const size_t na = (size_t)np * np, nv = (size_t)n * np;
+  if (e >= na + nv) return;
+  double2* a = a_all + (size_t)b * stride;
+  double2* v = v_all + (size_t)b * stride;
+  const double4* pr = prm + (size_t)b * max_pairs;
+  auto jmat = [&](int k, double2& J00, double2& J01, double2& J10, double2& J11) {
+    const double4 R = pr[k];
+    const double2 em = make_double2(R.z, R.w);
+    J00 = make_double2(R.x, 0.0);
+    J01 = make_double2(R.y, 0.0);
+    J10 = cscale(em, -R.y);
+    J11 = cscale(em, R.x);
+  };
+  if (e < na) {
+    const int P = (int)(e / np), Q = (int)(e % np);
+    int p0, p1, q0, q1;
+    circle_pair(n, r, P, p0, p1);
+    circle_pair(n, r, Q, q0, q1);
+    double2 J00, J01, J10, J11, K00, K01, K10, K11;
+    jmat(Q, J00, J01, J10, J11);
+    jmat(P, K00, K01, K10, K11);
+    const double2 a00 = a[(size_t)p0 * ld + q0], a01 = a[(size_t)p0 * ld + q1];
+    const double2 a10 = a[(size_t)p1 * ld + q0], a11 = a[(size_t)p1 * ld + q1];
+    const double2 b00 = cadd(cmul(a00, J00), cmul(a01, J10));
+    const double2 b01 = cadd(cmul(a00, J01), cmul(a01, J11));
+    const double2 b10 = cadd(cmul(a10, J00), cmul(a11, J10));
+    const double2 b11 = cadd(cmul(a10, J01), cmul(a11, J11));
+    double2 c00 = cadd(cmulc(K00, b00), cmulc(K10, b10));
+    double2 c01 = cadd(cmulc(K00, b01), cmulc(K10, b11));
+    double2 c10 = cadd(cmulc(K01, b00), cmulc(K11, b10));
+    double2 c11 = cadd(cmulc(K01, b01), cmulc(K11, b11));
+    if (P == Q) {
+      c00.y = 0.0; c11.y = 0.0;
+      c01 = cz(); c10 = cz();
+    }
+    a[(size_t)p0 * ld + q0] = c00; a[(size_t)p0 * ld + q1] = c01;
+    a[(size_t)p1 * ld + q0] = c10; a[(size_t)p1 * ld + q1] = c11;
+  } else {
+    const size_t f = e - na;
+    const int i = (int)(f / np), Q = (int)(f % np);
+    int q0, q1;
+    circle_pair(n, r, Q, q0, q1);
+    double2 J00, J01, J10, J11;
+    jmat(Q, J00, J01, J10, J11);
+    const double2 x = v[(size_t)i * ld + q0], y = v[(size_t)i * ld + q1];
+    v[(size_t)i * ld + q0] = cadd(cmul(x, J00), cmul(y, J10));
+    v[(size_t)i * ld + q1] = cadd(cmul(x, J01), cmul(y, J11));
+  }
+}
+
+// end of a sweep: no rotation -> converged; clear the rotation flags
+__global__ void jgrid_sweep_end_kernel(int* conv, int* rot, int batch) {
+  const int b = blockIdx.x * blockDim.x + threadIdx.x;
+  if (b >= batch) return;
+  if (!conv[b] && !rot[b]) conv[b] = 1;
+  rot[b] = 0;
 }
 
 // ---------------------------------------------------------------- select ----
@@ -502,6 +637,45 @@ void launch_jacobi(double2* a, double2* v, size_t stride, const int* ncols_dev, 
                    int batch, int* error, hipStream_t st) {
   hipLaunchKernelGGL(jacobi_kernel, dim3(batch), dim3(512), 0, st, a, v, stride, ncols_dev,
                      ncols_max, error);
+}
+
+size_t jacobi_grid_bytes(int ncols_max, int batch) {
+  return (size_t)batch * (ncols_max / 2) * sizeof(double4) + (size_t)batch * (sizeof(double) + 2 * sizeof(int));
+}
+
+int launch_jacobi_grid(double2* a, double2* v, size_t stride, const int* ncols_dev, int ncols_max, int batch,
+                       void* ws, int* conv_host_buf, int max_sweeps, int* error, hipStream_t st) {
+  const int max_pairs = ncols_max / 2;
+  char* p = static_cast<char*>(ws);
+  double4* prm = reinterpret_cast<double4*>(p);
+  p += (size_t)batch * max_pairs * sizeof(double4);
+  double* scale = reinterpret_cast<double*>(p);
+  p += (size_t)batch * sizeof(double);
+  int* conv = reinterpret_cast<int*>(p);
+  int* rot = conv + batch;
+  hipLaunchKernelGGL(jgrid_init_kernel, dim3(std::max(1, ncols_max * ncols_max / 256 / 64), batch), dim3(256), 0,
+                     st, a, v, stride, ncols_dev, ncols_max, scale, conv, rot);
+  const size_t work = (size_t)max_pairs * max_pairs + (size_t)ncols_max * max_pairs;
+  const dim3 ugrid((unsigned)((work + 255) / 256), batch);
+  const dim3 pgrid((unsigned)((max_pairs + 255) / 256), batch);
+  for (int s = 0; s < max_sweeps; ++s) {
+    for (int r = 0; r < ncols_max - 1; ++r) {
+      hipLaunchKernelGGL(jgrid_params_kernel, pgrid, dim3(256), 0, st, a, stride, ncols_dev, ncols_max, r, prm,
+                         max_pairs, scale, conv, rot);
+      hipLaunchKernelGGL(jgrid_update_kernel, ugrid, dim3(256), 0, st, a, v, stride, ncols_dev, ncols_max, r,
+                         prm, max_pairs, conv);
+    }
+    hipLaunchKernelGGL(jgrid_sweep_end_kernel, dim3((batch + 255) / 256), dim3(256), 0, st, conv, rot, batch);
+    // stop once every state has converged (one host read per sweep)
+    if (hipMemcpyAsync(conv_host_buf, conv, sizeof(int) * batch, hipMemcpyDeviceToHost, st) != hipSuccess ||
+        hipStreamSynchronize(st) != hipSuccess)
+      return -1;
+    bool all = true;
+    for (int b = 0; b < batch; ++b) all = all && conv_host_buf[b];
+    if (all) return 0;
+  }
+  hipLaunchKernelGGL(jacobi_lds_check_kernel, dim3((batch + 255) / 256), dim3(256), 0, st, conv, batch, error);
+  return 0;
 }
 
 size_t jacobi_lds_log_doubles(int ncols_max) {
