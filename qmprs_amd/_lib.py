@@ -195,6 +195,7 @@ def profile_read() -> dict:
 QMX_OPT_SPLIT_SWEEPS = 2
 QMX_OPT_WINDOW_BLOCKS_PER_CU = 3
 QMX_OPT_FWD_BLOCKS_PER_CU = 4
+QMX_OPT_STAGED_WINDOWS = 5
 
 
 def set_option(option: int, value: int) -> None:

@@ -34,6 +34,7 @@ struct qmx_handle {
   bool split_sweeps = true;             // QMX_OPT_SPLIT_SWEEPS
   int win_blocks_per_cu = 16;           // QMX_OPT_WINDOW_BLOCKS_PER_CU
   int fwd_blocks_per_cu = 4;            // QMX_OPT_FWD_BLOCKS_PER_CU
+  bool staged_windows = true;           // QMX_OPT_STAGED_WINDOWS
   hipStream_t aux[2] = {nullptr, nullptr};
   hipEvent_t pass_ev[2] = {nullptr, nullptr};
   hipEvent_t join_ev[2] = {nullptr, nullptr};
@@ -739,6 +740,12 @@ void job_window_pass(qmx_handle* h, SweepJob<S>& J, const std::vector<Window>& w
   pa.layers = J.layers_used;
   pa.skip = J.skip;
   pa.part = J.L.part;
+  // groups whose lanes stride the state by >= 4 amplitudes load poorly:
+  // stage those windows through LDS when the tile keeps >= 3 low bits
+  const int low_run = __builtin_ctzll(~pbits);
+  const int stride_bits = std::min(__builtin_ctzll(pa.gu.restg | (1ull << 63)),
+                                   __builtin_ctzll(pa.gp.restg | (1ull << 63)));
+  pa.staged = (h->staged_windows && stride_bits >= 2 && low_run >= 3 && pa.t - low_run <= 8) ? 1 : 0;
   PhaseTimer pt(h, J.st, QMX_PHASE_WINDOW_PASS);
   launch_window_pass(pa, wblk, J.batch, J.st);
   pt.stop(1);
@@ -1029,6 +1036,11 @@ int qmx_set_option(qmx_handle* h, int option, int value) {
   if (option == QMX_OPT_WINDOW_BLOCKS_PER_CU) {
     if (value < 1 || value > 16) return fail(h, QMX_ERR_INVALID, "window blocks per CU: 1..16");
     h->win_blocks_per_cu = value;
+    return QMX_OK;
+  }
+  if (option == QMX_OPT_STAGED_WINDOWS) {
+    if (value != 0 && value != 1) return fail(h, QMX_ERR_INVALID, "staged windows: 0 or 1");
+    h->staged_windows = value != 0;
     return QMX_OK;
   }
   if (option == QMX_OPT_FWD_BLOCKS_PER_CU) {

@@ -95,6 +95,7 @@ struct WinPassArgs {
   const int* layers;
   const int* skip;
   double* part;        // [B][nblk][512]
+  int staged;          // copy tiles through LDS in tile order (see window_pass_kernel)
 };
 
 // Forward circuit pass (qmx_window.hip): up to kMaxStages stages of gates,

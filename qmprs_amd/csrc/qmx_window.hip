@@ -254,7 +254,13 @@ QMX_DEV int gate_code(uint64_t mask, int n, int site, int w) {
 
 // complex64 fits 4 waves per SIMD without spills; complex128 runs at 2 (a
 // 4-wave budget would spill its FP64 groups)
-template <typename S, int WPE>
+// STAGED: the tiles of T' and phi are copied between HBM and LDS in tile
+// order by all threads (coalesced runs of 2^c0 amplitudes, c0 = number of
+// lowest state bits inside the tile) and the register groups are read from
+// and written to LDS.  The host selects it for windows whose groups stride
+// the state by 4 or more amplitudes per lane (gate bits at the lowest state
+// bits), where direct group loads would scatter every wave-wide access.
+template <typename S, int WPE, bool STAGED>
 __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(WPE))) void window_pass_kernel(
     WinPassArgs<S> a) {
   using C = typename StoreC<S>::C;
@@ -268,6 +274,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(WPE))) void
   __shared__ uint32_t xog[2][16];
   __shared__ int xol[2][16];
   __shared__ int rdep[8];
+  __shared__ uint32_t hoff[STAGED ? 256 : 1];
   const int b = blockIdx.y;
   const int n = a.n;
   if (a.skip && a.skip[b]) return;
@@ -290,6 +297,13 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(WPE))) void
     const size_t gi = ((size_t)b * a.lmax + layer) * n + site;
     gmat[j][e] = T::mk(a.gates[gi * 16 + e]);
     if (e == 0) gcode[j] = gate_code(j < np ? a.gp.mask : a.gu.mask, n, site, wsane(n, site, a.widths[gi]));
+  }
+  // staged copies: tile-local index l -> state offset (l & lomask) + hoff[l >> c0]
+  const uint64_t pbits = ((1ull << n) - 1ull) & ~a.qmask;
+  const int c0 = STAGED ? __builtin_ctzll(~pbits) : 0;
+  if (STAGED) {
+    for (int h = threadIdx.x; h < (1 << (a.t - c0)); h += blockDim.x)
+      hoff[h] = (uint32_t)wdeposit((uint64_t)h << c0, pbits);
   }
   __syncthreads();
   const int t = a.t;
@@ -334,12 +348,23 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(WPE))) void
   // a per-lane part and the per-step part rdep[k]
   const int rlane = xl_i + group_roffl(a.gu, wave * kper + (lane >> 4));
   d4v acc_r = {0, 0, 0, 0}, acc_i = {0, 0, 0, 0};
+  const uint32_t lomask = (1u << c0) - 1u;
   for (long tile = blockIdx.x; tile < ntiles; tile += gridDim.x) {
-    const uint32_t gb = (uint32_t)wdeposit((uint64_t)tile, a.qmask) + roffg;
+    const uint32_t base = (uint32_t)wdeposit((uint64_t)tile, a.qmask);
+    const uint32_t gb = base + roffg;
+    if (STAGED) {
+      for (int l = threadIdx.x; l < (1 << t); l += blockDim.x) {
+        const uint32_t off = base + ((uint32_t)l & lomask) + hoff[l >> c0];
+        sT[swz(l)] = tsrc[off];
+        sP[swz(l)] = phi[off];
+      }
+      __syncthreads();
+    }
+    c2 v[16];
     if (active) {
-      c2 v[16];
 #pragma unroll
-      for (int x = 0; x < 16; ++x) v[x] = T::mk(to_c(src[gb + xg[x]]));
+      for (int x = 0; x < 16; ++x)
+        v[x] = STAGED ? T::mk(to_c(lds[swz(roffl + xl[x])])) : T::mk(to_c(src[gb + xg[x]]));
       for (int j = 0; j <= ngate; ++j) {
         if (j == lds_after) {
 #pragma unroll
@@ -348,7 +373,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(WPE))) void
         }
         if (j < ngate) grp_apply<C>(gcode[g0 + j], v, gmat[g0 + j]);
       }
-      if (store_dst) {
+      if (!STAGED && store_dst) {
 #pragma unroll
         for (int x = 0; x < 16; ++x) dst[gb + xg[x]] = from_c<S>(make_double2(v[x].x, v[x].y));
       }
@@ -368,6 +393,22 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(WPE))) void
       acc_i = __builtin_amdgcn_mfma_f64_16x16x4f64(-tv.y, pv.x, acc_i, 0, 0, 0);
     }
     __syncthreads();
+    if (STAGED) {
+      // phi's final values (all window gates) replace its R operand in LDS,
+      // then both tiles go back to HBM in tile order (T' holds its final
+      // values since the group phase)
+      if (active && !tside) {
+#pragma unroll
+        for (int x = 0; x < 16; ++x) sP[swz(roffl + xl[x])] = from_c<S>(make_double2(v[x].x, v[x].y));
+      }
+      __syncthreads();
+      for (int l = threadIdx.x; l < (1 << t); l += blockDim.x) {
+        const uint32_t off = base + ((uint32_t)l & lomask) + hoff[l >> c0];
+        if (!first) tdst[off] = sT[swz(l)];
+        phi[off] = sP[swz(l)];
+      }
+      __syncthreads();
+    }
   }
   // combine waves: D[i][j] at lane j + 16 (i % 4), item i / 4
   double* red = reinterpret_cast<double*>(wsm);   // 4 waves x 512
@@ -646,7 +687,10 @@ void launch_window_pass(const WinPassArgs<S>& a, int nblk, int batch, hipStream_
   const size_t red = 4 * 512 * sizeof(double);
   if (shm < red) shm = red;
   constexpr int wpe = sizeof(S) == 8 ? 4 : 2;
-  hipLaunchKernelGGL((window_pass_kernel<S, wpe>), dim3(nblk, batch), dim3(256), shm, st, a);
+  if (a.staged)
+    hipLaunchKernelGGL((window_pass_kernel<S, wpe, true>), dim3(nblk, batch), dim3(256), shm, st, a);
+  else
+    hipLaunchKernelGGL((window_pass_kernel<S, wpe, false>), dim3(nblk, batch), dim3(256), shm, st, a);
 }
 template void launch_window_pass<float2>(const WinPassArgs<float2>&, int, int, hipStream_t);
 template void launch_window_pass<double2>(const WinPassArgs<double2>&, int, int, hipStream_t);
