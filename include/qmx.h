@@ -69,8 +69,9 @@ const char* qmx_last_error(const qmx_handle* h);
  * blocks per compute unit over the whole batch (1..16, default 4). */
 #define QMX_OPT_FWD_BLOCKS_PER_CU 4
 /* QMX_OPT_STAGED_WINDOWS: 1 (default) copies the tiles of windows whose gate
- * bits are the lowest state bits between HBM and LDS in tile order, 0 loads
- * their register groups directly.  Results are identical. */
+ * bits are the lowest state bits, and whose register groups stride the state
+ * on both sides, between HBM and LDS in tile order; 0 loads their register
+ * groups directly.  Results are identical. */
 #define QMX_OPT_STAGED_WINDOWS 5
 int qmx_set_option(qmx_handle* h, int option, int value);
 
