@@ -174,6 +174,18 @@ int qmx_circuit_state(qmx_handle* h, int num_qubits, int batch, int num_layers, 
                       void* fidelity, void* state_out, void* workspace, size_t workspace_bytes,
                       void* stream);
 
+/* Schedule of the forward circuit passes that qmx_circuit_state and the
+ * sweeps' circuit rebuilds launch (no reference counterpart: host-only
+ * inspection of the schedule; no GPU needed).  Forward position f is gate
+ * (layer L - 1 - f / N, site f % N).  records gets 3 ints per gate in the
+ * order the gates are applied: pass index, stage index (over all passes),
+ * f; stage_masks[stage] = the stage's 4 state bits.  num_records /
+ * num_stages are always set; returns QMX_ERR_WORKSPACE when a capacity is
+ * too small. */
+int qmx_forward_plan(int num_qubits, int num_layers, int* records, size_t record_capacity,
+                     unsigned long long* stage_masks, size_t mask_capacity, int* num_records,
+                     int* num_stages);
+
 /* Whole Sequential.prepare_state (base.py:58-106 -> sequential.py:543-600)
  * for a batch of same-size states: chi-truncation, layer extraction, sweeps
  * and the final fidelity |<psi|C|0..0>|.  Gate emission into a circuit

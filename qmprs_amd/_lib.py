@@ -87,6 +87,7 @@ SIGNATURES = {
     "qmx_sweep": (_INT, [_VP, _INT, _INT, _INT, _VP, _INT, _VP, _INT, _VP, _VP, _VP, _VP, _VP, _VP,
                          _VP, _VP, _VP, _SZ, _VP]),
     "qmx_circuit_state": (_INT, [_VP, _INT, _INT, _INT, _VP, _VP, _VP, _INT, _VP, _VP, _VP, _VP, _SZ, _VP]),
+    "qmx_forward_plan": (_INT, [_INT, _INT, _VP, _SZ, _VP, _SZ, ctypes.POINTER(_INT), ctypes.POINTER(_INT)]),
     "qmx_encode_workspace": (_SZ, [ctypes.POINTER(EncodeConfig)]),
     "qmx_encode": (_INT, [_VP, ctypes.POINTER(EncodeConfig), _VP, _VP, _VP, _VP, _VP, _VP, _VP, _VP,
                           _SZ, _VP]),
@@ -201,6 +202,24 @@ QMX_OPT_STAGED_WINDOWS = 5
 def set_option(option: int, value: int) -> None:
     h = handle()
     check(load().qmx_set_option(h, option, int(value)), h)
+
+
+def forward_plan(num_qubits: int, num_layers: int):
+    """Schedule of the forward circuit passes (host only): (records, masks)
+    with records an (K, 3) int array of (pass, stage, forward position) in
+    application order and masks the 4-bit state mask of each stage."""
+    import numpy as np
+
+    lib = load()
+    nr, ns = _INT(0), _INT(0)
+    lib.qmx_forward_plan(num_qubits, num_layers, None, 0, None, 0, ctypes.byref(nr), ctypes.byref(ns))
+    rec = np.zeros((max(nr.value, 1), 3), dtype=np.int32)
+    masks = np.zeros(max(ns.value, 1), dtype=np.uint64)
+    st = lib.qmx_forward_plan(num_qubits, num_layers, rec.ctypes.data, rec.size, masks.ctypes.data,
+                              masks.size, ctypes.byref(nr), ctypes.byref(ns))
+    if st != 0:
+        raise ValueError(f"qmx_forward_plan failed with status {st}")
+    return rec[:nr.value], masks[:ns.value]
 
 
 def set_split_sweeps(enable: bool) -> None:

@@ -291,44 +291,89 @@ GroupMap make_group(int n, uint64_t p, uint64_t u) {
   return g;
 }
 
-// Forward (circuit-order) stages: windows of consecutive gates on 4-bit sets,
-// grouped into passes of up to kMaxStages stages whose bits fit the tile.
+// Forward (circuit-order) passes over the light cone of a bit window.  Two
+// gates conflict when their worst-case bits intersect, and a gate may run
+// once every earlier conflicting gate has run.  Each pass picks the window of
+// W consecutive state bits (W = kFwdMaxUnion, or all n bits when the tile is
+// the whole state) whose closure is largest: every remaining gate inside the
+// window whose earlier conflicting remaining gates are taken too.  Across
+// layers this is a band of the staircase that runs ahead of the window's
+// lower edge, so one HBM pass applies up to W(W-1)/2 gates instead of one
+// layer's W-1.  The pass is then cut the same way into stages over
+// contiguous 4-bit sets (register groups, LDS exchange between stages).
 struct FwdPass {
   int nstage;
-  int f0[kMaxStages], f1[kMaxStages];
   uint64_t mask[kMaxStages];
+  int gstart[kMaxStages + 1];
+  int fpos[kMaxFwdGates];
 };
 
-std::vector<FwdPass> plan_fwd(int n, int kmax) {
-  std::vector<Window> st;
-  int f = 0;
-  while (f < kmax) {
-    Window w{f, f, 0};
-    while (w.s1 < kmax && w.s1 - w.s0 < kMaxWindow) {
-      uint64_t nb = w.umask | worst_bits(n, w.s1 % n);
-      if (__builtin_popcountll(nb) > 4) break;
-      w.umask = nb;
-      ++w.s1;
-    }
-    for (int bit = 0; bit < n && __builtin_popcountll(w.umask) < 4; ++bit) w.umask |= 1ull << bit;
-    st.push_back(w);
-    f = w.s1;
+// gates of rem (forward order) inside `allowed` whose earlier conflicting
+// gates of rem are taken as well
+void cone_take(int n, const std::vector<int>& rem, uint64_t allowed, std::vector<int>& took) {
+  took.clear();
+  uint64_t blocked = 0;
+  for (int f : rem) {
+    const uint64_t b = worst_bits(n, f % n);
+    if (!(b & ~allowed) && !(b & blocked))
+      took.push_back(f);
+    else
+      blocked |= b;
   }
+}
+
+std::vector<FwdPass> plan_fwd(int n, int kmax) {
+  const int W = n <= kFwdTileBits ? n : kFwdMaxUnion;
+  std::vector<int> rem(kmax), took, best, prem, sbest;
+  for (int f = 0; f < kmax; ++f) rem[f] = f;
   std::vector<FwdPass> out;
-  size_t i = 0;
-  while (i < st.size()) {
-    FwdPass p{};
-    uint64_t um = 0;
-    while (i < st.size() && p.nstage < kMaxStages) {
-      uint64_t nb = um | st[i].umask;
-      if (p.nstage > 0 && __builtin_popcountll(nb) > kFwdMaxUnion) break;
-      um = nb;
-      p.f0[p.nstage] = st[i].s0;
-      p.f1[p.nstage] = st[i].s1;
-      p.mask[p.nstage] = st[i].umask;
-      ++p.nstage;
-      ++i;
+  while (!rem.empty()) {
+    best.clear();
+    int a0 = 0;
+    for (int a = 0; a + W <= n; ++a) {
+      cone_take(n, rem, ((1ull << W) - 1ull) << a, took);
+      if (took.size() > best.size()) {
+        best = took;
+        a0 = a;
+      }
     }
+    FwdPass p{};
+    int ng = 0;
+    prem = best;
+    while (!prem.empty() && p.nstage < kMaxStages && ng < kMaxFwdGates) {
+      sbest.clear();
+      uint64_t smask = 0;
+      for (int c = a0; c + 4 <= a0 + W; ++c) {
+        cone_take(n, prem, 15ull << c, took);
+        if (took.size() > sbest.size()) {
+          sbest = took;
+          smask = 15ull << c;
+        }
+      }
+      // a prefix of a stage is still closed
+      if (ng + (int)sbest.size() > kMaxFwdGates) sbest.resize(kMaxFwdGates - ng);
+      p.mask[p.nstage] = smask;
+      p.gstart[p.nstage] = ng;
+      for (int f : sbest) p.fpos[ng++] = f;
+      ++p.nstage;
+      std::vector<int> keep;
+      size_t k = 0;
+      for (int f : prem) {
+        if (k < sbest.size() && sbest[k] == f)
+          ++k;
+        else
+          keep.push_back(f);
+      }
+      prem.swap(keep);
+    }
+    p.gstart[p.nstage] = ng;
+    // gates of the window left over (caps) stay in rem for the next passes
+    std::vector<int> keep;
+    std::vector<char> taken(kmax, 0);
+    for (int j = 0; j < ng; ++j) taken[p.fpos[j]] = 1;
+    for (int f : rem)
+      if (!taken[f]) keep.push_back(f);
+    rem.swap(keep);
     out.push_back(p);
   }
   return out;
@@ -566,46 +611,53 @@ std::vector<std::vector<int>> forward_circuit_passes(int n, int lmax) {
   return plan_passes(n, pos, sites, kApplyMaxUnion);
 }
 
-// Forward circuit passes (register-group stages).  Needs n >= 5.
+// Forward circuit passes (register-group stages) from |0..0>.  Needs n >= 5.
+// The support of the state grows with the bits the gates touch; passes
+// before it covers every bit visit only the tiles inside it.
 template <typename S>
 int run_fwd(qmx_handle* h, int n, int batch, int lmax, const std::vector<FwdPass>& passes, S* psi,
             const double2* gates, const int* widths, const int* layers, const int* skip,
             const double2* ref, double* dot_part, int* dot_counter, double2* dot_out,
             int dot_stride, hipStream_t st) {
+  if (passes.empty()) return fail(h, QMX_ERR_INVALID, "empty forward-pass plan");
   const int t = std::min(n, kFwdTileBits);
-  const long ntiles = 1l << (n - t);
-  const int nblk = (int)std::max(1l, std::min(ntiles, ((long)h->fwd_blocks_per_cu * h->cus + batch - 1) / batch));
+  const uint64_t all = (1ull << n) - 1ull;
+  uint64_t live = 0;
   for (size_t pi = 0; pi < passes.size(); ++pi) {
     const FwdPass& fp = passes[pi];
-    uint64_t um = 0;
+    uint64_t um = 0, touched = 0;
     for (int k = 0; k < fp.nstage; ++k) um |= fp.mask[k];
+    for (int j = 0; j < fp.gstart[fp.nstage]; ++j) touched |= worst_bits(n, fp.fpos[j] % n);
     const uint64_t p = tile_bits(n, t, um);
     FwdArgs<S> a{};
     a.n = n;
     a.lmax = lmax;
     a.t = t;
     a.nstage = fp.nstage;
-    a.qmask = ((1ull << n) - 1ull) & ~p;
-    for (int k = 0; k < fp.nstage; ++k) {
-      a.gs[k] = make_group(n, p, fp.mask[k]);
-      a.f0[k] = fp.f0[k];
-      a.f1[k] = fp.f1[k];
-    }
+    a.from_basis0 = pi == 0 ? 1 : 0;
+    a.live_in = live;
+    live |= touched;
+    a.live_out = live;
+    a.full = (live == all || pi + 1 == passes.size()) ? 1 : 0;
+    const uint64_t qmask = all & ~p;
+    a.tmask = a.full ? qmask : (qmask & live);
+    for (int k = 0; k < fp.nstage; ++k) a.gs[k] = make_group(n, p, fp.mask[k]);
+    for (int k = 0; k <= fp.nstage; ++k) a.gstart[k] = fp.gstart[k];
+    for (int j = 0; j < fp.gstart[fp.nstage]; ++j) a.fpos[j] = fp.fpos[j];
     a.psi = psi;
     a.gates = gates;
     a.widths = widths;
     a.layers = layers;
     a.skip = skip;
-    // the overlap with ref is a separate coalesced pass (below): in the last
-    // circuit pass the gate bits are the lowest ones, where a fused read of
-    // the complex128 reference would be strided
-    a.ref = nullptr;
-    a.dot_part = dot_part;
-    a.dot_counter = dot_counter;
-    a.dot_out = dot_out;
-    a.dot_stride = dot_stride;
+    const long ntiles = 1l << __builtin_popcountll(a.tmask);
+    const int nblk =
+        (int)std::max(1l, std::min(ntiles, ((long)h->fwd_blocks_per_cu * h->cus + batch - 1) / batch));
     launch_fwd_pass(a, nblk, batch, st);
+    if (a.full) live = all;
   }
+  // the overlap with ref is a separate coalesced pass: in the last circuit
+  // pass the gate bits are the lowest ones, where a fused read of the
+  // complex128 reference would be strided
   if (ref) launch_overlap<S>(psi, ref, n, batch, dot_part, dot_counter, dot_out, dot_stride, skip, st);
   QMX_LAUNCHED(h, "forward passes");
   return QMX_OK;
@@ -665,6 +717,7 @@ struct SweepJob {
   const int* skip = nullptr;
   bool use_goal = false;
   bool active = true;
+  std::vector<FwdPass> fwd_plan;   // forward passes of the circuit rebuild
 };
 
 template <typename S>
@@ -688,13 +741,14 @@ template <typename S>
 int job_rebuild(qmx_handle* h, SweepJob<S>& J, int s, bool final_state) {
   const int n = J.n;
   const int* skip = final_state ? nullptr : J.skip;
-  launch_basis0(J.L.phi, n, J.batch, skip, J.st);
   const double2* ref = J.fidelity ? J.psi_ref : nullptr;
   double2* dot = J.fidelity ? J.fidelity + s : nullptr;
   if (n >= 5 && h->sweep_mode != QMX_SWEEP_STEP) {
-    QMX_CALL(run_fwd<S>(h, n, J.batch, J.lmax, plan_fwd(n, J.lmax * n), J.L.phi, J.gates, J.widths,
-                        J.layers_used, skip, ref, J.L.dot_part, J.L.dot_counter, dot, J.nsweeps + 1, J.st));
+    if (J.fwd_plan.empty()) J.fwd_plan = plan_fwd(n, J.lmax * n);
+    QMX_CALL(run_fwd<S>(h, n, J.batch, J.lmax, J.fwd_plan, J.L.phi, J.gates, J.widths, J.layers_used, skip,
+                        ref, J.L.dot_part, J.L.dot_counter, dot, J.nsweeps + 1, J.st));
   } else {
+    launch_basis0(J.L.phi, n, J.batch, skip, J.st);
     QMX_CALL(run_passes<S>(h, n, J.batch, J.lmax, 1, 0, forward_circuit_passes(n, J.lmax), J.L.phi,
                            J.gates, J.widths, J.layers_used, skip, ref, J.L.dot_part, J.L.dot_counter,
                            dot, J.nsweeps + 1, h->cus, J.st));
@@ -1056,6 +1110,37 @@ int qmx_set_option(qmx_handle* h, int option, int value) {
     return QMX_OK;
   }
   return fail(h, QMX_ERR_INVALID, "unknown option");
+}
+
+int qmx_forward_plan(int num_qubits, int num_layers, int* records, size_t record_capacity,
+                     unsigned long long* stage_masks, size_t mask_capacity, int* num_records,
+                     int* num_stages) {
+  if (num_qubits < 5 || num_qubits > kMaxQubits || num_layers < 1 || !num_records || !num_stages)
+    return QMX_ERR_INVALID;
+  const auto plan = plan_fwd(num_qubits, num_layers * num_qubits);
+  int nr = 0, ns = 0;
+  for (const FwdPass& p : plan) {
+    nr += p.gstart[p.nstage];
+    ns += p.nstage;
+  }
+  *num_records = nr;
+  *num_stages = ns;
+  if (!records || !stage_masks || record_capacity < static_cast<size_t>(3) * nr ||
+      mask_capacity < static_cast<size_t>(ns))
+    return QMX_ERR_WORKSPACE;
+  int r = 0, sg = 0;
+  for (size_t pi = 0; pi < plan.size(); ++pi) {
+    const FwdPass& p = plan[pi];
+    for (int k = 0; k < p.nstage; ++k, ++sg) {
+      stage_masks[sg] = p.mask[k];
+      for (int j = p.gstart[k]; j < p.gstart[k + 1]; ++j, ++r) {
+        records[3 * r] = static_cast<int>(pi);
+        records[3 * r + 1] = sg;
+        records[3 * r + 2] = p.fpos[j];
+      }
+    }
+  }
+  return QMX_OK;
 }
 
 int qmx_profile_enable(qmx_handle* h, int enable) {

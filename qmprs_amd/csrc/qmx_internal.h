@@ -100,25 +100,27 @@ struct WinPassArgs {
 
 // Forward circuit pass (qmx_window.hip): up to kMaxStages stages of gates,
 // each stage acting inside its own 4-bit group set, exchanged through LDS.
-constexpr int kMaxStages = 3;
+// Stage k applies the gates at forward positions fpos[gstart[k] .. gstart[k+1]).
+constexpr int kMaxStages = 12;
+constexpr int kMaxFwdGates = 64;
 constexpr int kFwdTileBits = 12;
-constexpr int kFwdMaxUnion = 8;      // stage bits per pass (>= 4 coalescing bits)
+constexpr int kFwdMaxUnion = 8;      // gate bits per pass (>= 4 coalescing bits)
 template <typename S>
 struct FwdArgs {
   int n, lmax, t, nstage;
-  uint64_t qmask;
+  int from_basis0;     // the input is |0..0> (psi is not read)
+  int full;            // write every amplitude (else only those inside live_out)
+  uint64_t tmask;      // tile-index bits visited (state bits outside the tile)
+  uint64_t live_in;    // amplitudes with a bit outside live_in are zero before the pass
+  uint64_t live_out;   // ... and outside live_out after it
   GroupMap gs[kMaxStages];
-  int f0[kMaxStages], f1[kMaxStages];   // forward positions of each stage
+  int gstart[kMaxStages + 1];
+  int fpos[kMaxFwdGates];
   S* psi;
   const double2* gates;
   const int* widths;
   const int* layers;
   const int* skip;
-  const double2* ref;  // optional overlap reference (complex128)
-  double* dot_part;
-  int* dot_counter;
-  double2* dot_out;
-  int dot_stride;
 };
 
 struct WinFinArgs {

@@ -127,22 +127,6 @@ QMX_DEV void wgate<double2>(double2* s, int t, int width, int m1, int m0, const 
 
 QMX_DEV int wsane(int n, int site, int w) { return (w == 2 && site < n - 1) ? 2 : 1; }
 
-// block sum of two doubles per lane; red[0..1] valid for all lanes after return
-QMX_DEV void block_sum2(double (&v)[2], double* red) {
-  const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6, nw = blockDim.x >> 6;
-  for (int j = 0; j < 2; ++j) {
-    double s = wave_sum(v[j]);
-    if (lane == 0) red[wid * 2 + j] = s;
-  }
-  __syncthreads();
-  if (threadIdx.x < 2) {
-    double s = 0.0;
-    for (int w = 0; w < nw; ++w) s += red[w * 2 + threadIdx.x];
-    red[threadIdx.x] = s;
-  }
-  __syncthreads();
-}
-
 // step -> (layer, site) of the sweep order (reverse circuit order)
 QMX_DEV void step_gate(int n, int s, int& layer, int& site) {
   layer = s / n;
@@ -428,64 +412,64 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(WPE))) void
 }
 
 // ---- forward circuit pass ------------------------------------------------
-// Applies the gates of up to kMaxStages stages (forward circuit positions
-// f0..f1-1 each; gate (layer L_b - 1 - f / N, site f % N)) to every state.
-// Stage 0 reads HBM, the last stage writes HBM (and accumulates
-// sum conj(ref) * psi when ref is given), intermediate stages exchange
-// through the LDS tile.
+// Applies the gates of up to kMaxStages stages (stage k: forward circuit
+// positions fpos[gstart[k] .. gstart[k+1]), gate (layer L_b - 1 - f / N,
+// site f % N), positions f >= L_b N skipped) to every state.  Stage 0 reads
+// HBM, the last stage writes HBM, intermediate stages exchange through the
+// LDS tile.
+// Support: the circuit starts from |0..0>, and a gate only spreads the state
+// over its own bits, so before the pass every amplitude at an index with a
+// bit outside live_in is zero (not read: the memory there may be stale) and
+// after it every amplitude outside live_out is.  Only tiles inside live_out
+// are visited (tmask: the tile-index bits), and only amplitudes inside
+// live_out are written, except in a full pass, which writes every amplitude.
+// from_basis0: the input is |0..0> itself (psi is not read).
 template <typename S>
 __global__ __launch_bounds__(256) void fwd_pass_kernel(FwdArgs<S> a) {
   using C = typename StoreC<S>::C;
   using T = CT<C>;
   using c2 = typename T::c2;
   extern __shared__ char wsm[];
-  __shared__ c2 gmat[kMaxStages][kMaxWindow][16];
-  __shared__ int gcode[kMaxStages][kMaxWindow];
-  __shared__ int gcount[kMaxStages];
-  __shared__ double red[4 * 2];
-  __shared__ double scratch[256];
-  __shared__ double tot[2];
-  __shared__ int s_flag;
+  __shared__ c2 gmat[kMaxFwdGates][16];
+  __shared__ int gcode[kMaxFwdGates];
   const int b = blockIdx.y;
   const int n = a.n;
   if (a.skip && a.skip[b]) return;
   const int lb = a.layers[b];
-  // one gate entry per thread (kMaxStages * kMaxWindow * 16 <= 2 * 256)
-  for (int t = threadIdx.x; t < kMaxStages * kMaxWindow * 16; t += blockDim.x) {
-    const int e = t & 15, gslot = t >> 4;
-    const int st = gslot / kMaxWindow, j = gslot % kMaxWindow;
-    if (st < a.nstage) {
-      const int f = a.f0[st] + j;
-      if (f < a.f1[st] && f < lb * n) {
-        const int layer = lb - 1 - f / n, site = f % n;
-        const size_t gi = ((size_t)b * a.lmax + layer) * n + site;
-        // grp_apply applies the adjoint of what it is given: stage G^dagger
-        const double2 gv = a.gates[gi * 16 + (e & 3) * 4 + (e >> 2)];
-        gmat[st][j][e] = T::mk(make_double2(gv.x, -gv.y));
-        if (e == 0) gcode[st][j] = gate_code(a.gs[st].mask, n, site, wsane(n, site, a.widths[gi]));
-      }
+  const int ng = a.gstart[a.nstage];
+  // one gate entry per thread (kMaxFwdGates * 16 <= 4 * 256); positions past
+  // this state's circuit get code -1 (no gate)
+  for (int t = threadIdx.x; t < ng * 16; t += blockDim.x) {
+    const int e = t & 15, j = t >> 4;
+    const int f = a.fpos[j];
+    if (f < lb * n) {
+      int st = 0;
+      while (a.gstart[st + 1] <= j) ++st;
+      const int layer = lb - 1 - f / n, site = f % n;
+      const size_t gi = ((size_t)b * a.lmax + layer) * n + site;
+      // grp_apply applies the adjoint of what it is given: stage G^dagger
+      const double2 gv = a.gates[gi * 16 + (e & 3) * 4 + (e >> 2)];
+      gmat[j][e] = T::mk(make_double2(gv.x, -gv.y));
+      if (e == 0) gcode[j] = gate_code(a.gs[st].mask, n, site, wsane(n, site, a.widths[gi]));
+    } else if (e == 0) {
+      gcode[j] = -1;
     }
-  }
-  if (threadIdx.x < kMaxStages) {
-    const int st = threadIdx.x;
-    int c = 0;
-    if (st < a.nstage) c = max(0, min(a.f1[st], lb * n) - a.f0[st]);
-    gcount[st] = c;
   }
   __syncthreads();
   int total = 0;
-  for (int st = 0; st < a.nstage; ++st) total += gcount[st];
-  if (total == 0 && a.ref == nullptr) return;
+  for (int j = 0; j < ng; ++j) total += a.fpos[j] < lb * n ? 1 : 0;
+  const uint64_t all = (1ull << n) - 1ull;
+  // a dense pass without gates leaves the state as it is
+  if (total == 0 && !a.from_basis0 && a.live_in == all) return;
   const int t = a.t;
   const size_t len = (size_t)1 << n;
   S* lds = reinterpret_cast<S*>(wsm);
   S* psi = a.psi + (size_t)b * len;
-  const double2* ref = a.ref ? a.ref + (size_t)b * len : nullptr;
-  const long ntiles = 1l << (n - t);
+  const long ntiles = 1l << __popcll(a.tmask);
   const int ngroups = 1 << (t - 4);
-  double acc[2] = {0.0, 0.0};
+  const uint32_t dead_in = (uint32_t)(~a.live_in & all), dead_out = (uint32_t)(~a.live_out & all);
   for (long tile = blockIdx.x; tile < ntiles; tile += gridDim.x) {
-    const uint32_t base = (uint32_t)wdeposit((uint64_t)tile, a.qmask);
+    const uint32_t base = (uint32_t)wdeposit((uint64_t)tile, a.tmask);
     for (int st = 0; st < a.nstage; ++st) {
       const GroupMap& gm = a.gs[st];
       const bool last = (st + 1 == a.nstage);
@@ -495,23 +479,25 @@ __global__ __launch_bounds__(256) void fwd_pass_kernel(FwdArgs<S> a) {
         c2 v[16];
         if (st == 0) {
 #pragma unroll
-          for (int x = 0; x < 16; ++x) v[x] = T::mk(to_c(psi[base + roffg + xoffs(gm.ubit, x)]));
+          for (int x = 0; x < 16; ++x) {
+            const uint32_t ad = base + roffg + xoffs(gm.ubit, x);
+            if (ad & dead_in)
+              v[x] = T::mk(make_double2(0.0, 0.0));
+            else if (a.from_basis0)
+              v[x] = T::mk(make_double2(ad == 0u ? 1.0 : 0.0, 0.0));
+            else
+              v[x] = T::mk(to_c(psi[ad]));
+          }
         } else {
 #pragma unroll
           for (int x = 0; x < 16; ++x) v[x] = T::mk(to_c(lds[swz(roffl + (int)xoffs(gm.ux, x))]));
         }
-        for (int j = 0; j < gcount[st]; ++j) grp_apply<C>(gcode[st][j], v, gmat[st][j]);
+        for (int j = a.gstart[st]; j < a.gstart[st + 1]; ++j) grp_apply<C>(gcode[j], v, gmat[j]);
         if (last) {
 #pragma unroll
           for (int x = 0; x < 16; ++x) {
             const uint32_t ad = base + roffg + xoffs(gm.ubit, x);
-            S sv = from_c<S>(make_double2(v[x].x, v[x].y));
-            if (total > 0) psi[ad] = sv;
-            if (ref) {
-              double2 r = ref[ad], pv = to_c(sv);
-              acc[0] = fma(r.x, pv.x, fma(r.y, pv.y, acc[0]));
-              acc[1] = fma(r.x, pv.y, fma(-r.y, pv.x, acc[1]));
-            }
+            if (a.full || !(ad & dead_out)) psi[ad] = from_c<S>(make_double2(v[x].x, v[x].y));
           }
         } else {
 #pragma unroll
@@ -520,38 +506,6 @@ __global__ __launch_bounds__(256) void fwd_pass_kernel(FwdArgs<S> a) {
       }
       // stage exchange, and the next tile's stage 0 must not overwrite LDS early
       if (a.nstage > 1) __syncthreads();
-    }
-  }
-  if (ref) {
-    block_sum2(acc, red);
-    double* pb = a.dot_part + (size_t)b * gridDim.x * 2;
-    if (threadIdx.x < 2) pb[(size_t)blockIdx.x * 2 + threadIdx.x] = red[threadIdx.x];
-    __threadfence();
-    __syncthreads();
-    if (threadIdx.x == 0) {
-      int prev = atomicAdd(a.dot_counter + b, 1);
-      s_flag = (prev == (int)gridDim.x - 1) ? 1 : 0;
-    }
-    __syncthreads();
-    if (s_flag) {
-      __threadfence();
-      // fixed-order sum of the block partials
-      const int groups = blockDim.x / 2;
-      const int jv = threadIdx.x % 2, g0 = threadIdx.x / 2;
-      double s = 0.0;
-      for (int q = g0; q < (int)gridDim.x; q += groups) s += pb[(size_t)q * 2 + jv];
-      scratch[threadIdx.x] = s;
-      __syncthreads();
-      if (threadIdx.x < 2) {
-        double z = 0.0;
-        for (int gg = 0; gg < groups; ++gg) z += scratch[gg * 2 + threadIdx.x];
-        tot[threadIdx.x] = z;
-      }
-      __syncthreads();
-      if (threadIdx.x == 0) {
-        a.dot_out[(size_t)b * a.dot_stride] = make_double2(tot[0], tot[1]);
-        a.dot_counter[b] = 0;
-      }
     }
   }
 }

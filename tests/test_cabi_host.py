@@ -217,3 +217,96 @@ def test_qiskit_adapter_is_optional():
     except ImportError:
         with pytest.raises(ImportError):
             wire.to_qiskit([], 2)
+
+
+def _worst_bits(n, site):
+    m = 1 << (n - 1 - site)
+    if site < n - 1:
+        m |= 1 << (n - 2 - site)
+    return m
+
+
+@pytest.mark.parametrize("n,layers", [(5, 1), (5, 4), (8, 3), (12, 6), (13, 5), (16, 10), (20, 10),
+                                      (24, 3)])
+def test_forward_plan_is_a_valid_schedule(n, layers):
+    """Every gate exactly once, inside its stage's 4 contiguous bits, stages of
+    a pass inside one window of at most 8 bits (n > 12), and for every qubit
+    the gates touching it in forward order (the reference applies layers in
+    order, sequential.py:173-213; sites of a block in order)."""
+    mod, _ = _lib()
+    rec, masks = mod.forward_plan(n, layers)
+    k = layers * n
+    assert sorted(rec[:, 2].tolist()) == list(range(k))
+    for m in masks.tolist():
+        assert bin(m).count("1") == 4
+        low = (m & -m).bit_length() - 1
+        assert m == 15 << low
+    union = {}
+    for p, st, f in rec.tolist():
+        assert _worst_bits(n, f % n) & ~int(masks[st]) == 0
+        union[p] = union.get(p, 0) | int(masks[st])
+    if n > 12:
+        for u in union.values():
+            assert bin(u).count("1") <= 8
+    last = [-1] * n
+    for _, _, f in rec.tolist():
+        b = _worst_bits(n, f % n)
+        for q in range(n):
+            if (b >> q) & 1:
+                assert f > last[q]
+                last[q] = f
+    # stage indices and pass indices only grow
+    assert np.all(np.diff(rec[:, 0]) >= 0) and np.all(np.diff(rec[:, 1]) >= 0)
+    if (n, layers) == (20, 10):
+        # the light cone of an 8-bit window: 11 passes instead of one per
+        # 7 gates of a layer (29)
+        assert rec[-1, 0] + 1 <= 12
+
+
+def test_forward_plan_applies_the_same_circuit():
+    """Applying random gates in the plan's order gives the state of applying
+    them in forward order (n = 13: more than one tile window)."""
+    mod, _ = _lib()
+    rng = np.random.default_rng(5)
+    n, layers = 13, 4
+    k = layers * n
+    gates = []
+    for f in range(k):
+        site = f % n
+        d = 4 if site < n - 1 else 2
+        q, _ = np.linalg.qr(rng.normal(size=(d, d)) + 1j * rng.normal(size=(d, d)))
+        gates.append(q)
+
+    def apply(psi, f):
+        site = f % n
+        t = psi.reshape([2] * n)   # axis a = qubit n - 1 - a (site a)
+        g = gates[f]
+        if g.shape[0] == 4:
+            t = np.moveaxis(t, (site, site + 1), (0, 1)).reshape(4, -1)
+            t = (g @ t).reshape([2, 2] + [2] * (n - 2))
+            t = np.moveaxis(t, (0, 1), (site, site + 1))
+        else:
+            t = np.moveaxis(t, site, 0).reshape(2, -1)
+            t = np.moveaxis((g @ t).reshape([2] + [2] * (n - 1)), 0, site)
+        return t.reshape(-1)
+
+    psi0 = np.zeros(1 << n, complex)
+    psi0[0] = 1
+    ref = psi0
+    for f in range(k):
+        ref = apply(ref, f)
+    rec, _ = mod.forward_plan(n, layers)
+    out = psi0
+    for f in rec[:, 2].tolist():
+        out = apply(out, f)
+    assert np.allclose(out, ref, atol=1e-12)
+    assert not np.allclose(ref, psi0)
+
+
+def test_forward_plan_rejects_bad_sizes():
+    mod, lib = _lib()
+    nr, ns = ctypes.c_int(0), ctypes.c_int(0)
+    assert lib.qmx_forward_plan(4, 2, None, 0, None, 0, ctypes.byref(nr), ctypes.byref(ns)) != 0
+    assert lib.qmx_forward_plan(31, 2, None, 0, None, 0, ctypes.byref(nr), ctypes.byref(ns)) != 0
+    st = lib.qmx_forward_plan(10, 2, None, 0, None, 0, ctypes.byref(nr), ctypes.byref(ns))
+    assert st == mod.QMX_ERR_WORKSPACE and nr.value == 20 and ns.value >= 1

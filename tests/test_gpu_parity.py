@@ -375,3 +375,42 @@ def test_staged_windows_equal_direct(gpu, dtype):
     torch.cuda.synchronize()
     assert torch.equal(direct.gates, staged.gates)
     assert torch.equal(direct.fidelity, staged.fidelity)
+
+
+def _random_unitary(rng, d):
+    q, r = np.linalg.qr(rng.normal(size=(d, d)) + 1j * rng.normal(size=(d, d)))
+    return q * (np.diag(r) / np.abs(np.diag(r)))
+
+
+@pytest.mark.parametrize("n,layers,seed", [(13, 3, 0), (16, 4, 1), (20, 3, 2)])
+def test_circuit_state_large_n(gpu, n, layers, seed):
+    """qmx_circuit_state at sizes with many tiles (the forward passes run over
+    the light cone of 8-bit windows and skip tiles outside the state's
+    support) against the oracle's dense forward application, for two states
+    of different layer counts in one batch; random unitaries, blocks ending
+    at random sites."""
+    eng = _engine()
+    rng = np.random.default_rng(seed)
+    B = 2
+    g = np.zeros((B, layers, n, 4, 4), dtype=np.complex128)
+    w = np.full((B, layers, n), 2, dtype=np.int32)
+    for b in range(B):
+        for l in range(layers):
+            w[b, l, n - 1] = 1
+            for s in rng.choice(n - 1, size=2, replace=False):
+                w[b, l, s] = 1
+            for s in range(n):
+                d = 4 if w[b, l, s] == 2 else 2
+                g[b, l, s, :d, :d] = _random_unitary(rng, d)
+    used = np.array([layers, layers - 1], dtype=np.int32)
+    gates = torch.from_numpy(g).to(gpu).contiguous()
+    widths = torch.from_numpy(w).to(gpu).contiguous()
+    lu = torch.from_numpy(used).to(gpu)
+    state = eng.circuit_state(gates, widths, lu, dtype=torch.complex128)
+    torch.cuda.synchronize()
+    for b in range(B):
+        gl = _gate_list_from_device(g[b], w[b], int(used[b]))
+        ref = O.circuit_state(n, gl)
+        err = np.abs(state[b].cpu().numpy() - ref).max()
+        print(f"n={n} state {b} layers {used[b]} max err {err:.3e}")
+        assert err < 1e-12
