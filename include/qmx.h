@@ -68,10 +68,10 @@ const char* qmx_last_error(const qmx_handle* h);
 /* QMX_OPT_FWD_BLOCKS_PER_CU: grid of the forward-circuit rebuild passes, in
  * blocks per compute unit over the whole batch (1..16, default 4). */
 #define QMX_OPT_FWD_BLOCKS_PER_CU 4
-/* QMX_OPT_STAGED_WINDOWS: 1 (default) copies the tiles of windows whose gate
- * bits are the lowest state bits, and whose register groups stride the state
- * on both sides, between HBM and LDS in tile order; 0 loads their register
- * groups directly.  Results are identical. */
+/* QMX_OPT_STAGED_WINDOWS: windows whose register groups stride the state
+ * (gate bits at the lowest state bits) can copy their tiles between HBM and
+ * LDS in tile order.  1 (default) does so when the tile's contiguous low run
+ * is at least 2^7 amplitudes, 2 from 2^3, 0 never.  Results are identical. */
 #define QMX_OPT_STAGED_WINDOWS 5
 int qmx_set_option(qmx_handle* h, int option, int value);
 

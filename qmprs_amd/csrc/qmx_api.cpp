@@ -34,7 +34,7 @@ struct qmx_handle {
   bool split_sweeps = true;             // QMX_OPT_SPLIT_SWEEPS
   int win_blocks_per_cu = 16;           // QMX_OPT_WINDOW_BLOCKS_PER_CU
   int fwd_blocks_per_cu = 4;            // QMX_OPT_FWD_BLOCKS_PER_CU
-  bool staged_windows = true;           // QMX_OPT_STAGED_WINDOWS
+  int staged_windows = 1;               // QMX_OPT_STAGED_WINDOWS (0, 1, 2)
   hipStream_t aux[2] = {nullptr, nullptr};
   hipEvent_t pass_ev[2] = {nullptr, nullptr};
   hipEvent_t join_ev[2] = {nullptr, nullptr};
@@ -794,14 +794,18 @@ void job_window_pass(qmx_handle* h, SweepJob<S>& J, const std::vector<Window>& w
   pa.layers = J.layers_used;
   pa.skip = J.skip;
   pa.part = J.L.part;
-  // when the groups of both sides stride the state by >= 4 amplitudes per
-  // lane (the tile is the lowest state bits), staging the tiles through LDS
-  // in tile order is faster (measured at N=20: 289 vs 320 us per launch);
-  // with one side strided it is slower, so it is not used there
+  // When a side's groups stride the state by >= 4 amplitudes per lane (its
+  // gate bits are the lowest state bits), staging the tiles through LDS in
+  // tile order pays if the tile's contiguous low run is long enough.
+  // Measured at N=20 (us per launch, direct -> staged): both sides strided
+  // 322 -> 244, one side by 16 amplitudes 355 -> 287 (low run 7), one side
+  // by 4 amplitudes 287 -> 271 (low run 9) but 251 -> 273 (low run 6).
+  // Mode 1 (default) stages from a low run of 7 bits, mode 2 from 3.
   const int low_run = __builtin_ctzll(~pbits);
-  const int stride_bits = std::min(__builtin_ctzll(pa.gu.restg | (1ull << 63)),
+  const int stride_bits = std::max(__builtin_ctzll(pa.gu.restg | (1ull << 63)),
                                    __builtin_ctzll(pa.gp.restg | (1ull << 63)));
-  pa.staged = (h->staged_windows && stride_bits >= 2 && low_run >= 3 && pa.t - low_run <= 8) ? 1 : 0;
+  const int min_run = h->staged_windows == 2 ? 3 : 7;
+  pa.staged = (h->staged_windows && stride_bits >= 2 && low_run >= min_run && pa.t - low_run <= 8) ? 1 : 0;
   PhaseTimer pt(h, J.st, QMX_PHASE_WINDOW_PASS);
   launch_window_pass(pa, wblk, J.batch, J.st);
   pt.stop(1);
@@ -1095,8 +1099,8 @@ int qmx_set_option(qmx_handle* h, int option, int value) {
     return QMX_OK;
   }
   if (option == QMX_OPT_STAGED_WINDOWS) {
-    if (value != 0 && value != 1) return fail(h, QMX_ERR_INVALID, "staged windows: 0 or 1");
-    h->staged_windows = value != 0;
+    if (value < 0 || value > 2) return fail(h, QMX_ERR_INVALID, "staged windows: 0, 1 or 2");
+    h->staged_windows = value;
     return QMX_OK;
   }
   if (option == QMX_OPT_FWD_BLOCKS_PER_CU) {

@@ -337,10 +337,33 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(WPE))) void
     const uint32_t base = (uint32_t)wdeposit((uint64_t)tile, a.qmask);
     const uint32_t gb = base + roffg;
     if (STAGED) {
-      for (int l = threadIdx.x; l < (1 << t); l += blockDim.x) {
-        const uint32_t off = base + ((uint32_t)l & lomask) + hoff[l >> c0];
-        sT[swz(l)] = tsrc[off];
-        sP[swz(l)] = phi[off];
+      constexpr int kPer = (1 << kWinTileBits) / 256;
+      if (t == kWinTileBits && blockDim.x == 256) {
+        // half of the loads in flight before the first LDS store (all of
+        // them would not fit the register budget of 4 waves per SIMD)
+#pragma unroll 1
+        for (int h = 0; h < 2; ++h) {
+          S tv[kPer / 2], pv[kPer / 2];
+#pragma unroll
+          for (int i = 0; i < kPer / 2; ++i) {
+            const int l = threadIdx.x + 256 * (h * (kPer / 2) + i);
+            const uint32_t off = base + ((uint32_t)l & lomask) + hoff[l >> c0];
+            tv[i] = tsrc[off];
+            pv[i] = phi[off];
+          }
+#pragma unroll
+          for (int i = 0; i < kPer / 2; ++i) {
+            const int l = threadIdx.x + 256 * (h * (kPer / 2) + i);
+            sT[swz(l)] = tv[i];
+            sP[swz(l)] = pv[i];
+          }
+        }
+      } else {
+        for (int l = threadIdx.x; l < (1 << t); l += blockDim.x) {
+          const uint32_t off = base + ((uint32_t)l & lomask) + hoff[l >> c0];
+          sT[swz(l)] = tsrc[off];
+          sP[swz(l)] = phi[off];
+        }
       }
       __syncthreads();
     }

@@ -357,11 +357,13 @@ def test_encode_ragged_rejects_bad_input(gpu):
         eng.encode_ragged([np.ones(8, np.complex64)], [0], 2, 1)
 
 
-@pytest.mark.parametrize("dtype", [torch.complex64, torch.complex128])
-def test_staged_windows_equal_direct(gpu, dtype):
+@pytest.mark.parametrize("dtype,mode", [(torch.complex64, 1), (torch.complex128, 1), (torch.complex64, 2),
+                                        (torch.complex128, 2)])
+def test_staged_windows_equal_direct(gpu, dtype, mode):
     """Windows whose gate bits are the lowest state bits move their tiles
-    through LDS in tile order; the arithmetic is the same as the direct group
-    loads, so results are identical bit for bit."""
+    through LDS in tile order (mode 1: tiles with a contiguous low run of
+    2^7 amplitudes, mode 2: from 2^3); the arithmetic is the same as the
+    direct group loads, so results are identical bit for bit."""
     from qmprs_amd import _lib
     eng = _engine()
     n, chi, L, S = 13, 16, 3, 2
@@ -369,9 +371,10 @@ def test_staged_windows_equal_direct(gpu, dtype):
     _lib.set_option(_lib.QMX_OPT_STAGED_WINDOWS, 0)
     try:
         direct = eng.encode(svs, chi, L, S, sweep_dtype=dtype)
+        _lib.set_option(_lib.QMX_OPT_STAGED_WINDOWS, mode)
+        staged = eng.encode(svs, chi, L, S, sweep_dtype=dtype)
     finally:
         _lib.set_option(_lib.QMX_OPT_STAGED_WINDOWS, 1)
-    staged = eng.encode(svs, chi, L, S, sweep_dtype=dtype)
     torch.cuda.synchronize()
     assert torch.equal(direct.gates, staged.gates)
     assert torch.equal(direct.fidelity, staged.fidelity)
