@@ -794,6 +794,9 @@ void job_window_pass(qmx_handle* h, SweepJob<S>& J, const std::vector<Window>& w
   pa.layers = J.layers_used;
   pa.skip = J.skip;
   pa.part = J.L.part;
+  // after a sweep's last window neither T' nor phi is read again (the next
+  // sweep rebuilds phi from |0..0> and restarts T' from the target)
+  pa.store = wi + 1 < wins.size() ? 1 : 0;
   // When a side's groups stride the state by >= 4 amplitudes per lane (its
   // gate bits are the lowest state bits), staging the tiles through LDS in
   // tile order pays if the tile's contiguous low run is long enough.

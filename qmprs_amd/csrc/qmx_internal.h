@@ -96,6 +96,7 @@ struct WinPassArgs {
   const int* skip;
   double* part;        // [B][nblk][512]
   int staged;          // copy tiles through LDS in tile order (see window_pass_kernel)
+  int store;           // write T' and phi back (0 for a sweep's last window: neither is read again)
 };
 
 // Forward circuit pass (qmx_window.hip): up to kMaxStages stages of gates,

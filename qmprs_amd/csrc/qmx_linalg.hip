@@ -227,6 +227,25 @@ __global__ __launch_bounds__(1024) void jacobi_lds_sweep_kernel(double2* a_all, 
   __syncthreads();
   const int np = n / 2;
   const double abs_tol = 1e-17 * s_scale + 1e-300;
+  // this thread's pair blocks (P, Q), P <= Q, row-major over the upper
+  // triangle of pair blocks; the same in every round (the kernel is
+  // launched with 1024 threads: kBlkPer * 1024 covers every block)
+  constexpr int kBlkPer = (kJacobiLdsMax / 2 * (kJacobiLdsMax / 2 + 1) / 2 + 1023) / 1024;
+  const int nblocks = np * (np + 1) / 2;
+  int bP[kBlkPer], bQ[kBlkPer];
+#pragma unroll
+  for (int k = 0; k < kBlkPer; ++k) {
+    const int e = tid + k * nt;
+    int P = 0, Q = 0;
+    if (e < nblocks) {
+      P = (int)((2.0 * np + 1.0 - sqrt((2.0 * np + 1.0) * (2.0 * np + 1.0) - 8.0 * e)) * 0.5);
+      while (P > 0 && tri_index(P, P, np) > e) --P;
+      while (P + 1 < np && tri_index(P + 1, P + 1, np) <= e) ++P;
+      Q = P + (e - tri_index(P, P, np));
+    }
+    bP[k] = P;
+    bQ[k] = Q;
+  }
   for (int r = 0; r < n - 1; ++r) {
     if (tid < np) {
       const int m = tid;
@@ -253,13 +272,10 @@ __global__ __launch_bounds__(1024) void jacobi_lds_sweep_kernel(double2* a_all, 
     __syncthreads();
     // blocks (P, Q), P <= Q: B' = J_P^H B J_Q with B = A[{p0, p1}][{q0, q1}];
     // (Q, P) is its conjugate transpose and is never stored separately
-    const int nblocks = np * (np + 1) / 2;
-    for (int e = tid; e < nblocks; e += nt) {
-      // e -> (P, Q), P <= Q, row-major over the upper triangle of pair blocks
-      int P = (int)((2.0 * np + 1.0 - sqrt((2.0 * np + 1.0) * (2.0 * np + 1.0) - 8.0 * e)) * 0.5);
-      while (P > 0 && tri_index(P, P, np) > e) --P;
-      while (P + 1 < np && tri_index(P + 1, P + 1, np) <= e) ++P;
-      const int Q = P + (e - tri_index(P, P, np));
+#pragma unroll
+    for (int k = 0; k < kBlkPer; ++k) {
+      if (tid + k * nt >= nblocks) break;
+      const int P = bP[k], Q = bQ[k];
       const int p0 = pp[P], p1 = qq[P], q0 = pp[Q], q1 = qq[Q];
       // one LDS access per entry: (i, j) with i > j is the conjugate of (j, i)
       auto get = [&](int i, int j) {

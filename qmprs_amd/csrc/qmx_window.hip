@@ -320,7 +320,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(WPE))) void
   const S* src = tside ? tsrc : phi;
   S* dst = tside ? tdst : phi;
   S* lds = tside ? sT : sP;
-  const bool store_dst = tside ? !first : true;
+  const bool store_dst = a.store && (tside ? !first : true);
   const int g0 = tside ? 0 : np;
   const int ngate = tside ? np : nc;
   const int lds_after = tside ? np : 1;
@@ -400,7 +400,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(WPE))) void
       acc_i = __builtin_amdgcn_mfma_f64_16x16x4f64(-tv.y, pv.x, acc_i, 0, 0, 0);
     }
     __syncthreads();
-    if (STAGED) {
+    if (STAGED && a.store) {
       // phi's final values (all window gates) replace its R operand in LDS,
       // then both tiles go back to HBM in tile order (T' holds its final
       // values since the group phase)
