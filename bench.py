@@ -9,9 +9,10 @@ sweeps, final fidelity) of one batch of such states on the GPU.
 Prints one JSON line (rank 0).  ``roofline`` describes the dominant kernel,
 the windowed sweep's streaming pass (window_pass_kernel): per launch it reads
 T' and phi and writes both back, 32 B per complex64 amplitude per state (24 B
-for the first window of a sweep, which reads the target and writes no T'),
-so algorithmic bytes per launch = B * 2^N * (32 (W - 1) + 24) / W averaged
-over the W windows of a sweep.  It is divided by the average launch duration,
+for the first window of a sweep, which reads the target and writes no T',
+16 B for the last, whose results are not read again), so algorithmic bytes
+per launch = B * 2^N * (32 (W - 2) + 24 + 16) / W averaged over the W
+windows of a sweep.  It is divided by the average launch duration,
 measured with HIP events recorded by the library on the stream the kernel is
 launched on, launch by launch, over the timed steps.  ``traffic`` is filled
 from rocprofv3 FETCH_SIZE / WRITE_SIZE counters when --traffic-kib gives them
@@ -278,7 +279,7 @@ def main() -> int:
     avg_launch_s = wp_ms / 1000.0 / max(wp_launches, 1)
     jobs = 2 if B >= 2 else 1
     windows = max(1, round(wp_launches / (args.steps * S * jobs)))
-    amp_bytes = (32.0 * (windows - 1) + 24.0) / windows
+    amp_bytes = 16.0 if windows == 1 else (32.0 * (windows - 2) + 24.0 + 16.0) / windows
     total_bytes = args.steps * S * windows * B * (2 ** n) * amp_bytes
     bytes_per_launch = total_bytes / max(wp_launches, 1)
     achieved = total_bytes / (wp_ms / 1000.0) / 1e9
