@@ -710,10 +710,11 @@ def gate_list_to_layers(gate_list, layers_template):
 def prepare_state(statevector, bond_dimension: int, num_layers: int = 1,
                   num_sweeps: int = 0, fidelity_threshold: float = 1 - 1e-6,
                   gauge: str = "canonical", layer_mode: str = "dense",
-                  cutoff: float = CUTOFF):
+                  cutoff: float = CUTOFF, polar: str = "lapack"):
     """Returns a dict with the unitary layers (circuit order), the flat gate
     list, the dense circuit state C|0..0>, the fidelity |<psi|C|0>| and the
-    chi-truncated target."""
+    chi-truncated target.  ``polar``: "lapack" (scipy) or "jacobi" (the
+    canonical completion of a rank-deficient environment)."""
     if not isinstance(num_layers, int) or num_layers < 1:
         raise ValueError("The number of layers must be a positive integer.")
     if not isinstance(bond_dimension, int) or bond_dimension < 1:
@@ -728,10 +729,80 @@ def prepare_state(statevector, bond_dimension: int, num_layers: int = 1,
     gl = layer_gate_list(layers)
     trace = []
     if num_sweeps > 0:
-        gl = optimize_gate_list(target, n, gl, num_sweeps, trace)
+        gl = optimize_gate_list(target, n, gl, num_sweeps, trace, polar)
         layers = gate_list_to_layers(gl, layers)
     out_state = circuit_state(n, gl)
     fid = abs(np.vdot(psi, out_state))
     return {"layers": layers, "gates": gl, "state": out_state, "fidelity": fid,
             "target": target, "spectra": spectra, "num_qubits": n,
             "overlap_trace": trace, "psi": psi}
+
+
+# --------------------------------------------------------------------------
+# Alternative schedules (notebook "State Preparation using MPS Sequential",
+# markdown cell 22; Rudolph et al., arXiv:2209.00595).  The reference runs
+# DallOall (all layers decomposed, then all optimised: prepare_state above)
+# and names two iterative variants without implementing them:
+#   IterDiOi   -- decompose layer i from the current residual, optimise that
+#                 layer alone against the residual, then peel it off;
+#   IterDiOall -- decompose layer i, optimise every layer so far against the
+#                 chi-MPS target, and recompute the residual C^dag |target>.
+# Parity unpinned against the reference (it has no code for them); pinned by
+# their reductions to DallOall (num_sweeps = 0, or num_layers = 1) and by the
+# monotone local updates of the sweep.
+# --------------------------------------------------------------------------
+
+SCHEDULES = ("DallOall", "IterDiOi", "IterDiOall")
+
+
+def schedule_layers(target: np.ndarray, num_sites: int, num_layers: int, num_sweeps: int,
+                    schedule: str, fidelity_threshold: float = 1 - 1e-6,
+                    cutoff: float = CUTOFF, polar: str = "lapack"):
+    """Layers (circuit order) of an iterative schedule on the chi-MPS state
+    ``target``; the early exit of sequential.py:384-391 is checked on the
+    residual after each layer."""
+    if schedule not in ("IterDiOi", "IterDiOall"):
+        raise ValueError(f"unknown schedule {schedule!r}")
+    unit = target / np.linalg.norm(target)
+    resid = unit.copy()
+    layers = []                      # extraction order
+    for _ in range(num_layers):
+        layer = generate_layer(bond2_mps(resid, num_sites, "canonical", cutoff))
+        if schedule == "IterDiOi":
+            gl = optimize_gate_list(resid, num_sites, layer_gate_list([layer]), num_sweeps,
+                                    polar=polar)
+            layer = gate_list_to_layers(gl, [layer])[0]
+            layers.append(layer)
+            resid = apply_inverse_layer_dense(resid, num_sites, layer)
+        else:
+            layers.append(layer)
+            circ = layers[::-1]
+            gl = optimize_gate_list(target, num_sites, layer_gate_list(circ), num_sweeps,
+                                    polar=polar)
+            layers = gate_list_to_layers(gl, circ)[::-1]
+            resid = unit.copy()
+            for lay in layers:
+                resid = apply_inverse_layer_dense(resid, num_sites, lay)
+        if np.isclose(np.conj(resid[0]), 1 + 0j, atol=1 - fidelity_threshold):
+            break
+    return layers[::-1]
+
+
+def prepare_state_schedule(statevector, bond_dimension: int, num_layers: int = 1,
+                           num_sweeps: int = 0, schedule: str = "DallOall",
+                           fidelity_threshold: float = 1 - 1e-6, cutoff: float = CUTOFF,
+                           polar: str = "lapack"):
+    """prepare_state under one of SCHEDULES; same result dictionary keys as
+    ``prepare_state`` (layers, gates, state, fidelity, target)."""
+    if schedule == "DallOall":
+        return prepare_state(statevector, bond_dimension, num_layers, num_sweeps,
+                             fidelity_threshold, cutoff=cutoff, polar=polar)
+    psi = ket(statevector)
+    n = int(np.log2(psi.size))
+    target, _ = chi_truncate(psi, n, bond_dimension, cutoff)
+    layers = schedule_layers(target, n, num_layers, num_sweeps, schedule, fidelity_threshold,
+                             cutoff, polar)
+    gl = layer_gate_list(layers)
+    state = circuit_state(n, gl)
+    return {"layers": layers, "gates": gl, "state": state, "fidelity": abs(np.vdot(psi, state)),
+            "target": target, "num_qubits": n, "psi": psi}

@@ -241,6 +241,81 @@ def encode_prepare(statevectors, bond_dimension: int, num_layers: int = 1,
     return Prepared(psi, target, gates, widths, used)
 
 
+SCHEDULES = ("DallOall", "IterDiOi", "IterDiOall")
+
+
+def schedule_layers(psi: torch.Tensor, target: torch.Tensor, num_layers: int, num_sweeps: int,
+                    schedule: str, fidelity_threshold: float = DEFAULT_THRESHOLD,
+                    cutoff: float = CUTOFF, sweep_dtype=torch.complex64):
+    """Iterative schedules named by the reference's notebook (markdown cell 22,
+    after Rudolph et al., arXiv:2209.00595), from the layer and sweep kernels:
+
+    IterDiOi   -- layer i from the current residual (qmx_generate_layer), swept
+                  alone against that residual (qmx_sweep), then peeled off
+                  (qmx_apply_layer, inverse);
+    IterDiOall -- layer i from the residual, then every layer so far swept
+                  against the chi-MPS target and the residual recomputed as
+                  C^dag |target> / |target|.
+    The early exit of sequential.py:384-391 is checked on each state's
+    residual after every layer; finished states keep their layers.
+    ``psi`` (normalised input) and ``target`` (chi-MPS state) are (B, 2^N)
+    complex128.  Returns (gates, widths, layers_used, fidelity |<psi|C|0>|)."""
+    if schedule not in ("IterDiOi", "IterDiOall"):
+        raise ValueError(f"unknown schedule {schedule!r}; expected one of {SCHEDULES}")
+    b, length = target.shape
+    n = num_qubits_of(length)
+    dev = target.device
+    unit = (target / torch.linalg.vector_norm(target, dim=1, keepdim=True)).contiguous()
+    resid = unit.clone()
+    gates = torch.zeros((b, num_layers, n, 4, 4), dtype=torch.complex128, device=dev)
+    widths = torch.ones((b, num_layers, n), dtype=torch.int32, device=dev)
+    used = torch.zeros(b, dtype=torch.int32, device=dev)
+    active = torch.ones(b, dtype=torch.bool, device=dev)
+    sweep_target = target.to(sweep_dtype).contiguous()
+    for i in range(num_layers):
+        g, w, _ = generate_layer(resid.clone(), cutoff)
+        if schedule == "IterDiOi":
+            lg, lw = g.unsqueeze(1).contiguous(), w.unsqueeze(1).contiguous()
+            if num_sweeps > 0:
+                one = active.to(torch.int32)
+                sweep(resid.to(sweep_dtype).contiguous(), lg, lw, one, num_sweeps)
+            g, w = lg[:, 0], lw[:, 0]
+        gates[:, i] = torch.where(active[:, None, None, None], g, gates[:, i])
+        widths[:, i] = torch.where(active[:, None], w, widths[:, i])
+        used = torch.where(active, torch.full_like(used, i + 1), used)
+        if schedule == "IterDiOi":
+            new = apply_layer(resid.clone(), g, w, inverse=True)
+        else:
+            if num_sweeps > 0:
+                sweep(sweep_target, gates, widths, torch.where(active, used, torch.zeros_like(used)),
+                      num_sweeps)
+            new = unit.clone()
+            for l in range(i + 1):
+                apply_layer(new, gates[:, l], widths[:, l], inverse=True)
+        resid = torch.where(active[:, None], new, resid).contiguous()
+        amp0 = resid[:, 0].conj()
+        active &= (amp0 - 1).abs() > (1 - fidelity_threshold) + 1e-5
+        if not bool(active.any()):
+            break
+    _, fid = circuit_state(gates, widths, used, psi_ref=psi.contiguous())
+    return gates, widths, used, fid.abs()
+
+
+def encode_schedule(statevectors, bond_dimension: int, num_layers: int = 1, num_sweeps: int = 0,
+                    schedule: str = "IterDiOall", fidelity_threshold: float = DEFAULT_THRESHOLD,
+                    cutoff: float = CUTOFF, sweep_dtype=torch.complex64):
+    """prepare_state of a batch under ``schedule`` (SCHEDULES); "DallOall" is
+    ``encode``.  Returns (gates, widths, layers_used, fidelity)."""
+    if schedule == "DallOall":
+        r = encode(statevectors, bond_dimension, num_layers, num_sweeps, fidelity_threshold, cutoff,
+                   sweep_dtype)
+        return r.gates, r.widths, r.layers_used, r.fidelity[:, -1].abs()
+    sv = as_device_states(statevectors)
+    psi, target, _ = chi_truncate(sv, bond_dimension, cutoff)
+    return schedule_layers(psi, target, num_layers, num_sweeps, schedule, fidelity_threshold, cutoff,
+                           sweep_dtype)
+
+
 @dataclass
 class RaggedResult:
     gates: torch.Tensor        # (L, N, 4, 4) complex128

@@ -25,7 +25,10 @@ class Sequential(MPSEncoder):
     Extra keyword arguments of ``prepare_mps`` / ``prepare_state`` beyond the
     reference's ``num_layers`` and ``num_sweeps``:
     ``sweep_dtype`` (torch.complex64 default, or torch.complex128) -- storage
-    of the sweep's dense states.
+    of the sweep's dense states;
+    ``schedule`` ("DallOall" default, the reference's method; "IterDiOi" or
+    "IterDiOall", the iterative variants its notebook names) -- see
+    ``qmprs_amd.engine.schedule_layers``.
     """
 
     def __init__(self, circuit_framework) -> None:
@@ -104,6 +107,20 @@ class Sequential(MPSEncoder):
                                         int(used[0].item()))
         return self._circuit_from_unitary_layers(mps, layers)
 
+    def _scheduled_circuit(self, mps: MPS, num_layers: int, num_sweeps: int, schedule: str,
+                           sweep_dtype=torch.complex64):
+        """IterDiOi / IterDiOall on the GPU (engine.schedule_layers)."""
+        psi = mps._psi.reshape(1, -1).to(torch.complex128).contiguous()
+        target = mps.state.reshape(1, -1).to(torch.complex128).contiguous()
+        gates, widths, used, fid = engine.schedule_layers(
+            psi, target, num_layers, num_sweeps, schedule, self.fidelity_threshold,
+            sweep_dtype=sweep_dtype)
+        self.last_info = {"fidelity": float(fid[0].item()), "layers_used": int(used[0].item()),
+                          "schedule": schedule}
+        layers = engine.gates_to_layers(gates[0].cpu().numpy(), widths[0].cpu().numpy(),
+                                        int(used[0].item()))
+        return self._circuit_from_unitary_layers(mps, layers)
+
     def prepare_mps(self, mps: MPS, **kwargs):
         """sequential.py:588-600."""
         num_layers = kwargs.get("num_layers", 1)
@@ -112,5 +129,10 @@ class Sequential(MPSEncoder):
             raise ValueError("The number of layers must be a positive integer.")
         if not isinstance(num_sweeps, int) or num_sweeps < 0:
             raise ValueError("The number of sweeps must be a non-negative integer.")
-        return self._sequential_unitary_circuit(mps, num_layers, num_sweeps,
-                                                kwargs.get("sweep_dtype", torch.complex64))
+        schedule = kwargs.get("schedule", "DallOall")
+        if schedule not in engine.SCHEDULES:
+            raise ValueError(f"schedule must be one of {engine.SCHEDULES}")
+        sweep_dtype = kwargs.get("sweep_dtype", torch.complex64)
+        if schedule != "DallOall":
+            return self._scheduled_circuit(mps, num_layers, num_sweeps, schedule, sweep_dtype)
+        return self._sequential_unitary_circuit(mps, num_layers, num_sweeps, sweep_dtype)

@@ -417,3 +417,58 @@ def test_circuit_state_large_n(gpu, n, layers, seed):
         err = np.abs(state[b].cpu().numpy() - ref).max()
         print(f"n={n} state {b} layers {used[b]} max err {err:.3e}")
         assert err < 1e-12
+
+
+@pytest.mark.parametrize("schedule", ["IterDiOi", "IterDiOall"])
+@pytest.mark.parametrize("n,chi,layers,sweeps,seed", [(6, 8, 3, 3, 0), (9, 16, 3, 2, 1)])
+def test_schedule_matches_oracle(gpu, schedule, n, chi, layers, sweeps, seed):
+    """The iterative schedules (layer / sweep / inverse-layer kernels driven by
+    engine.schedule_layers) against the oracle's restatement, complex128
+    sweeps: same layer count, gates to 1e-7, fidelity to 1e-9, C|0..0> to
+    1e-8 (L2)."""
+    eng = _engine()
+    sv = O.random_uniform_state(n, seed)
+    gates, widths, used, fid = eng.encode_schedule(sv, chi, layers, sweeps, schedule=schedule,
+                                                   sweep_dtype=torch.complex128)
+    torch.cuda.synchronize()
+    ref = O.prepare_state_schedule(sv, chi, layers, sweeps, schedule=schedule, polar="jacobi")
+    assert int(used[0]) == len(ref["layers"])
+    gl = _gate_list_from_device(gates[0].cpu().numpy(), widths[0].cpu().numpy(), int(used[0]))
+    gdiff = max(np.abs(a[2] - b[2]).max() for a, b in zip(gl, ref["gates"]))
+    state = O.circuit_state(n, gl)
+    serr = np.linalg.norm(state - ref["state"])
+    print(f"{schedule} n={n} fid {float(fid[0]):.12f} ref {ref['fidelity']:.12f} gate err {gdiff:.2e} "
+          f"state err {serr:.2e}")
+    assert gdiff < 1e-7
+    assert abs(float(fid[0]) - ref["fidelity"]) < 1e-9
+    assert serr < 1e-8
+
+
+def test_schedule_batch_early_exit_and_sequential_api(gpu):
+    """A batch where one state stops early (an exactly encodable product
+    state) keeps that state's layers while the other continues; the
+    Sequential API's schedule keyword returns a circuit whose state has the
+    reported fidelity."""
+    from qmprs_amd.circuit import GateCircuit
+    from qmprs_amd.synthesis.mps_encoding import Sequential
+    eng = _engine()
+    n = 6
+    prod = np.zeros(2 ** n, dtype=np.complex128)
+    prod[0] = 1.0
+    rnd = O.random_uniform_state(n, 7)
+    svs = np.stack([prod, rnd])
+    gates, widths, used, fid = eng.encode_schedule(svs, 8, 3, 2, schedule="IterDiOall",
+                                                   sweep_dtype=torch.complex128)
+    torch.cuda.synchronize()
+    for b, sv in enumerate(svs):
+        ref = O.prepare_state_schedule(sv, 8, 3, 2, schedule="IterDiOall", polar="jacobi")
+        assert int(used[b]) == len(ref["layers"])
+        assert abs(float(fid[b]) - ref["fidelity"]) < 1e-9
+    assert int(used[0]) == 1 and int(used[1]) == 3
+    enc = Sequential(circuit_framework=GateCircuit)
+    circ = enc.prepare_state(rnd, 8, num_layers=3, num_sweeps=2, schedule="IterDiOi",
+                             sweep_dtype=torch.complex128)
+    got = abs(np.vdot(O.ket(rnd), circ.get_statevector()))
+    assert abs(got - enc.last_info["fidelity"]) < 1e-9
+    with pytest.raises(ValueError):
+        enc.prepare_state(rnd, 8, num_layers=2, schedule="Brickwall")
