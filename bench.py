@@ -198,6 +198,82 @@ def ragged_main(args) -> int:
     return 0
 
 
+LARGE = {"num_qubits": 26, "bond_dimension": 512, "num_layers": 20, "num_sweeps": 50,
+         "generator_bond": 1024, "seed": 2026}
+
+
+def _large_state(dev):
+    """Seeded random MPS of bond 1024 at N=26 (complex Gaussian tensors drawn
+    on the host, contracted left to right in complex128 on the GPU),
+    normalised, cast to complex64: 2^26 amplitudes, 512 MB."""
+    import torch
+    lg = LARGE
+    n, bond = lg["num_qubits"], lg["generator_bond"]
+    gen = torch.Generator(device="cpu").manual_seed(lg["seed"])
+    x = None
+    for i in range(n):
+        dl = min(bond, 2 ** i, 2 ** (n - i))
+        dr = min(bond, 2 ** (i + 1), 2 ** (n - i - 1))
+        a = (torch.randn(dl, 2, dr, generator=gen, dtype=torch.float64)
+             + 1j * torch.randn(dl, 2, dr, generator=gen, dtype=torch.float64)) / np.sqrt(2.0)
+        a = a.to(dev).reshape(dl, 2 * dr)
+        x = a if x is None else x @ a
+        x = x.reshape(-1, dr)
+    psi = x.reshape(1, -1)
+    return (psi / torch.linalg.vector_norm(psi)).to(torch.complex64).contiguous()
+
+
+def large_main(args) -> int:
+    """One large state (N=26, chi=512, 20 layers, 50 sweeps) per step."""
+    import torch
+    from qmprs_amd import _lib, engine
+    lg = LARGE
+    n, chi, L, S = lg["num_qubits"], lg["bond_dimension"], lg["num_layers"], lg["num_sweeps"]
+    dev = _lib.require_gpu()
+    sv = _large_state(dev)
+    torch.cuda.synchronize()
+    for _ in range(args.warmup):
+        engine.encode(sv, chi, L, S)
+    torch.cuda.synchronize()
+    _lib.profile_enable(True)
+    _lib.profile_read()
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        res = engine.encode(sv, chi, L, S)
+    torch.cuda.synchronize()
+    total = time.perf_counter() - t0
+    prof = _lib.profile_read()
+    _lib.profile_enable(False)
+    wp_ms, wp_launches = prof["window_pass"]
+    windows = max(1, round(wp_launches / (args.steps * S)))
+    amp_bytes = 16.0 if windows == 1 else (32.0 * (windows - 2) + 24.0 + 16.0) / windows
+    total_bytes = args.steps * S * windows * (2 ** n) * amp_bytes
+    achieved = total_bytes / (wp_ms / 1000.0) / 1e9
+    fid = res.fidelity[0].abs().cpu().numpy()
+    line = {
+        "metric": "state-preps/sec, one large state (N=26, chi=512, 20 layers, 50 sweeps)",
+        "value": round(args.steps / total, 5), "unit": "state-preps/s", "n_gpus": 1, "steps": args.steps,
+        "warmup": args.warmup, "ms_per_step": round(1000 * total / args.steps, 1), "higher_is_better": True,
+        "scaling": "weak", "vs_baseline": None,
+        "dtype": "complex64 sweep storage, fp64 arithmetic and reductions",
+        "data": "synthetic: seeded random-MPS statevector (bond 1024, complex Gaussian)",
+        "config": {"workload": "large single state: N=26, bond_dimension=512, num_layers=20, num_sweeps=50, "
+                               "complex64 input", "batch": 1},
+        "fidelity": {"before_sweeps": float(fid[0]), "final": float(fid[-1]),
+                     "layers_used": int(res.layers_used[0])},
+        "phases_ms_per_step": {p: {"ms": round(v[0] / args.steps, 1), "launches": int(v[1] / args.steps)}
+                               for p, v in prof.items()},
+        "roofline": {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                     "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None,
+                     "kernel": "window_pass_kernel<float2, 4>",
+                     "avg_launch_us": round(1000.0 * wp_ms / max(wp_launches, 1), 2),
+                     "launches": int(wp_launches), "windows_per_sweep": windows, "states_per_launch": 1,
+                     "bytes_per_launch": int(total_bytes / max(wp_launches, 1))},
+    }
+    print(json.dumps(line))
+    return 0
+
+
 def main() -> int:
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -216,9 +292,10 @@ def main() -> int:
     ap.add_argument("--copy-calibration", action="store_true",
                     help="before the run, copy a 1 GiB buffer 3 times (a plain copy "
                          "kernel of known size for calibrating HBM counters)")
-    ap.add_argument("--workload", choices=("single-size", "ragged"), default="single-size",
+    ap.add_argument("--workload", choices=("single-size", "ragged", "large"), default="single-size",
                     help="single-size: the N=20, chi=64 batch (default); ragged: 256 states of "
-                         "mixed N and chi through the ragged scheduler")
+                         "mixed N and chi through the ragged scheduler; large: one N=26, chi=512 "
+                         "state, 20 layers, 50 sweeps (use --steps 1 --warmup 0)")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py runs on one GPU only: this encoder shards across GPUs by state "
@@ -230,6 +307,8 @@ def main() -> int:
     from qmprs_amd import _lib, engine
     if args.workload == "ragged":
         return ragged_main(args)
+    if args.workload == "large":
+        return large_main(args)
 
     wl = WORKLOAD
     n, chi, L, S = wl["num_qubits"], wl["bond_dimension"], wl["num_layers"], wl["num_sweeps"]

@@ -250,16 +250,24 @@ struct LayerLayout {
   int* error;
 };
 constexpr int kChain2MaxBlocks = 256;
-constexpr int kJacobiSweeps = 30;   // sweep launches of the LDS / grid Jacobi per bond
+constexpr int kJacobiSweeps = 30;   // sweep launches of the LDS Jacobi per bond (17-18 used at 2r = 128)
+// sweeps of the grid Jacobi (2r > 512): up to 34 were needed on the Gram
+// matrices of N=26 random-MPS states of bond 1024 at chi = 512
+constexpr int kJacobiGridSweeps = 80;
 constexpr int kWinMaxBlocks = 256;
+constexpr int kWinMaxBlocksLarge = 1024;
 
 // window-pass blocks per state: about win_blocks_per_cu resident blocks per CU
-// over the whole batch
+// over the whole batch.  Each block writes a 512-double partial; per state the
+// sweep layout holds max(2^(n-10) * 32, kWinMaxBlocks * 512) doubles, so large
+// states (few per batch) may use up to kWinMaxBlocksLarge blocks (more would
+// make the one-block finalize's partial sum the bottleneck).
 int window_blocks(const qmx_handle* h, int n, int batch) {
   const long ntiles = 1l << (n - std::min(n, kWinTileBits));
   long per = ((long)h->win_blocks_per_cu * h->cus + batch - 1) / batch;
   per = std::max(1l, std::min(per, ntiles));
-  return (int)std::min<long>(per, kWinMaxBlocks);
+  const long held = std::max<long>(kWinMaxBlocks, ((1l << (n - std::min(n, kSweepTileBits))) * 32) / 512);
+  return (int)std::min<long>(per, std::min<long>(held, kWinMaxBlocksLarge));
 }
 
 
@@ -485,7 +493,7 @@ int run_chi(qmx_handle* h, int n, int batch, int chi, double cutoff, const void*
     launch_gram_reduce(L.part, nchunks, L.ncur, nmax, L.a, nm2, batch, st);
     if (L.jgrid) {
       std::vector<int> conv(batch);
-      if (launch_jacobi_grid(L.a, L.v, nm2, L.ncur, nmax, batch, L.jgrid, conv.data(), kJacobiSweeps,
+      if (launch_jacobi_grid(L.a, L.v, nm2, L.ncur, nmax, batch, L.jgrid, conv.data(), kJacobiGridSweeps,
                              L.error, st) != 0)
         return fail(h, QMX_ERR_HIP, "grid Jacobi");
     } else if (L.jlog && jacobi_lds_setup(nmax) == hipSuccess) {
@@ -832,6 +840,13 @@ void job_window_finalize(qmx_handle* h, SweepJob<S>& J, const std::vector<Window
   fa.part = J.L.part;
   fa.overlap = J.overlap ? J.overlap + s : nullptr;
   fa.overlap_stride = J.nsweeps;
+  fa.pstride = 1;
+  if (fa.nblk > kWinMaxBlocks) {
+    // large grids (few large states): pre-sum groups of 32 partials with a
+    // grid of blocks, so the one-block finalize reads nblk / 32 of them
+    launch_window_reduce(J.L.part, fa.nblk, 32, J.batch, J.st);
+    fa.pstride = 32;
+  }
   launch_window_finalize(fa, J.batch, J.st);
 }
 

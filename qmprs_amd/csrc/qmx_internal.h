@@ -126,6 +126,7 @@ struct FwdArgs {
 
 struct WinFinArgs {
   int n, lmax, nblk;
+  int pstride;         // partial slots summed: 0, pstride, 2 pstride, ... (< nblk)
   int s0, s1;
   uint64_t umask;
   double2* gates;
@@ -233,6 +234,9 @@ void launch_chain2_step(const Chain2Args&, int nblk, int batch, hipStream_t);
 template <typename S> void launch_window_pass(const WinPassArgs<S>&, int nblk, int batch, hipStream_t);
 template <typename S> void launch_fwd_pass(const FwdArgs<S>&, int nblk, int batch, hipStream_t);
 void launch_window_finalize(const WinFinArgs&, int batch, hipStream_t);
+// pre-sums groups of `group` window-pass partial blocks into the first slot of
+// each group (large grids, before the one-block finalize)
+void launch_window_reduce(double* part, int nblk, int group, int batch, hipStream_t st);
 void launch_ran_gates(const RanArgs&, int batch, hipStream_t);
 void launch_amp0(const Amp0Args&, hipStream_t);
 

@@ -548,15 +548,18 @@ __global__ __launch_bounds__(256) void window_finalize_kernel(WinFinArgs a) {
   const int kb = a.layers[b] * n;
   if (a.s0 >= kb) return;
   const int nc = min(a.s1, kb) - a.s0;
-  // R = sum of the block partials in a fixed order
+  // R = sum of the block partials (slots 0, ps, 2 ps, ...) in a fixed order
   const double* pb = a.part + (size_t)b * a.nblk * 512;
+  const int ps = a.pstride > 0 ? a.pstride : 1;
+  const size_t st4 = (size_t)ps * 512;
   for (int e = threadIdx.x; e < 512; e += blockDim.x) {
     double s = 0.0;
     int qq = 0;
-    for (; qq + 3 < a.nblk; qq += 4)
-      s += ((pb[(size_t)qq * 512 + e] + pb[(size_t)(qq + 1) * 512 + e]) +
-            (pb[(size_t)(qq + 2) * 512 + e] + pb[(size_t)(qq + 3) * 512 + e]));
-    for (; qq < a.nblk; ++qq) s += pb[(size_t)qq * 512 + e];
+    for (; qq + 3 * ps < a.nblk; qq += 4 * ps) {
+      const double* q0 = pb + (size_t)qq * 512 + e;
+      s += ((q0[0] + q0[st4]) + (q0[2 * st4] + q0[3 * st4]));
+    }
+    for (; qq < a.nblk; qq += ps) s += pb[(size_t)qq * 512 + e];
     part_sum[e] = s;
   }
   if (threadIdx.x == 0) {
@@ -674,6 +677,27 @@ template void launch_window_pass<double2>(const WinPassArgs<double2>&, int, int,
 
 void launch_window_finalize(const WinFinArgs& a, int batch, hipStream_t st) {
   hipLaunchKernelGGL(window_finalize_kernel, dim3(batch), dim3(256), 0, st, a);
+}
+
+// block k of state b sums partial slots [k group, min((k + 1) group, nblk))
+// into slot k group (each thread owns elements e, e + 256 of every slot)
+__global__ __launch_bounds__(256) void window_reduce_kernel(double* part, int nblk, int group) {
+  double* pb = part + (size_t)blockIdx.y * nblk * 512;
+  const int q0 = blockIdx.x * group, q1 = min(q0 + group, nblk);
+  for (int e = threadIdx.x; e < 512; e += blockDim.x) {
+    double s = 0.0;
+    int q = q0;
+    for (; q + 3 < q1; q += 4)
+      s += ((pb[(size_t)q * 512 + e] + pb[(size_t)(q + 1) * 512 + e]) +
+            (pb[(size_t)(q + 2) * 512 + e] + pb[(size_t)(q + 3) * 512 + e]));
+    for (; q < q1; ++q) s += pb[(size_t)q * 512 + e];
+    pb[(size_t)q0 * 512 + e] = s;
+  }
+}
+
+void launch_window_reduce(double* part, int nblk, int group, int batch, hipStream_t st) {
+  hipLaunchKernelGGL(window_reduce_kernel, dim3((nblk + group - 1) / group, batch), dim3(256), 0, st, part,
+                     nblk, group);
 }
 
 }  // namespace qmx

@@ -472,3 +472,30 @@ def test_schedule_batch_early_exit_and_sequential_api(gpu):
     assert abs(got - enc.last_info["fidelity"]) < 1e-9
     with pytest.raises(ValueError):
         enc.prepare_state(rnd, 8, num_layers=2, schedule="Brickwall")
+
+
+def test_large_grid_window_sweep_equals_step_sweep(gpu):
+    """N=23, one state: the window pass runs on 512 blocks and the finalize
+    reads the partial environments pre-summed by groups of 32; the result is
+    the gate-by-gate sweep's (complex128)."""
+    eng = _engine()
+    from qmprs_amd import _lib
+    n = 23
+    sv = O.random_mps_state(n, 16, 11)
+    prep = eng.encode_prepare(sv, 16, 2, sweep_dtype=torch.complex128)
+    out = {}
+    try:
+        for mode in ("step", "window"):
+            _lib.set_sweep_mode(mode)
+            gates = prep.gates.clone()
+            res = eng.sweep(prep.target, gates, prep.widths, prep.layers_used, 1, psi_ref=prep.psi,
+                            circuit_out=True)
+            torch.cuda.synchronize()
+            out[mode] = (gates, res.circuit[0], res.fidelity[0].abs())
+    finally:
+        _lib.set_sweep_mode("auto")
+    gd = (out["step"][0] - out["window"][0]).abs().max().item()
+    sd = torch.linalg.vector_norm(out["step"][1] - out["window"][1]).item()
+    fd = (out["step"][2] - out["window"][2]).abs().max().item()
+    print(f"n={n}: gate diff {gd:.3e} state diff {sd:.3e} fidelity diff {fd:.3e}")
+    assert gd < 1e-10 and sd < 1e-10 and fd < 1e-12
