@@ -499,3 +499,23 @@ def test_large_grid_window_sweep_equals_step_sweep(gpu):
     fd = (out["step"][2] - out["window"][2]).abs().max().item()
     print(f"n={n}: gate diff {gd:.3e} state diff {sd:.3e} fidelity diff {fd:.3e}")
     assert gd < 1e-10 and sd < 1e-10 and fd < 1e-12
+
+
+@pytest.mark.parametrize("n,chi", [(2, 2), (3, 2), (4, 4), (5, 4), (9, 32)])
+def test_encode_small_sizes(gpu, n, chi):
+    """The smallest states (gate-by-gate sweeps and tile-sized forward passes
+    below 5 qubits; light-cone rebuilds covering the whole state from 5 to
+    12) end to end against the oracle, complex128 sweeps."""
+    eng = _engine()
+    sv = O.random_uniform_state(n, 40 + n)
+    res = eng.encode(sv, chi, 3, 2, sweep_dtype=torch.complex128)
+    torch.cuda.synchronize()
+    ref = O.prepare_state(sv, chi, num_layers=3, num_sweeps=2, polar="jacobi")
+    assert int(res.layers_used[0]) == len(ref["layers"])
+    fid = abs(res.fidelity[0, -1].item())
+    gl = _gate_list_from_device(res.gates[0].cpu().numpy(), res.widths[0].cpu().numpy(),
+                                int(res.layers_used[0]))
+    serr = np.linalg.norm(O.circuit_state(n, gl) - ref["state"])
+    print(f"n={n} chi={chi} fid {fid:.12f} ref {ref['fidelity']:.12f} state err {serr:.2e}")
+    assert abs(fid - ref["fidelity"]) < 1e-9
+    assert serr < 1e-8
