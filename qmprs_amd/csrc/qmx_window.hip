@@ -491,26 +491,42 @@ __global__ __launch_bounds__(256) void fwd_pass_kernel(FwdArgs<S> a) {
   const long ntiles = 1l << __popcll(a.tmask);
   const int ngroups = 1 << (t - 4);
   const uint32_t dead_in = (uint32_t)(~a.live_in & all), dead_out = (uint32_t)(~a.live_out & all);
+  // One group per thread (ngroups <= 256 = blockDim for tiles of <= 2^12
+  // amplitudes).  Stage 0 always uses the same group map, so the next tile's
+  // stage-0 loads are issued as soon as this tile's are consumed and stay in
+  // flight while this tile's stages run.
+  const bool has = (int)threadIdx.x < ngroups;
+  const GroupMap& g0m = a.gs[0];
+  const uint32_t roffg0 = has ? group_roffg(g0m, threadIdx.x) : 0u;
+  c2 pre[16];
+  auto load0 = [&](uint32_t bse) {
+#pragma unroll
+    for (int x = 0; x < 16; ++x) {
+      const uint32_t ad = bse + roffg0 + xoffs(g0m.ubit, x);
+      if (ad & dead_in)
+        pre[x] = T::mk(make_double2(0.0, 0.0));
+      else if (a.from_basis0)
+        pre[x] = T::mk(make_double2(ad == 0u ? 1.0 : 0.0, 0.0));
+      else
+        pre[x] = T::mk(to_c(psi[ad]));
+    }
+  };
+  if (has && (long)blockIdx.x < ntiles) load0((uint32_t)wdeposit((uint64_t)blockIdx.x, a.tmask));
   for (long tile = blockIdx.x; tile < ntiles; tile += gridDim.x) {
     const uint32_t base = (uint32_t)wdeposit((uint64_t)tile, a.tmask);
     for (int st = 0; st < a.nstage; ++st) {
       const GroupMap& gm = a.gs[st];
       const bool last = (st + 1 == a.nstage);
-      for (int grp = threadIdx.x; grp < ngroups; grp += blockDim.x) {
-        const uint32_t roffg = group_roffg(gm, grp);
+      if (has) {
+        const int grp = threadIdx.x;
+        const uint32_t roffg = st == 0 ? roffg0 : group_roffg(gm, grp);
         const int roffl = group_roffl(gm, grp);
         c2 v[16];
         if (st == 0) {
 #pragma unroll
-          for (int x = 0; x < 16; ++x) {
-            const uint32_t ad = base + roffg + xoffs(gm.ubit, x);
-            if (ad & dead_in)
-              v[x] = T::mk(make_double2(0.0, 0.0));
-            else if (a.from_basis0)
-              v[x] = T::mk(make_double2(ad == 0u ? 1.0 : 0.0, 0.0));
-            else
-              v[x] = T::mk(to_c(psi[ad]));
-          }
+          for (int x = 0; x < 16; ++x) v[x] = pre[x];
+          const long nxt = tile + gridDim.x;
+          if (nxt < ntiles) load0((uint32_t)wdeposit((uint64_t)nxt, a.tmask));
         } else {
 #pragma unroll
           for (int x = 0; x < 16; ++x) v[x] = T::mk(to_c(lds[swz(roffl + (int)xoffs(gm.ux, x))]));
