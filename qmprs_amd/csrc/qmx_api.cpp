@@ -108,14 +108,14 @@ struct PhaseTimer {
   PhaseTimer(qmx_handle* h_, hipStream_t st_, int phase_) : h(h_), st(st_), phase(phase_) {
     if (h->profiling) {
       start = pool_event(h);
-      if (start) hipEventRecord(start, st);
+      if (start) (void)hipEventRecord(start, st);
     }
   }
   void stop(long long launches) {
     if (!start) return;
     hipEvent_t end = pool_event(h);
     if (!end) return;
-    hipEventRecord(end, st);
+    (void)hipEventRecord(end, st);
     h->recs.push_back({phase, start, end, launches});
     start = nullptr;
   }
@@ -607,12 +607,6 @@ std::vector<std::vector<int>> inverse_layer_passes(int n) {
   return plan_passes(n, pos, sites, kApplyMaxUnion);
 }
 
-std::vector<std::vector<int>> forward_layer_passes(int n) {
-  std::vector<int> pos, sites;
-  for (int i = 0; i < n; ++i) { pos.push_back(i); sites.push_back(i); }
-  return plan_passes(n, pos, sites, kApplyMaxUnion);
-}
-
 std::vector<std::vector<int>> forward_circuit_passes(int n, int lmax) {
   std::vector<int> pos, sites;
   for (int f = 0; f < lmax * n; ++f) { pos.push_back(f); sites.push_back(f % n); }
@@ -1090,14 +1084,17 @@ int qmx_create(int device, qmx_handle** out) {
 
 int qmx_destroy(qmx_handle* h) {
   if (!h) return QMX_OK;
-  for (auto& r : h->recs) { hipEventDestroy(r.start); hipEventDestroy(r.end); }
-  for (auto e : h->pool) hipEventDestroy(e);
-  for (int i = 0; i < 2; ++i) {
-    if (h->aux[i]) hipStreamDestroy(h->aux[i]);
-    if (h->pass_ev[i]) hipEventDestroy(h->pass_ev[i]);
-    if (h->join_ev[i]) hipEventDestroy(h->join_ev[i]);
+  for (auto& r : h->recs) {
+    (void)hipEventDestroy(r.start);
+    (void)hipEventDestroy(r.end);
   }
-  if (h->call_ev) hipEventDestroy(h->call_ev);
+  for (auto e : h->pool) (void)hipEventDestroy(e);
+  for (int i = 0; i < 2; ++i) {
+    if (h->aux[i]) (void)hipStreamDestroy(h->aux[i]);
+    if (h->pass_ev[i]) (void)hipEventDestroy(h->pass_ev[i]);
+    if (h->join_ev[i]) (void)hipEventDestroy(h->join_ev[i]);
+  }
+  if (h->call_ev) (void)hipEventDestroy(h->call_ev);
   delete h;
   return QMX_OK;
 }

@@ -520,7 +520,6 @@ __global__ __launch_bounds__(512) void select_kernel(const double2* a_all, const
                                                      double* spectra, size_t spec_stride) {
   __shared__ double lam[2 * kJacobiMaxPairs];
   __shared__ int perm[2 * kJacobiMaxPairs];
-  __shared__ int s_k;
   const int b = blockIdx.x;
   const int n = ncols_dev[b];
   const double2* a = a_all + (size_t)b * stride;
@@ -555,7 +554,6 @@ __global__ __launch_bounds__(512) void select_kernel(const double2* a_all, const
       if (k > chi) k = chi;
       if (k < 1) k = 1;
     }
-    s_k = k;
     k_out[(size_t)b * kstride] = k;
   }
   __syncthreads();
