@@ -550,7 +550,11 @@ __global__ __launch_bounds__(256) void fwd_pass_kernel(FwdArgs<S> a) {
 }
 
 // Replay the window on the 16-dim space of U (one block per state).
+// It runs beside the other half-batch's window pass, whose waves fill the
+// SIMDs; its own waves take the highest issue priority so that this short,
+// latency-bound chain is not slowed down by them.
 __global__ __launch_bounds__(256) void window_finalize_kernel(WinFinArgs a) {
+  __builtin_amdgcn_s_setprio(3);
   __shared__ double2 q[256];
   __shared__ double2 qn[256];
   __shared__ double2 gm[kMaxWindow][16];
@@ -564,33 +568,42 @@ __global__ __launch_bounds__(256) void window_finalize_kernel(WinFinArgs a) {
   const int kb = a.layers[b] * n;
   if (a.s0 >= kb) return;
   const int nc = min(a.s1, kb) - a.s0;
+  // the window's gates, one entry per thread (nc * 16 <= 128), and the
+  // positions of each gate's bits inside U (rank among U's bits); issued
+  // before the partial sum so that both latencies overlap
+  if ((int)threadIdx.x < nc * 16) {
+    const int j = threadIdx.x >> 4, e = threadIdx.x & 15;
+    int layer, site;
+    step_gate(n, a.s0 + j, layer, site);
+    const size_t gi = ((size_t)b * a.lmax + layer) * n + site;
+    gm[j][e] = a.gates[gi * 16 + e];
+    if (e == 0) {
+      const int w = wsane(n, site, a.widths[gi]);
+      gw[j] = w;
+      const int hb = n - 1 - site;
+      gh[j] = __popcll(a.umask & ((1ull << hb) - 1ull));
+      gl[j] = w == 2 ? __popcll(a.umask & ((1ull << (hb - 1)) - 1ull)) : 0;
+    }
+  }
   // R = sum of the block partials (slots 0, ps, 2 ps, ...) in a fixed order
   const double* pb = a.part + (size_t)b * a.nblk * 512;
   const int ps = a.pstride > 0 ? a.pstride : 1;
   const size_t st4 = (size_t)ps * 512;
+  // (8 partials per step, all loads issued before the adds: the partials
+  // come from L2 / HBM and the sum is latency-bound)
   for (int e = threadIdx.x; e < 512; e += blockDim.x) {
     double s = 0.0;
     int qq = 0;
-    for (; qq + 3 * ps < a.nblk; qq += 4 * ps) {
+#pragma unroll 2
+    for (; qq + 7 * ps < a.nblk; qq += 8 * ps) {
       const double* q0 = pb + (size_t)qq * 512 + e;
-      s += ((q0[0] + q0[st4]) + (q0[2 * st4] + q0[3 * st4]));
+      double v[8];
+#pragma unroll
+      for (int u = 0; u < 8; ++u) v[u] = q0[u * st4];
+      s += (((v[0] + v[1]) + (v[2] + v[3])) + ((v[4] + v[5]) + (v[6] + v[7])));
     }
     for (; qq < a.nblk; qq += ps) s += pb[(size_t)qq * 512 + e];
     part_sum[e] = s;
-  }
-  if (threadIdx.x == 0) {
-    // positions of each gate's bits inside U (rank among U's bits)
-    for (int j = 0; j < nc; ++j) {
-      int layer, site;
-      step_gate(n, a.s0 + j, layer, site);
-      size_t gi = ((size_t)b * a.lmax + layer) * n + site;
-      int w = wsane(n, site, a.widths[gi]);
-      gw[j] = w;
-      int hb = n - 1 - site;
-      gh[j] = __popcll(a.umask & ((1ull << hb) - 1ull));
-      gl[j] = w == 2 ? __popcll(a.umask & ((1ull << (hb - 1)) - 1ull)) : 0;
-      for (int e = 0; e < 16; ++e) gm[j][e] = a.gates[gi * 16 + e];
-    }
   }
   __syncthreads();
   const int x = threadIdx.x >> 4, y = threadIdx.x & 15;
