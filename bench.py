@@ -37,7 +37,7 @@ HBM_PEAK_GBS = 8000.0
 # from rocprofv3 --pmc FETCH_SIZE and WRITE_SIZE runs (FETCH_SIZE x 2 and
 # WRITE_SIZE x 1, the factors a plain 1 GiB copy reads on the same counters):
 # states per launch -> (read KiB, write KiB).  Summaries: profiles/window_pass_n20_*.md.
-MEASURED_TRAFFIC_KIB = {32: (524975.2, 528713.0)}
+MEASURED_TRAFFIC_KIB = {32: (524977.0, 528714.0)}
 WORKLOAD = {"num_qubits": 20, "bond_dimension": 64, "num_layers": 10, "num_sweeps": 30,
             "generator_bond": 96}
 
