@@ -36,10 +36,8 @@ struct qmx_handle {
   int fwd_blocks_per_cu = 4;            // QMX_OPT_FWD_BLOCKS_PER_CU
   int staged_windows = 1;               // QMX_OPT_STAGED_WINDOWS (0, 1, 2)
   hipStream_t aux[2] = {nullptr, nullptr};   // the two half-batch jobs
-  hipStream_t pass_stream = nullptr;         // their window passes, interleaved
   hipEvent_t pass_ev[2] = {nullptr, nullptr};
-  hipEvent_t fin_ev[2] = {nullptr, nullptr};
-  hipEvent_t join_ev[3] = {nullptr, nullptr, nullptr};
+  hipEvent_t join_ev[2] = {nullptr, nullptr};
   hipEvent_t call_ev = nullptr;
 };
 
@@ -718,7 +716,6 @@ struct SweepJob {
   S* circuit_out;
   SweepLayout<S> L;
   hipStream_t st;
-  hipStream_t pass_st = nullptr;   // stream of the window passes (st when null)
   const int* skip = nullptr;
   bool use_goal = false;
   bool active = true;
@@ -814,9 +811,8 @@ void job_window_pass(qmx_handle* h, SweepJob<S>& J, const std::vector<Window>& w
                                    __builtin_ctzll(pa.gp.restg | (1ull << 63)));
   const int min_run = h->staged_windows == 2 ? 3 : 7;
   pa.staged = (h->staged_windows && stride_bits >= 2 && low_run >= min_run && pa.t - low_run <= 8) ? 1 : 0;
-  const hipStream_t ps = J.pass_st ? J.pass_st : J.st;
-  PhaseTimer pt(h, ps, QMX_PHASE_WINDOW_PASS);
-  launch_window_pass(pa, wblk, J.batch, ps);
+  PhaseTimer pt(h, J.st, QMX_PHASE_WINDOW_PASS);
+  launch_window_pass(pa, wblk, J.batch, J.st);
   pt.stop(1);
 }
 
@@ -907,33 +903,17 @@ int run_sweep_jobs(qmx_handle* h, std::vector<SweepJob<S>>& jobs) {
     }
     if (!any) break;
     PhaseTimer swt(h, s0, QMX_PHASE_SWEEP);
-    if (windowed && nj > 1) {
-      // All passes go, interleaved job by job, into one stream, so they run
-      // back to back with no cross-stream hand-off between them; each job's
-      // finalize runs on the job's own stream under the other job's pass,
-      // and the job's next pass waits for it (it reads the updated gates,
-      // and the finalize reads the partials the next pass overwrites).
-      const hipStream_t ps = jobs[0].pass_st;
-      for (size_t j = 0; j < nj; ++j) QMX_HIP(h, hipEventRecord(h->fin_ev[j], jobs[j].st));
+    if (windowed) {
       for (size_t wi = 0; wi < wins.size(); ++wi) {
         for (size_t j = 0; j < nj; ++j) {
           SweepJob<S>& J = jobs[j];
           if (!J.active) continue;
-          QMX_HIP(h, hipStreamWaitEvent(ps, h->fin_ev[j], 0));
+          // passes of the two jobs never overlap: wait for the other job's last pass
+          if (nj > 1) QMX_HIP(h, hipStreamWaitEvent(J.st, h->pass_ev[1 - j], 0));
           job_window_pass(h, J, wins, wi);
-          QMX_HIP(h, hipEventRecord(h->pass_ev[j], ps));
-          QMX_HIP(h, hipStreamWaitEvent(J.st, h->pass_ev[j], 0));
+          if (nj > 1) QMX_HIP(h, hipEventRecord(h->pass_ev[j], J.st));
           job_window_finalize(h, J, wins, wi, s);
-          QMX_HIP(h, hipEventRecord(h->fin_ev[j], J.st));
         }
-      }
-      QMX_LAUNCHED(h, "windowed sweep");
-    } else if (windowed) {
-      for (size_t wi = 0; wi < wins.size(); ++wi) {
-        SweepJob<S>& J = jobs[0];
-        if (!J.active) continue;
-        job_window_pass(h, J, wins, wi);
-        job_window_finalize(h, J, wins, wi, s);
       }
       QMX_LAUNCHED(h, "windowed sweep");
     } else {
@@ -992,24 +972,19 @@ int sweep_dispatch(qmx_handle* h, int n, int batch, int lmax, const int* layers_
   for (int i = 0; i < 2; ++i) {
     if (!h->aux[i]) QMX_HIP(h, hipStreamCreateWithFlags(&h->aux[i], hipStreamNonBlocking));
     if (!h->pass_ev[i]) QMX_HIP(h, hipEventCreateWithFlags(&h->pass_ev[i], hipEventDisableTiming));
-    if (!h->fin_ev[i]) QMX_HIP(h, hipEventCreateWithFlags(&h->fin_ev[i], hipEventDisableTiming));
   }
-  for (int i = 0; i < 3; ++i)
+  for (int i = 0; i < 2; ++i)
     if (!h->join_ev[i]) QMX_HIP(h, hipEventCreateWithFlags(&h->join_ev[i], hipEventDisableTiming));
-  if (!h->pass_stream) QMX_HIP(h, hipStreamCreateWithFlags(&h->pass_stream, hipStreamNonBlocking));
   if (!h->call_ev) QMX_HIP(h, hipEventCreateWithFlags(&h->call_ev, hipEventDisableTiming));
   const int b1 = (batch + 1) / 2;
   jobs.push_back(job(0, b1, h->aux[0]));
   jobs.push_back(job(b1, batch - b1, h->aux[1]));
-  for (auto& J : jobs) J.pass_st = h->pass_stream;
   // the auxiliary streams start after the caller's earlier work ...
   QMX_HIP(h, hipEventRecord(h->call_ev, st));
   for (int i = 0; i < 2; ++i) QMX_HIP(h, hipStreamWaitEvent(h->aux[i], h->call_ev, 0));
-  QMX_HIP(h, hipStreamWaitEvent(h->pass_stream, h->call_ev, 0));
   const int rc = run_sweep_jobs<S>(h, jobs);
   // ... and the caller's later work waits for both
   for (int i = 0; i < 2; ++i) QMX_CALL(stream_join(h, h->join_ev[i], h->aux[i], st));
-  QMX_CALL(stream_join(h, h->join_ev[2], h->pass_stream, st));
   return rc;
 }
 
@@ -1118,11 +1093,9 @@ int qmx_destroy(qmx_handle* h) {
   for (int i = 0; i < 2; ++i) {
     if (h->aux[i]) (void)hipStreamDestroy(h->aux[i]);
     if (h->pass_ev[i]) (void)hipEventDestroy(h->pass_ev[i]);
-    if (h->fin_ev[i]) (void)hipEventDestroy(h->fin_ev[i]);
   }
-  for (int i = 0; i < 3; ++i)
+  for (int i = 0; i < 2; ++i)
     if (h->join_ev[i]) (void)hipEventDestroy(h->join_ev[i]);
-  if (h->pass_stream) (void)hipStreamDestroy(h->pass_stream);
   if (h->call_ev) (void)hipEventDestroy(h->call_ev);
   delete h;
   return QMX_OK;
