@@ -254,7 +254,7 @@ int qmx_encode_ragged(qmx_handle* h, int count, const qmx_ragged_state* states, 
  * row per two-qubit gate slot to 15 su(4) coefficients c; the proposal is
  * G exp(i sum_k c_k l_k) (Gell-Mann basis, tr(l_a l_b) = 2 delta_ab).  It
  * proposes starting gates for the sweeps of Sequential._optimize_unitary_layers
- * (sequential.py:509-541); the plain sweeps remain the fallback and the judge.
+ * (sequential.py:509-541); the plain sweeps remain the fallback and decide the final fidelity.
  *
  * Parameters (FP32 master, BF16 shadow with the same offsets):
  *   W1[in][width], b1[width], W2[width][width], b2[width], W3[width][out], b3[out]

@@ -4,7 +4,7 @@
 // coefficients c (Gell-Mann basis, tr(l_a l_b) = 2 delta_ab); the proposed
 // gate is G exp(i sum_k c_k l_k).  It has no counterpart in the reference: it
 // proposes warm-start gates for the environment sweeps (sequential.py:
-// 509-541), which stay the judge of fidelity.
+// 509-541), which decide the final fidelity.
 //
 // GEMMs run on v_mfma_f32_16x16x16bf16_1k (BF16 operands, FP32 accumulation):
 //   A[i][k] lane i + 16 (k / 4), element k % 4

@@ -14,7 +14,7 @@ training and inference all run on the GPU through libqmx:
   (``qmx_mlp_forward`` / ``qmx_mlp_backward`` on MFMA, ``qmx_mlp_adam``);
 * at inference the proposal G1 exp(i sum c_k l_k) replaces G1 for a state
   only if its fidelity is at least that of G1; otherwise the state keeps the
-  plain gates.  The sweeps then continue as usual and stay the judge.
+  plain gates.  The sweeps then continue as usual and decide the final fidelity.
 """
 
 from __future__ import annotations
