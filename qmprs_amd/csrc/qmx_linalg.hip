@@ -181,8 +181,10 @@ __global__ __launch_bounds__(512) void jacobi_kernel(double2* a_all, double2* v_
 // the same parallel (circle) ordering and rotations as jacobi_kernel.  Each
 // sweep logs its rotations (c, s, e^{-i phi} per pair and round); a second
 // kernel applies the log to the rows of V, 16 rows per block held in LDS.
-// The host launches sweep / update pairs; a state whose sweep rotated nothing
-// is converged and every later launch returns at once for it.
+// The host launches sweep / update pairs; a state none of whose off-diagonal
+// entries passes the rotation test at the start of a sweep is converged (that
+// sweep would rotate nothing and is not run), as is one whose sweep rotated
+// nothing, and every later launch returns at once for it.
 constexpr int kJacobiLdsMax = 128;
 constexpr int kJacobiRowsPerBlock = 16;
 
@@ -227,6 +229,27 @@ __global__ __launch_bounds__(1024) void jacobi_lds_sweep_kernel(double2* a_all, 
   __syncthreads();
   const int np = n / 2;
   const double abs_tol = 1e-17 * s_scale + 1e-300;
+  // a sweep would rotate nothing iff no off-diagonal entry passes the rotation
+  // test now: then the state is converged and the sweep is not run
+  for (int e = tid; e < ntri; e += nt) {
+    int i = (int)((2.0 * n + 1.0 - sqrt((2.0 * n + 1.0) * (2.0 * n + 1.0) - 8.0 * e)) * 0.5);
+    while (i > 0 && tri_index(i, i, n) > e) --i;
+    while (i + 1 < n && tri_index(i + 1, i + 1, n) <= e) ++i;
+    const int j = i + (e - tri_index(i, i, n));
+    if (j == i) continue;
+    const double ag = cabs_(tri[e]);
+    if (ag > abs_tol && ag > 1e-15 * sqrt(fabs(tri[tri_index(i, i, n)].x * tri[tri_index(j, j, n)].x)))
+      any_rot = 1;
+  }
+  __syncthreads();
+  const bool go = any_rot;
+  __syncthreads();
+  if (!go) {
+    if (tid == 0) state[b] = 1;
+    return;
+  }
+  if (tid == 0) any_rot = 0;
+  __syncthreads();
   // this thread's pair blocks (P, Q), P <= Q, row-major over the upper
   // triangle of pair blocks; the same in every round (the kernel is
   // launched with 1024 threads: kBlkPer * 1024 covers every block)
@@ -391,7 +414,8 @@ __global__ void jacobi_lds_check_kernel(const int* state, int batch, int* error)
 // input (one kernel, no separate parameter pass); the thread of each diagonal
 // 2x2 block logs its pair's rotation.  After the n - 1 rounds of a sweep one
 // kernel applies the logged rotations to the rows of V held in LDS, in round
-// order.  A sweep that rotates nothing marks its state converged, its
+// order.  After each sweep a state none of whose off-diagonal entries passes
+// the rotation test is converged (its next sweep would rotate nothing), its
 // eigenvalues (the diagonal) are copied into the first buffer if the last round
 // wrote the second, and later launches return at once for it.
 
@@ -492,7 +516,7 @@ __global__ __launch_bounds__(256) void jgrid_round_kernel(const double2* ain_all
                                                           const double* diag_in, const double2* off_in,
                                                           double* diag_out, double2* off_out,
                                                           double4* log_all, size_t log_stride, int max_pairs,
-                                                          const double* scale, const int* conv, int* rot) {
+                                                          const double* scale, const int* conv) {
   const int b = blockIdx.y;
   if (conv[b]) return;
   const int n = ncols_dev[b], np = n / 2;
@@ -505,7 +529,6 @@ __global__ __launch_bounds__(256) void jgrid_round_kernel(const double2* ain_all
   const double tol = 1e-17 * scale[b] + 1e-300;
   int pr[2][2];
   double4 R[2];
-  bool rotated = false;
 #pragma unroll
   for (int k = 0; k < 2; ++k) {
     const int m = k ? Q : P;
@@ -513,12 +536,8 @@ __global__ __launch_bounds__(256) void jgrid_round_kernel(const double2* ain_all
     bool rk;
     R[k] = jgrid_rotation(diag_in[(size_t)b * ld + pr[k][0]], diag_in[(size_t)b * ld + pr[k][1]],
                           off_in[(size_t)b * max_pairs + m], tol, rk);
-    if (k == 0) rotated = rk;
   }
-  if (P == Q) {
-    log_all[(size_t)b * log_stride + (size_t)r * max_pairs + P] = R[0];
-    if (rotated) rot[b] = 1;
-  }
+  if (P == Q) log_all[(size_t)b * log_stride + (size_t)r * max_pairs + P] = R[0];
   const int p0 = pr[0][0], p1 = pr[0][1], q0 = pr[1][0], q1 = pr[1][1];
   const double2 J00 = make_double2(R[1].x, 0.0), J01 = make_double2(R[1].y, 0.0);
   const double2 J10 = cscale(make_double2(R[1].z, R[1].w), -R[1].y);
@@ -618,9 +637,31 @@ __global__ __launch_bounds__(1024) void jgrid_rows_kernel(double2* v_all, size_t
   for (int e = threadIdx.x; e < nr * n; e += blockDim.x) v[(size_t)(r0 + e / n) * ld + e % n] = vrows[e];
 }
 
-// End of a sweep (final = 0): a state that rotated nothing is converged; its
-// diagonal moves to the first buffer when the last round wrote the second.
-// After the last sweep (final = 1): the same copy for the states still running.
+// rot[b] = 1 if an off-diagonal entry of the current A passes the rotation
+// test (the next sweep would rotate); rot is clear on entry
+__global__ __launch_bounds__(256) void jgrid_check_kernel(const double2* acur_all, size_t stride,
+                                                          const int* ncols_dev, int ld, const double* scale,
+                                                          const int* conv, int* rot) {
+  const int b = blockIdx.y;
+  if (conv[b]) return;
+  const int n = ncols_dev[b];
+  const double2* a = acur_all + (size_t)b * stride;
+  const double tol = 1e-17 * scale[b] + 1e-300;
+  bool pass = false;
+  for (size_t e = blockIdx.x * (size_t)blockDim.x + threadIdx.x; e < (size_t)n * n;
+       e += (size_t)gridDim.x * blockDim.x) {
+    const int i = (int)(e / n), j = (int)(e % n);
+    if (j <= i) continue;
+    const double ag = cabs_(a[(size_t)i * ld + j]);
+    if (ag > tol && ag > 1e-15 * sqrt(fabs(a[(size_t)i * ld + i].x * a[(size_t)j * ld + j].x))) pass = true;
+  }
+  if (pass) rot[b] = 1;
+}
+
+// End of a sweep (final = 0): a state whose A passed no rotation test is
+// converged (its next sweep would rotate nothing and is not run); its diagonal
+// moves to the first buffer when the last round wrote the second.  After the
+// last sweep (final = 1): the same copy for the states still running.
 __global__ void jgrid_sweep_end_kernel(int* conv, int* rot, const double2* acur_all, double2* a_all, size_t stride,
                                        const int* ncols_dev, int ld, int final_pass) {
   __shared__ int copy;
@@ -838,13 +879,14 @@ int launch_jacobi_grid(double2* a, double2* v, size_t stride, const int* ncols_d
       double2* nxt = (cur == a) ? a2 : a;
       const int rn = (r + 1 == rounds) ? 0 : r + 1;
       hipLaunchKernelGGL(jgrid_round_kernel, rgrid, dim3(256), 0, st, cur, nxt, stride, ncols_dev, ncols_max, r,
-                         rn, diag[k], off[k], diag[1 - k], off[1 - k], lg, log_stride, max_pairs, scale, conv,
-                         rot);
+                         rn, diag[k], off[k], diag[1 - k], off[1 - k], lg, log_stride, max_pairs, scale, conv);
       cur = nxt;
       k = 1 - k;
     }
     hipLaunchKernelGGL(jgrid_rows_kernel, vgrid, dim3(1024), rows_lds, st, v, stride, ncols_dev, ncols_max, lg,
                        log_stride, max_pairs, rounds, rpb, conv);
+    hipLaunchKernelGGL(jgrid_check_kernel, dim3(256, batch), dim3(256), 0, st, cur, stride, ncols_dev, ncols_max,
+                       scale, conv, rot);
     hipLaunchKernelGGL(jgrid_sweep_end_kernel, dim3(batch), dim3(256), 0, st, conv, rot, cur, a, stride,
                        ncols_dev, ncols_max, 0);
     // stop once every state has converged (one host read per sweep)
