@@ -250,7 +250,7 @@ struct LayerLayout {
   int* error;
 };
 constexpr int kChain2MaxBlocks = 256;
-constexpr int kJacobiSweeps = 30;   // sweep launches of the LDS Jacobi per bond (17-18 used at 2r = 128)
+constexpr int kJacobiSweeps = 30;   // sweep launches of the LDS Jacobi per bond (16-17 run at 2r = 128)
 // sweeps of the grid Jacobi (2r > 512): up to 34 were needed on the Gram
 // matrices of N=26 random-MPS states of bond 1024 at chi = 512
 constexpr int kJacobiGridSweeps = 80;
