@@ -510,7 +510,11 @@ __global__ void jgrid_init_kernel(const double2* a_all, double2* v_all, size_t s
 // one round: block (P, Q) of aout = J_P^H (block (P, Q) of ain) J_Q.  The
 // rotations come from diag_in / off_in (the diagonal and this round's pair
 // entries of ain, contiguous); the entries of aout that the next round (rn)
-// pairs, and the diagonal, also go to diag_out / off_out.
+// pairs, and the diagonal, also go to diag_out / off_out.  A thread takes row
+// pair P and kJgridQPerThread column pairs Q = Qh + h qs (qs = np / kJgridQPerThread
+// rounded up), so consecutive lanes stay on consecutive Q.
+constexpr int kJgridQPerThread = 1;   // 2 and 4 measured slower (fewer waves in flight)
+
 __global__ __launch_bounds__(256) void jgrid_round_kernel(const double2* ain_all, double2* aout_all, size_t stride,
                                                           const int* ncols_dev, int ld, int r, int rn,
                                                           const double* diag_in, const double2* off_in,
@@ -520,56 +524,60 @@ __global__ __launch_bounds__(256) void jgrid_round_kernel(const double2* ain_all
   const int b = blockIdx.y;
   if (conv[b]) return;
   const int n = ncols_dev[b], np = n / 2;
+  const int qs = (np + kJgridQPerThread - 1) / kJgridQPerThread;
   const size_t e = blockIdx.x * (size_t)blockDim.x + threadIdx.x;
-  if (e >= (size_t)np * np) return;
+  if (e >= (size_t)np * qs) return;
   const double2* a = ain_all + (size_t)b * stride;
   double2* ao = aout_all + (size_t)b * stride;
-  const int P = (int)(e / np), Q = (int)(e % np);
+  const int P = (int)(e / qs), Qh = (int)(e % qs);
   const int rr = r % (n - 1), rrn = rn % (n - 1);
   const double tol = 1e-17 * scale[b] + 1e-300;
-  int pr[2][2];
-  double4 R[2];
+  const double* dg = diag_in + (size_t)b * ld;
+  const double2* of = off_in + (size_t)b * max_pairs;
+  int p0, p1;
+  circle_pair_rr(n, rr, P, p0, p1);
+  bool rotated;
+  const double4 RP = jgrid_rotation(dg[p0], dg[p1], of[P], tol, rotated);
+  if (Qh == 0) log_all[(size_t)b * log_stride + (size_t)r * max_pairs + P] = RP;
+  const double2 K00 = make_double2(RP.x, 0.0), K01 = make_double2(RP.y, 0.0);
+  const double2 K10 = cscale(make_double2(RP.z, RP.w), -RP.y);
+  const double2 K11 = cscale(make_double2(RP.z, RP.w), RP.x);
 #pragma unroll
-  for (int k = 0; k < 2; ++k) {
-    const int m = k ? Q : P;
-    circle_pair_rr(n, rr, m, pr[k][0], pr[k][1]);
-    bool rk;
-    R[k] = jgrid_rotation(diag_in[(size_t)b * ld + pr[k][0]], diag_in[(size_t)b * ld + pr[k][1]],
-                          off_in[(size_t)b * max_pairs + m], tol, rk);
-  }
-  if (P == Q) log_all[(size_t)b * log_stride + (size_t)r * max_pairs + P] = R[0];
-  const int p0 = pr[0][0], p1 = pr[0][1], q0 = pr[1][0], q1 = pr[1][1];
-  const double2 J00 = make_double2(R[1].x, 0.0), J01 = make_double2(R[1].y, 0.0);
-  const double2 J10 = cscale(make_double2(R[1].z, R[1].w), -R[1].y);
-  const double2 J11 = cscale(make_double2(R[1].z, R[1].w), R[1].x);
-  const double2 K00 = make_double2(R[0].x, 0.0), K01 = make_double2(R[0].y, 0.0);
-  const double2 K10 = cscale(make_double2(R[0].z, R[0].w), -R[0].y);
-  const double2 K11 = cscale(make_double2(R[0].z, R[0].w), R[0].x);
-  const double2 a00 = a[(size_t)p0 * ld + q0], a01 = a[(size_t)p0 * ld + q1];
-  const double2 a10 = a[(size_t)p1 * ld + q0], a11 = a[(size_t)p1 * ld + q1];
-  const double2 b00 = cadd(cmul(a00, J00), cmul(a01, J10));
-  const double2 b01 = cadd(cmul(a00, J01), cmul(a01, J11));
-  const double2 b10 = cadd(cmul(a10, J00), cmul(a11, J10));
-  const double2 b11 = cadd(cmul(a10, J01), cmul(a11, J11));
-  double2 c00 = cadd(cmulc(K00, b00), cmulc(K10, b10));
-  double2 c01 = cadd(cmulc(K00, b01), cmulc(K10, b11));
-  double2 c10 = cadd(cmulc(K01, b00), cmulc(K11, b10));
-  double2 c11 = cadd(cmulc(K01, b01), cmulc(K11, b11));
-  if (P == Q) {
-    c00.y = 0.0; c11.y = 0.0;
-    c01 = cz(); c10 = cz();
-  }
-  ao[(size_t)p0 * ld + q0] = c00; ao[(size_t)p0 * ld + q1] = c01;
-  ao[(size_t)p1 * ld + q0] = c10; ao[(size_t)p1 * ld + q1] = c11;
-  const int ei[4] = {p0, p0, p1, p1}, ej[4] = {q0, q1, q0, q1};
-  const double2 ev[4] = {c00, c01, c10, c11};
+  for (int h = 0; h < kJgridQPerThread; ++h) {
+    const int Q = Qh + h * qs;
+    if (Q >= np) break;
+    int q0, q1;
+    circle_pair_rr(n, rr, Q, q0, q1);
+    const double4 RQ = jgrid_rotation(dg[q0], dg[q1], of[Q], tol, rotated);
+    const double2 J00 = make_double2(RQ.x, 0.0), J01 = make_double2(RQ.y, 0.0);
+    const double2 J10 = cscale(make_double2(RQ.z, RQ.w), -RQ.y);
+    const double2 J11 = cscale(make_double2(RQ.z, RQ.w), RQ.x);
+    const double2 a00 = a[(size_t)p0 * ld + q0], a01 = a[(size_t)p0 * ld + q1];
+    const double2 a10 = a[(size_t)p1 * ld + q0], a11 = a[(size_t)p1 * ld + q1];
+    const double2 b00 = cadd(cmul(a00, J00), cmul(a01, J10));
+    const double2 b01 = cadd(cmul(a00, J01), cmul(a01, J11));
+    const double2 b10 = cadd(cmul(a10, J00), cmul(a11, J10));
+    const double2 b11 = cadd(cmul(a10, J01), cmul(a11, J11));
+    double2 c00 = cadd(cmulc(K00, b00), cmulc(K10, b10));
+    double2 c01 = cadd(cmulc(K00, b01), cmulc(K10, b11));
+    double2 c10 = cadd(cmulc(K01, b00), cmulc(K11, b10));
+    double2 c11 = cadd(cmulc(K01, b01), cmulc(K11, b11));
+    if (P == Q) {
+      c00.y = 0.0; c11.y = 0.0;
+      c01 = cz(); c10 = cz();
+    }
+    ao[(size_t)p0 * ld + q0] = c00; ao[(size_t)p0 * ld + q1] = c01;
+    ao[(size_t)p1 * ld + q0] = c10; ao[(size_t)p1 * ld + q1] = c11;
+    const int ei[4] = {p0, p0, p1, p1}, ej[4] = {q0, q1, q0, q1};
+    const double2 ev[4] = {c00, c01, c10, c11};
 #pragma unroll
-  for (int k = 0; k < 4; ++k) {
-    const int i = ei[k], j = ej[k];
-    if (i == j) {
-      diag_out[(size_t)b * ld + i] = ev[k].x;
-    } else if (i < j && circle_partner_rr(n, rrn, i) == j) {
-      off_out[(size_t)b * max_pairs + circle_index_rr(n, rrn, i, j)] = ev[k];
+    for (int k = 0; k < 4; ++k) {
+      const int i = ei[k], j = ej[k];
+      if (i == j) {
+        diag_out[(size_t)b * ld + i] = ev[k].x;
+      } else if (i < j && circle_partner_rr(n, rrn, i) == j) {
+        off_out[(size_t)b * max_pairs + circle_index_rr(n, rrn, i, j)] = ev[k];
+      }
     }
   }
 }
@@ -869,7 +877,7 @@ int launch_jacobi_grid(double2* a, double2* v, size_t stride, const int* ncols_d
     return -1;
   hipLaunchKernelGGL(jgrid_init_kernel, dim3(std::max(1, ncols_max * ncols_max / 256 / 64), batch), dim3(256), 0,
                      st, a, v, stride, ncols_dev, ncols_max, diag[0], off[0], max_pairs, scale, conv, rot);
-  const size_t work = (size_t)max_pairs * max_pairs;
+  const size_t work = (size_t)max_pairs * ((max_pairs + kJgridQPerThread - 1) / kJgridQPerThread);
   const dim3 rgrid((unsigned)((work + 255) / 256), batch);
   const dim3 vgrid((unsigned)((ncols_max + rpb - 1) / rpb), batch);
   double2* cur = a;   // buffer holding the current A of the running states
