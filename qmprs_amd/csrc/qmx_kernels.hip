@@ -547,7 +547,12 @@ __global__ __launch_bounds__(256) void sweep_step_kernel(SweepArgs<S> a) {
 //   rho_i   = Y_i^H Y_i         (Y_i = X_{i+1} viewed as 2^i x 2 r_{i+1})
 // The last block of each state finds the kept subspace of rho_i (FP64 Jacobi,
 // rsum2 cutoff, max bond 2) and stores its canonical basis W_i and r_i.
+constexpr int kC2Rows = 256;   // rows of Y_i per chunk (= threads per block)
+// LDS tile index with one 16-byte pad per 8 entries (rows of 4 entries per lane)
+QMX_DEV int c2pad(int l) { return l + (l >> 3); }
+
 __global__ __launch_bounds__(256) void chain2_step_kernel(Chain2Args a) {
+  __shared__ double2 sx[2 * kC2Rows * 4 + (2 * kC2Rows * 4) / 8];
   __shared__ double red[4 * 32];
   __shared__ double tot[32];
   __shared__ double scratch[256];
@@ -587,55 +592,81 @@ __global__ __launch_bounds__(256) void chain2_step_kernel(Chain2Args a) {
   double e[20];
 #pragma unroll
   for (int j = 0; j < 20; ++j) e[j] = 0.0;
-  for (size_t row = (size_t)blockIdx.x * blockDim.x + threadIdx.x; row < rows;
-       row += (size_t)gridDim.x * blockDim.x) {
-    double2 y[4];
-    if (i == n - 1) {
-      y[0] = xin[2 * row];
-      y[1] = xin[2 * row + 1];
-      y[2] = cz(); y[3] = cz();
-    } else {
-      // rows 2row, 2row+1 of X_{i+1} = Y_{i+1} W_{i+1}
-      double2 s[2][2];
+  // Rows of Y_i go in chunks of kC2Rows, one per thread (the same rows per
+  // thread as a grid-stride loop: kC2Rows == blockDim.x).  A chunk's input (2
+  // rows of X_{i+1}, or of X_N, per row of Y_i, nx values each) and its output
+  // rows of X_{i+1} are contiguous in HBM and pass through the LDS tile, so
+  // every wave-wide global access is coalesced.
+  const int nx = (i == n - 1) ? 1 : nin;
+  for (size_t r0 = (size_t)blockIdx.x * kC2Rows; r0 < rows; r0 += (size_t)gridDim.x * kC2Rows) {
+    const int nr = (int)min((size_t)kC2Rows, rows - r0);
+    const double2* src = xin + 2 * r0 * nx;
+    for (int l = threadIdx.x; l < 2 * nr * nx; l += blockDim.x) sx[c2pad(l)] = src[l];
+    __syncthreads();
+    const int lr = threadIdx.x;
+    const bool act = lr < nr;
+    double2 y[4] = {cz(), cz(), cz(), cz()};
+    double2 s[2][2];
+    if (act) {
+      if (i == n - 1) {
+        y[0] = sx[c2pad(2 * lr)];
+        y[1] = sx[c2pad(2 * lr + 1)];
+      } else {
+        // rows 2row, 2row+1 of X_{i+1} = Y_{i+1} W_{i+1}
 #pragma unroll
-      for (int h = 0; h < 2; ++h) {
-        const size_t xr = 2 * row + h;
-        double2 yin[4];
-        yin[0] = xin[xr * nin];
-        yin[1] = xin[xr * nin + 1];
-        if (nin == 4) {
-          yin[2] = xin[xr * nin + 2];
-          yin[3] = xin[xr * nin + 3];
-        } else {
-          yin[2] = cz(); yin[3] = cz();
+        for (int h = 0; h < 2; ++h) {
+          const int xr = 2 * lr + h;
+          double2 yin[4];
+          yin[0] = sx[c2pad(xr * nin)];
+          yin[1] = sx[c2pad(xr * nin + 1)];
+          if (nin == 4) {
+            yin[2] = sx[c2pad(xr * nin + 2)];
+            yin[3] = sx[c2pad(xr * nin + 3)];
+          } else {
+            yin[2] = cz(); yin[3] = cz();
+          }
+#pragma unroll
+          for (int c = 0; c < 2; ++c) {
+            double2 acc = cz();
+#pragma unroll
+            for (int j = 0; j < 4; ++j) acc = cfma(yin[j], wp[j][c], acc);
+            s[h][c] = acc;
+          }
         }
-#pragma unroll
-        for (int c = 0; c < 2; ++c) {
-          double2 acc = cz();
-#pragma unroll
-          for (int j = 0; j < 4; ++j) acc = cfma(yin[j], wp[j][c], acc);
-          s[h][c] = acc;
-        }
-        xout[xr * rcur] = s[h][0];
-        if (rcur == 2) xout[xr * rcur + 1] = s[h][1];
+        // Y_i row: columns (p, c) = p * rcur + c
+        const bool two = (rcur == 2);
+        y[0] = s[0][0];
+        y[1] = two ? s[0][1] : s[1][0];
+        y[2] = two ? s[1][0] : cz();
+        y[3] = two ? s[1][1] : cz();
       }
-      // Y_i row: columns (p, c) = p * rcur + c
-      const bool two = (rcur == 2);
-      y[0] = s[0][0];
-      y[1] = two ? s[0][1] : s[1][0];
-      y[2] = two ? s[1][0] : cz();
-      y[3] = two ? s[1][1] : cz();
     }
-    int t = 0;
+    __syncthreads();   // inputs consumed: the tile takes the outputs
+    if (i < n - 1) {
+      if (act) {
 #pragma unroll
-    for (int p = 0; p < 4; ++p)
-#pragma unroll
-      for (int q = p; q < 4; ++q) {
-        const double2 c = cmulc(y[p], y[q]);
-        e[2 * t] += c.x;
-        e[2 * t + 1] += c.y;
-        ++t;
+        for (int h = 0; h < 2; ++h) {
+          sx[c2pad((2 * lr + h) * rcur)] = s[h][0];
+          if (rcur == 2) sx[c2pad((2 * lr + h) * rcur + 1)] = s[h][1];
+        }
       }
+      __syncthreads();
+      double2* dst = xout + 2 * r0 * rcur;
+      for (int l = threadIdx.x; l < 2 * nr * rcur; l += blockDim.x) dst[l] = sx[c2pad(l)];
+    }
+    if (act) {
+      int t = 0;
+#pragma unroll
+      for (int p = 0; p < 4; ++p)
+#pragma unroll
+        for (int q = p; q < 4; ++q) {
+          const double2 c = cmulc(y[p], y[q]);
+          e[2 * t] += c.x;
+          e[2 * t + 1] += c.y;
+          ++t;
+        }
+    }
+    __syncthreads();   // before the next chunk's loads overwrite the tile
   }
   block_sum<20>(e, red);
   double* pb = a.part + (size_t)b * gridDim.x * 20;
