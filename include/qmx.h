@@ -220,14 +220,29 @@ int qmx_encode_prepare(qmx_handle* h, const qmx_encode_config* cfg, const void* 
                        void* gates, int* widths, int* layers_used, void* psi_out, void* target_out,
                        void* workspace, size_t workspace_bytes, void* stream);
 
+/* Grouped bond dimensions (no reference counterpart: each reference call
+ * has one bond_dimension, base.py:58-65).  qmx_encode of a batch of equal-size
+ * states whose bond dimensions differ: states [first[k], first[k] + count[k])
+ * use bond dimension chi[k]; the groups tile the batch in order (first[0] = 0,
+ * counts summing to cfg->batch).  Only the chi-truncation depends on the bond
+ * dimension and runs per group; the layer extraction and the sweeps run once
+ * over the whole batch.  cfg->bond_dimension is ignored; no fidelity goal.
+ * With one group the results equal qmx_encode's bit for bit. */
+size_t qmx_encode_grouped_workspace(const qmx_encode_config* cfg, const int* first, const int* count,
+                                    const int* chi, int ngroups);
+int qmx_encode_grouped(qmx_handle* h, const qmx_encode_config* cfg, const int* first, const int* count,
+                       const int* chi, int ngroups, const void* statevectors, void* gates, int* widths,
+                       int* layers_used, void* fidelity, void* workspace, size_t workspace_bytes,
+                       void* stream);
+
 /* Ragged batches (no reference counterpart: the reference encodes one state
  * per prepare_state call, base.py:58-106).  Each state brings its own qubit
- * count, bond dimension and device buffers; states with equal (N, chi) are
- * encoded together as one batch (qmx_encode), and the buckets, largest
- * estimated cost first, run on up to max_concurrency worker threads with
- * their own handles and streams, so they overlap on the GPU.  All states
- * share num_layers, num_sweeps and the dtypes.  Synchronises (returns when
- * every state is done).
+ * count, bond dimension and device buffers; states with equal N are encoded
+ * together as one batch (qmx_encode_grouped, ordered by bond dimension), and
+ * the buckets, largest estimated cost first, run on up to max_concurrency
+ * worker threads with their own handles and streams, so they overlap on the
+ * GPU.  All states share num_layers, num_sweeps and the dtypes.  Synchronises
+ * (returns when every state is done).
  *   gates        device, num_layers x N x 16 complex128 (qmx gate layout)
  *   widths       device, num_layers x N ints
  *   layers_used  device, 1 int

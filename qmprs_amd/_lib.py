@@ -93,6 +93,10 @@ SIGNATURES = {
                           _SZ, _VP]),
     "qmx_encode_prepare": (_INT, [_VP, ctypes.POINTER(EncodeConfig), _VP, _VP, _VP, _VP, _VP, _VP, _VP,
                                   _SZ, _VP]),
+    "qmx_encode_grouped_workspace": (_SZ, [ctypes.POINTER(EncodeConfig), ctypes.POINTER(_INT),
+                                           ctypes.POINTER(_INT), ctypes.POINTER(_INT), _INT]),
+    "qmx_encode_grouped": (_INT, [_VP, ctypes.POINTER(EncodeConfig), ctypes.POINTER(_INT), ctypes.POINTER(_INT),
+                                  ctypes.POINTER(_INT), _INT, _VP, _VP, _VP, _VP, _VP, _VP, _SZ, _VP]),
     "qmx_encode_ragged_workspace": (_SZ, [_INT, ctypes.POINTER(RaggedState), _INT, _INT, _INT, _INT]),
     "qmx_encode_ragged": (_INT, [_VP, _INT, ctypes.POINTER(RaggedState), _INT, _INT, _INT, _INT, _DBL, _DBL,
                                  _INT, _VP, _SZ, _VP]),

@@ -1336,6 +1336,8 @@ int qmx_circuit_state(qmx_handle* h, int num_qubits, int batch, int num_layers, 
                    state_out, workspace, workspace_bytes, stream);
 }
 
+}  // extern "C"
+
 namespace {
 struct EncodeLayout {
   ChiLayout chi;
@@ -1364,6 +1366,8 @@ EncodeLayout encode_layout(Carver& c, const qmx_encode_config& g) {
 }
 }  // namespace
 
+extern "C" {
+
 size_t qmx_encode_workspace(const qmx_encode_config* cfg) {
   if (!cfg || cfg->num_qubits < 2 || cfg->num_qubits > kMaxQubits || cfg->batch < 1 ||
       cfg->bond_dimension < 1 || cfg->num_layers < 1 || cfg->num_sweeps < 0 ||
@@ -1374,6 +1378,8 @@ size_t qmx_encode_workspace(const qmx_encode_config* cfg) {
   encode_layout(c, *cfg);
   return c.off + 256;
 }
+
+}  // extern "C"
 
 namespace {
 int encode_check(qmx_handle* h, const qmx_encode_config* cfg, size_t workspace_bytes) {
@@ -1417,6 +1423,8 @@ int encode_prepare(qmx_handle* h, const qmx_encode_config& g, const void* statev
 }
 }  // namespace
 
+extern "C" {
+
 int qmx_encode(qmx_handle* h, const qmx_encode_config* cfg, const void* statevectors,
                void* gates, int* widths, int* layers_used, void* fidelity, const double* goal,
                int* sweeps_done, void* workspace, size_t workspace_bytes, void* stream) {
@@ -1452,6 +1460,118 @@ int qmx_encode_prepare(qmx_handle* h, const qmx_encode_config* cfg, const void* 
                             amps * (g.sweep_dtype == QMX_C64 ? sizeof(float2) : sizeof(double2)),
                             hipMemcpyDeviceToDevice, st));
   return QMX_OK;
+}
+
+// ------------------------------------------------ grouped bond dimensions --
+// One encode of a batch of equal-size states whose bond dimensions differ:
+// states [first[k], first[k] + count[k]) share bond dimension chi[k] (the
+// groups tile the batch in order).  Only the chi-truncation depends on the
+// bond dimension, so it runs per group; normalisation, layer extraction and the
+// sweeps run once over the whole batch.  Used by qmx_encode_ragged (one call
+// per qubit count instead of one per (qubit count, bond dimension)).
+}  // extern "C"
+
+namespace {
+struct GroupedLayout {
+  std::vector<ChiLayout> chi;
+  double* nrm;
+  double* npart;
+  int* ncnt;
+  LayerLayout layer;
+  double2* psi;
+  double2* target;
+  double2* psi_w;
+  void* target_s;
+  void* sweep_ws;
+  size_t sweep_bytes;
+};
+
+constexpr int kGroupedNormBlocks = 256;
+
+GroupedLayout grouped_layout(Carver& c, const qmx_encode_config& g, const int* count, const int* chi, int ng) {
+  GroupedLayout E{};
+  const size_t len = state_len(g.num_qubits);
+  for (int k = 0; k < ng; ++k) E.chi.push_back(chi_layout(c, g.num_qubits, count[k], chi[k], false));
+  E.nrm = c.take<double>(g.batch);
+  E.npart = c.take<double>(static_cast<size_t>(g.batch) * kGroupedNormBlocks * 2);
+  E.ncnt = c.take<int>(g.batch);
+  E.layer = layer_layout(c, g.num_qubits, g.batch);
+  E.psi = c.take<double2>(len * g.batch);
+  E.target = c.take<double2>(len * g.batch);
+  E.psi_w = c.take<double2>(len * g.batch);
+  E.target_s = (g.sweep_dtype == QMX_C64) ? static_cast<void*>(c.take<float2>(len * g.batch))
+                                          : static_cast<void*>(E.target);
+  E.sweep_bytes = qmx_sweep_workspace(g.num_qubits, g.batch, g.sweep_dtype);
+  E.sweep_ws = c.take<char>(E.sweep_bytes);
+  return E;
+}
+
+bool groups_valid(const qmx_encode_config& g, const int* first, const int* count, const int* chi, int ng) {
+  if (ng < 1 || !first || !count || !chi) return false;
+  int next = 0;
+  for (int k = 0; k < ng; ++k) {
+    if (first[k] != next || count[k] < 1 || chi[k] < 1 || !chi_supported(g.num_qubits, chi[k])) return false;
+    next += count[k];
+  }
+  return next == g.batch;
+}
+}  // namespace
+
+extern "C" {
+
+size_t qmx_encode_grouped_workspace(const qmx_encode_config* cfg, const int* first, const int* count,
+                                    const int* chi, int ngroups) {
+  if (qmx_encode_workspace(cfg) == 0 || !groups_valid(*cfg, first, count, chi, ngroups)) return 0;
+  Carver c(nullptr);
+  grouped_layout(c, *cfg, count, chi, ngroups);
+  return c.off + 256;
+}
+
+int qmx_encode_grouped(qmx_handle* h, const qmx_encode_config* cfg, const int* first, const int* count,
+                       const int* chi, int ngroups, const void* statevectors, void* gates, int* widths,
+                       int* layers_used, void* fidelity, void* workspace, size_t workspace_bytes,
+                       void* stream) {
+  if (!h || !cfg) return QMX_ERR_INVALID;
+  const qmx_encode_config g = *cfg;
+  QMX_CALL(check_common(h, g.num_qubits, g.batch));
+  if (g.num_layers < 1) return fail(h, QMX_ERR_INVALID, "The number of layers must be a positive integer.");
+  if (g.num_sweeps < 0) return fail(h, QMX_ERR_INVALID, "num_sweeps must be >= 0");
+  if ((g.sv_dtype != QMX_C64 && g.sv_dtype != QMX_C128) ||
+      (g.sweep_dtype != QMX_C64 && g.sweep_dtype != QMX_C128))
+    return fail(h, QMX_ERR_INVALID, "dtype");
+  if (!groups_valid(g, first, count, chi, ngroups))
+    return fail(h, QMX_ERR_INVALID, "bond-dimension groups must tile the batch (bond_dimension <= 512 when it truncates)");
+  if (!statevectors || !gates || !widths || !layers_used || !fidelity || !workspace)
+    return fail(h, QMX_ERR_INVALID, "null pointer");
+  if (workspace_bytes < qmx_encode_grouped_workspace(cfg, first, count, chi, ngroups))
+    return fail(h, QMX_ERR_WORKSPACE, "workspace too small");
+  QMX_HIP(h, hipSetDevice(h->device));
+  hipStream_t st = static_cast<hipStream_t>(stream);
+  Carver c(workspace);
+  GroupedLayout E = grouped_layout(c, g, count, chi, ngroups);
+  const int n = g.num_qubits, B = g.batch;
+  const size_t len = state_len(n);
+  const size_t svb = len * (g.sv_dtype == QMX_C64 ? sizeof(float2) : sizeof(double2));
+  for (int k = 0; k < ngroups; ++k) {
+    const size_t o = static_cast<size_t>(first[k]);
+    QMX_CALL(run_chi(h, n, count[k], chi[k], g.cutoff, static_cast<const char*>(statevectors) + o * svb,
+                     g.sv_dtype, E.psi + o * len, E.target + o * len, nullptr, E.chi[k], st));
+    QMX_CALL(device_error_status(h, E.chi[k].error, st));
+  }
+  // psi_w = target / |target|, then as encode_prepare
+  int nb = static_cast<int>(std::min<size_t>(kGroupedNormBlocks, std::max<size_t>(1, len / 1024)));
+  launch_fill_int(E.ncnt, 0, B, st);
+  launch_norm2(static_cast<const double2*>(E.target), n, B, E.npart, E.ncnt, E.nrm, nb, st);
+  launch_scale_convert(static_cast<const double2*>(E.target), E.psi_w, n, B, E.nrm, st);
+  QMX_LAUNCHED(h, "normalise target");
+  QMX_CALL(run_extract(h, n, B, g.num_layers, g.fidelity_threshold, g.cutoff, E.psi_w,
+                       static_cast<double2*>(gates), widths, nullptr, layers_used, nullptr, E.layer, st));
+  if (g.sweep_dtype == QMX_C64)
+    launch_scale_convert(static_cast<const double2*>(E.target), static_cast<float2*>(E.target_s), n, B,
+                         nullptr, st);
+  QMX_LAUNCHED(h, "target conversion");
+  return qmx_sweep(h, n, B, g.num_layers, layers_used, g.num_sweeps, E.target_s, g.sweep_dtype, gates, widths,
+                   E.psi, fidelity, nullptr, nullptr, nullptr, nullptr, E.sweep_ws, E.sweep_bytes, stream);
 }
 
 // ---------------------------------------------------- learned warm start --

@@ -2,12 +2,14 @@
 // in one call (no reference counterpart: the reference encodes one state per
 // Sequential.prepare_state call, base.py:58-106).
 //
-// States are grouped into buckets of equal (num_qubits, bond_dimension);
-// buckets are ordered by estimated cost (2^N * N * layers * sweeps, largest
-// first) and handed to a pool of worker threads.  Each worker owns a qmx
-// handle and a non-blocking stream; per bucket it gathers the inputs into a
-// contiguous batch (device-to-device copies), runs qmx_encode, and scatters
-// gates, widths, layer counts and fidelities back to each state's buffers.
+// States are grouped into buckets of equal num_qubits, ordered inside a
+// bucket by bond dimension; buckets are ordered by estimated cost (2^N * N *
+// layers * sweeps, largest first) and handed to a pool of worker threads.
+// Each worker owns a qmx handle and a non-blocking stream; per bucket it
+// gathers the inputs into a contiguous batch (device-to-device copies), runs
+// one encode in which only the chi-truncation is split by bond dimension
+// (qmx_encode_grouped), and scatters gates, widths, layer counts and
+// fidelities back to each state's buffers.
 // The host synchronisations inside an encode (chi-chain error check, layer
 // extraction) then block only their own worker, so buckets overlap on the
 // GPU.  The caller's stream is joined at both ends.
@@ -31,8 +33,9 @@ int qmx_handle_device(const qmx_handle* h);
 namespace {
 
 struct Bucket {
-  int n, chi;
-  std::vector<int> idx;
+  int n;
+  std::vector<int> idx;                           // states, by (bond dimension, index)
+  std::vector<int> gfirst, gcount, gchi;          // bond-dimension groups of idx
   double cost;
   size_t off_in, off_gates, off_widths, off_used, off_fid, off_ws, ws_bytes;
 };
@@ -45,17 +48,27 @@ size_t sv_bytes(int dtype) { return dtype == QMX_C64 ? 8 : 16; }
 size_t plan(int count, const qmx_ragged_state* st, int L, int S, int sv_dtype, int sweep_dtype,
             std::vector<Bucket>& out) {
   if (count < 1 || !st || L < 1 || S < 0) return 0;
-  std::map<std::pair<int, int>, std::vector<int>> groups;
+  std::map<int, std::vector<int>> groups;
   for (int i = 0; i < count; ++i) {
     if (st[i].num_qubits < 2 || st[i].num_qubits > 30 || st[i].bond_dimension < 1) return 0;
-    groups[{st[i].num_qubits, st[i].bond_dimension}].push_back(i);
+    groups[st[i].num_qubits].push_back(i);
   }
   out.clear();
   for (auto& kv : groups) {
     Bucket b{};
-    b.n = kv.first.first;
-    b.chi = kv.first.second;
+    b.n = kv.first;
     b.idx = kv.second;
+    std::stable_sort(b.idx.begin(), b.idx.end(),
+                     [&](int x, int y) { return st[x].bond_dimension < st[y].bond_dimension; });
+    for (int k = 0; k < static_cast<int>(b.idx.size()); ++k) {
+      const int chi = st[b.idx[k]].bond_dimension;
+      if (b.gchi.empty() || b.gchi.back() != chi) {
+        b.gfirst.push_back(k);
+        b.gcount.push_back(0);
+        b.gchi.push_back(chi);
+      }
+      ++b.gcount.back();
+    }
     b.cost = static_cast<double>(b.idx.size()) * static_cast<double>(1ull << b.n) * b.n * L * (S + 1);
     out.push_back(b);
   }
@@ -73,8 +86,9 @@ size_t plan(int count, const qmx_ragged_state* st, int L, int S, int sv_dtype, i
     off += B * sizeof(int);
     b.off_fid = off = align256(off);
     off += B * (S + 1) * 16;
-    qmx_encode_config cfg{b.n, static_cast<int>(B), b.chi, L, S, sv_dtype, sweep_dtype, 1 - 1e-6, 1e-10};
-    b.ws_bytes = qmx_encode_workspace(&cfg);
+    qmx_encode_config cfg{b.n, static_cast<int>(B), b.gchi.back(), L, S, sv_dtype, sweep_dtype, 1 - 1e-6, 1e-10};
+    b.ws_bytes = qmx_encode_grouped_workspace(&cfg, b.gfirst.data(), b.gcount.data(), b.gchi.data(),
+                                              static_cast<int>(b.gchi.size()));
     if (b.ws_bytes == 0) return 0;
     b.off_ws = off = align256(off);
     off += b.ws_bytes;
@@ -160,13 +174,13 @@ int qmx_encode_ragged(qmx_handle* h, int count, const qmx_ragged_state* states, 
         fail(QMX_ERR_HIP, "gather");
         break;
       }
-      qmx_encode_config cfg{b.n, B, b.chi, L, S, sv_dtype, sweep_dtype, fidelity_threshold, cutoff};
-      int rc = qmx_encode(wh, &cfg, base + b.off_in, base + b.off_gates, reinterpret_cast<int*>(base + b.off_widths),
-                          reinterpret_cast<int*>(base + b.off_used), base + b.off_fid, nullptr, nullptr,
-                          base + b.off_ws, b.ws_bytes, ws);
+      qmx_encode_config cfg{b.n, B, b.gchi.back(), L, S, sv_dtype, sweep_dtype, fidelity_threshold, cutoff};
+      int rc = qmx_encode_grouped(wh, &cfg, b.gfirst.data(), b.gcount.data(), b.gchi.data(),
+                                  static_cast<int>(b.gchi.size()), base + b.off_in, base + b.off_gates,
+                                  reinterpret_cast<int*>(base + b.off_widths), reinterpret_cast<int*>(base + b.off_used),
+                                  base + b.off_fid, base + b.off_ws, b.ws_bytes, ws);
       if (rc != QMX_OK) {
-        fail(rc, std::string("bucket N=") + std::to_string(b.n) + " chi=" + std::to_string(b.chi) + ": " +
-                     qmx_last_error(wh));
+        fail(rc, std::string("bucket N=") + std::to_string(b.n) + ": " + qmx_last_error(wh));
         (void)hipStreamSynchronize(ws);
         qmx_destroy(wh);
         return;

@@ -206,6 +206,52 @@ def encode(statevectors, bond_dimension: int, num_layers: int = 1, num_sweeps: i
     return EncodeResult(gates, widths, used, fid, done)
 
 
+def bond_groups(bond_dimensions) -> tuple[list[int], list[int], list[int]]:
+    """Runs of equal bond dimension: (first, count, chi) per run, in order."""
+    first, count, chi = [], [], []
+    for i, d in enumerate(int(x) for x in bond_dimensions):
+        if chi and chi[-1] == d:
+            count[-1] += 1
+        else:
+            first.append(i)
+            count.append(1)
+            chi.append(d)
+    return first, count, chi
+
+
+def encode_grouped(statevectors, bond_dimensions, num_layers: int = 1, num_sweeps: int = 0,
+                   fidelity_threshold: float = DEFAULT_THRESHOLD, cutoff: float = CUTOFF,
+                   sweep_dtype=torch.complex64) -> EncodeResult:
+    """``encode`` of equal-size states with one bond dimension per state
+    (qmx_encode_grouped): the chi-truncation runs per run of equal bond
+    dimension, everything after it once over the batch."""
+    sv = as_device_states(statevectors)
+    b, length = sv.shape
+    n = num_qubits_of(length)
+    if len(bond_dimensions) != b:
+        raise ValueError("one bond dimension per state")
+    first, count, chi = bond_groups(bond_dimensions)
+    arr = lambda v: (ctypes.c_int * len(v))(*v)
+    f_a, c_a, x_a = arr(first), arr(count), arr(chi)
+    lib, h = _lib.load(), _lib.handle()
+    cfg = _lib.EncodeConfig(n, b, max(chi), int(num_layers), int(num_sweeps), _DTYPES[sv.dtype],
+                            _DTYPES[sweep_dtype], float(fidelity_threshold), float(cutoff))
+    need = lib.qmx_encode_grouped_workspace(ctypes.byref(cfg), f_a, c_a, x_a, len(chi))
+    if need == 0:
+        raise ValueError("invalid bond dimensions")
+    workspace = _lib.workspace(need, sv.device)
+    gates = torch.zeros((b, num_layers, n, 4, 4), dtype=torch.complex128, device=sv.device)
+    widths = torch.ones((b, num_layers, n), dtype=torch.int32, device=sv.device)
+    used = torch.zeros(b, dtype=torch.int32, device=sv.device)
+    fid = torch.zeros((b, num_sweeps + 1), dtype=torch.complex128, device=sv.device)
+    st = lib.qmx_encode_grouped(h, ctypes.byref(cfg), f_a, c_a, x_a, len(chi), _lib.ptr(sv), _lib.ptr(gates),
+                                _lib.ptr(widths), _lib.ptr(used), _lib.ptr(fid), _lib.ptr(workspace),
+                                workspace.numel(), _lib.stream_ptr())
+    _lib.check(st, h)
+    return EncodeResult(gates, widths, used, fid, torch.full((b,), int(num_sweeps), dtype=torch.int32,
+                                                             device=sv.device))
+
+
 @dataclass
 class Prepared:
     psi: torch.Tensor          # (B, 2^N) complex128 normalised input

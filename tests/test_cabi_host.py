@@ -72,6 +72,26 @@ def test_version_and_workspace_queries():
     assert lib.qmx_encode_workspace(ctypes.byref(cfg)) == 0
 
 
+def test_grouped_workspace_queries():
+    """qmx_encode_grouped_workspace: groups must tile the batch in order."""
+    mod, lib = _lib()
+    cfg = mod.EncodeConfig(12, 5, 32, 4, 3, mod.QMX_C64, mod.QMX_C64, 1 - 1e-6, 1e-10)
+    arr = lambda v: (ctypes.c_int * len(v))(*v)
+    ok = lib.qmx_encode_grouped_workspace(ctypes.byref(cfg), arr([0, 2]), arr([2, 3]), arr([8, 32]), 2)
+    assert ok > 0
+    one = lib.qmx_encode_grouped_workspace(ctypes.byref(cfg), arr([0]), arr([5]), arr([32]), 1)
+    assert one > 0
+    for first, count, chi in (([0, 3], [2, 3], [8, 32]),     # gap
+                              ([0, 2], [2, 2], [8, 32]),     # counts short of the batch
+                              ([0, 2], [2, 3], [8, 0]),      # bond dimension 0
+                              ([1], [5], [32])):             # not starting at 0
+        assert lib.qmx_encode_grouped_workspace(ctypes.byref(cfg), arr(first), arr(count), arr(chi),
+                                                len(chi)) == 0
+    cfg20 = mod.EncodeConfig(20, 2, 64, 4, 3, mod.QMX_C64, mod.QMX_C64, 1 - 1e-6, 1e-10)
+    assert lib.qmx_encode_grouped_workspace(ctypes.byref(cfg20), arr([0, 1]), arr([1, 1]), arr([64, 600]),
+                                            2) == 0      # truncating bond dimension above 512
+
+
 def test_encode_config_layout_matches_c_struct():
     mod, _ = _lib()
     # 7 ints then 2 doubles: doubles start at offset 32 (8-byte aligned)

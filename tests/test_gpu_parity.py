@@ -366,6 +366,47 @@ def test_encode_ragged_equals_bucket_encodes(gpu):
             assert torch.equal(res[i].fidelity, ref.fidelity[k])
 
 
+def test_encode_grouped_one_group_equals_encode(gpu):
+    """qmx_encode_grouped with one bond dimension is qmx_encode, bit for bit."""
+    eng = _engine()
+    svs = np.stack([O.random_mps_state(10, 16, 90 + i).astype(np.complex64) for i in range(3)])
+    L, S = 3, 2
+    a = eng.encode(svs, 8, L, S)
+    b = eng.encode_grouped(svs, [8, 8, 8], L, S)
+    torch.cuda.synchronize()
+    assert torch.equal(a.layers_used, b.layers_used)
+    assert torch.equal(a.gates, b.gates)
+    assert torch.equal(a.widths, b.widths)
+    assert torch.equal(a.fidelity, b.fidelity)
+
+
+def test_encode_ragged_mixed_bond_dimensions(gpu):
+    """States of one qubit count but different bond dimensions share one
+    encode (per-group chi-truncation, one layer extraction and sweep batch):
+    each state matches qmx_encode of its own (N, chi) bucket up to the
+    summation order of the batch-sized reductions (FP64 sweeps: gates within
+    1e-9, fidelities within 1e-10)."""
+    eng = _engine()
+    spec = [(10, 4), (10, 16), (11, 8), (10, 4), (11, 32), (10, 8), (11, 8)]
+    svs = [O.random_mps_state(n, 2 * chi, 110 + i).astype(np.complex64) for i, (n, chi) in enumerate(spec)]
+    L, S = 3, 3
+    res = eng.encode_ragged(svs, [c for _, c in spec], L, S, sweep_dtype=torch.complex128, max_concurrency=2)
+    buckets = {}
+    for i, (n, chi) in enumerate(spec):
+        buckets.setdefault((n, chi), []).append(i)
+    worst_g = worst_f = 0.0
+    for (n, chi), idx in buckets.items():
+        ref = eng.encode(np.stack([svs[i] for i in idx]), chi, L, S, sweep_dtype=torch.complex128)
+        for k, i in enumerate(idx):
+            assert res[i].layers_used == int(ref.layers_used[k])
+            assert torch.equal(res[i].widths, ref.widths[k])
+            worst_g = max(worst_g, (res[i].gates - ref.gates[k]).abs().max().item())
+            worst_f = max(worst_f, (res[i].fidelity - ref.fidelity[k]).abs().max().item())
+    print(f"max gate diff {worst_g:.2e}, max fidelity diff {worst_f:.2e}")
+    assert worst_g < 1e-9
+    assert worst_f < 1e-10
+
+
 def test_encode_ragged_rejects_bad_input(gpu):
     eng = _engine()
     with pytest.raises(ValueError):
