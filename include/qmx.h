@@ -240,9 +240,10 @@ int qmx_encode_grouped(qmx_handle* h, const qmx_encode_config* cfg, const int* f
  * count, bond dimension and device buffers; states with equal N are encoded
  * together as one batch (qmx_encode_grouped, ordered by bond dimension), and
  * the buckets, largest estimated cost first, run on up to max_concurrency
- * worker threads with their own handles and streams, so they overlap on the
- * GPU.  All states share num_layers, num_sweeps and the dtypes.  Synchronises
- * (returns when every state is done).
+ * worker threads (6 when max_concurrency <= 0) with their own handles and
+ * streams, so they overlap on the GPU.  All states share num_layers,
+ * num_sweeps and the dtypes.  Synchronises (returns when every state is
+ * done).
  *   gates        device, num_layers x N x 16 complex128 (qmx gate layout)
  *   widths       device, num_layers x N ints
  *   layers_used  device, 1 int

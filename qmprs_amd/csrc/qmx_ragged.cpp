@@ -130,7 +130,7 @@ int qmx_encode_ragged(qmx_handle* h, int count, const qmx_ragged_state* states, 
     (void)hipEventDestroy(start);
     return QMX_ERR_HIP;
   }
-  const int workers = std::max(1, std::min<int>(max_concurrency > 0 ? max_concurrency : 4,
+  const int workers = std::max(1, std::min<int>(max_concurrency > 0 ? max_concurrency : 6,
                                                static_cast<int>(buckets.size())));
   std::atomic<size_t> next{0};
   std::atomic<int> status{QMX_OK};

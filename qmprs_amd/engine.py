@@ -372,7 +372,7 @@ class RaggedResult:
 
 def encode_ragged(statevectors, bond_dimensions, num_layers: int = 1, num_sweeps: int = 0,
                   fidelity_threshold: float = DEFAULT_THRESHOLD, cutoff: float = CUTOFF,
-                  sweep_dtype=torch.complex64, max_concurrency: int = 4) -> list[RaggedResult]:
+                  sweep_dtype=torch.complex64, max_concurrency: int = 6) -> list[RaggedResult]:
     """prepare_state for states of different sizes in one call (qmx_encode_ragged).
 
     ``statevectors``: sequence of 1-D arrays / tensors (any power-of-two
