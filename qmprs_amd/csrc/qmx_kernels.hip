@@ -42,7 +42,7 @@ QMX_DEV void build_geom(Geom& g, int n, int tbits, uint64_t umask) {
   int t = tbits < n ? tbits : n;
   uint64_t full = (n >= 64) ? ~0ull : ((1ull << n) - 1ull);
   uint64_t p = umask & full;
-  for (int bit = 0; bit < n && __popcll(p) < t; ++bit) p |= (1ull << bit);
+  for (int bit = 0; bit < n && __popcll(p) < (unsigned)t; ++bit) p |= (1ull << bit);
   g.t = t;
   int c0 = 0;
   while (c0 < n && ((p >> c0) & 1ull)) ++c0;

@@ -152,7 +152,7 @@ __global__ __launch_bounds__(256) void mse_kernel(const float* y, const float* t
   red[threadIdx.x] = acc;
   __syncthreads();
   for (int s = 128; s > 0; s >>= 1) {
-    if (threadIdx.x < s) red[threadIdx.x] += red[threadIdx.x + s];
+    if ((int)threadIdx.x < s) red[threadIdx.x] += red[threadIdx.x + s];
     __syncthreads();
   }
   if (threadIdx.x == 0) part[blockIdx.x] = red[0] * inv_norm;

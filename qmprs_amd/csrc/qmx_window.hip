@@ -273,7 +273,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(WPE))) void
     if (threadIdx.x < 72) rdep[x] = group_roffl(a.gu, 4 * x);
   }
   // one gate entry per thread ((np + nc) * 16 <= 256)
-  if (threadIdx.x < (np + nc) * 16) {
+  if ((int)threadIdx.x < (np + nc) * 16) {
     const int j = threadIdx.x >> 4, e = threadIdx.x & 15;
     const int s = j < np ? a.p0 + j : a.s0 + (j - np);
     int layer, site;
