@@ -12,7 +12,7 @@ constexpr int kMaxPassGates = 16;
 constexpr int kMaxHighBits = 8;
 constexpr int kApplyTileBits = 11;   // 2^11 complex128 = 32 KiB LDS
 constexpr int kSweepTileBits = 10;   // two tiles of 2^10 complex128 = 32 KiB
-constexpr int kApplyMaxUnion = 7;    // gate bits per pass (>= 4 coalescing bits)
+constexpr int kApplyMaxUnion = 8;    // gate bits per pass (>= 3 coalescing bits; 8 high bits max)
 constexpr double kGsTol = 1e-7;      // canonical-gauge Gram-Schmidt threshold
 constexpr double kPhaseTol = 1e-12;  // Ran two-site phase fix skipped below this
 
