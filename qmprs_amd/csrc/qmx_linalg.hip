@@ -835,7 +835,7 @@ void launch_jacobi(double2* a, double2* v, size_t stride, const int* ncols_dev, 
                      ncols_max, error);
 }
 
-constexpr int kJgridRowsLds = 131072;   // LDS bytes of V rows per block in jgrid_rows_kernel
+constexpr int kJgridRowsLds = 65536;   // LDS bytes of V rows per block in jgrid_rows_kernel (4 rows of 1024: all CUs busy; 8 or 2 rows measured slower)
 
 // second A buffer, the rotation log of one sweep, two sets of round inputs
 // (diagonal, pair entries), scale, conv, rot
