@@ -92,6 +92,13 @@ def test_grouped_workspace_queries():
                                             2) == 0      # truncating bond dimension above 512
 
 
+def test_bond_groups_are_runs_in_order():
+    from qmprs_amd.engine import bond_groups
+    assert bond_groups([4, 4, 16, 4]) == ([0, 2, 3], [2, 1, 1], [4, 16, 4])
+    assert bond_groups([8]) == ([0], [1], [8])
+    assert bond_groups([]) == ([], [], [])
+
+
 def test_encode_config_layout_matches_c_struct():
     mod, _ = _lib()
     # 7 ints then 2 doubles: doubles start at offset 32 (8-byte aligned)
