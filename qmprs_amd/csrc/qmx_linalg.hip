@@ -80,6 +80,18 @@ __global__ void gram_reduce_kernel(const double2* part, int nchunks, const int* 
 // J_P^H A_PQ J_Q, independently of every other block.  V accumulates J.
 constexpr int kJacobiMaxPairs = 512;
 
+// A pair is rotated only if its off-diagonal entry exceeds kJacobiAbsTol times
+// the trace (and 1e-15 of the geometric mean of its diagonal entries).  The
+// Gram matrix carries rounding errors of order eps * trace from its own
+// formation, and every rotation that involves a large eigenvalue leaves
+// errors of that order in the entries it touches; with a floor well below
+// that level the rotations inside a cluster of numerically zero eigenvalues
+// (rank-deficient Gram matrices: fewer rows than columns) regenerate each
+// other's entries and a sweep may never come out clean.  A quarter of eps
+// ends those sweeps and leaves the kept subspace as accurate as the Gram
+// matrix it comes from.
+constexpr double kJacobiAbsTol = 0.25 * 2.220446049250313e-16;
+
 __global__ __launch_bounds__(512) void jacobi_kernel(double2* a_all, double2* v_all, size_t stride,
                                                      const int* ncols_dev, int ld, int* error) {
   __shared__ int pp[kJacobiMaxPairs], qq[kJacobiMaxPairs];
@@ -104,7 +116,7 @@ __global__ __launch_bounds__(512) void jacobi_kernel(double2* a_all, double2* v_
   __syncthreads();
   if (n < 2) return;
   const int np = n / 2;
-  const double abs_tol = 1e-17 * s_scale + 1e-300;
+  const double abs_tol = kJacobiAbsTol * s_scale + 1e-300;
   bool converged = false;
   for (int sweep = 0; sweep < 40; ++sweep) {
     __syncthreads();
@@ -228,7 +240,7 @@ __global__ __launch_bounds__(1024) void jacobi_lds_sweep_kernel(double2* a_all, 
   }
   __syncthreads();
   const int np = n / 2;
-  const double abs_tol = 1e-17 * s_scale + 1e-300;
+  const double abs_tol = kJacobiAbsTol * s_scale + 1e-300;
   // a sweep would rotate nothing iff no off-diagonal entry passes the rotation
   // test now: then the state is converged and the sweep is not run
   for (int e = tid; e < ntri; e += nt) {
@@ -531,7 +543,7 @@ __global__ __launch_bounds__(256) void jgrid_round_kernel(const double2* ain_all
   double2* ao = aout_all + (size_t)b * stride;
   const int P = (int)(e / qs), Qh = (int)(e % qs);
   const int rr = r % (n - 1), rrn = rn % (n - 1);
-  const double tol = 1e-17 * scale[b] + 1e-300;
+  const double tol = kJacobiAbsTol * scale[b] + 1e-300;
   const double* dg = diag_in + (size_t)b * ld;
   const double2* of = off_in + (size_t)b * max_pairs;
   int p0, p1;
@@ -654,7 +666,7 @@ __global__ __launch_bounds__(256) void jgrid_check_kernel(const double2* acur_al
   if (conv[b]) return;
   const int n = ncols_dev[b];
   const double2* a = acur_all + (size_t)b * stride;
-  const double tol = 1e-17 * scale[b] + 1e-300;
+  const double tol = kJacobiAbsTol * scale[b] + 1e-300;
   bool pass = false;
   for (size_t e = blockIdx.x * (size_t)blockDim.x + threadIdx.x; e < (size_t)n * n;
        e += (size_t)gridDim.x * blockDim.x) {

@@ -83,6 +83,25 @@ def test_chi_truncate_matches_oracle(gpu, n, chi, kind, seed):
         np.testing.assert_allclose(sp[i, :k], lam[:k], rtol=1e-9, atol=1e-13 * lam[0])
 
 
+def test_chi_truncate_rank_deficient_grams(gpu):
+    """N=20 states from bond-96 random MPSs truncated to chi=64 (the bench
+    workload's input).  Left of bond 7 every Gram matrix has more columns than
+    rows (16 x 16 of rank 4 at bond 2), so most of its eigenvalues are
+    numerically zero; the Jacobi must still converge, and each truncated state
+    matches the oracle."""
+    eng = _engine()
+    n, chi = 20, 64
+    svs = np.stack([O.random_mps_state(n, 96, s) for s in (1, 2, 3)])
+    _, target, _ = eng.chi_truncate(eng.as_device_states(svs), chi, spectra=True)
+    torch.cuda.synchronize()
+    for b in range(len(svs)):
+        ref, _ = O.chi_truncate(O.ket(svs[b]), n, chi)
+        got = target[b].cpu().numpy()
+        err = np.linalg.norm(got - ref) / np.linalg.norm(ref)
+        print(f"state {b} rel.err={err:.3e}")
+        assert err < 1e-10
+
+
 def test_chi_truncate_grid_batch_of_unequal_gram_sizes(gpu):
     """Grid Jacobi over a batch whose Gram matrices differ in size at every
     bond (600 x 600 beside at most 192 x 192) and converge after different
