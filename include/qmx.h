@@ -73,6 +73,13 @@ const char* qmx_last_error(const qmx_handle* h);
  * LDS in tile order.  1 (default) does so when the tile's contiguous low run
  * is at least 2^7 amplitudes, 2 from 2^3, 0 never.  Results are identical. */
 #define QMX_OPT_STAGED_WINDOWS 5
+/* QMX_OPT_SWEEP_RELAXATION: omega * 1000 for the sweeps' gate update,
+ * 1000..1950.  1000 (default) is the reference's update G' = conj(polar(E))
+ * (sequential.py:473-478); omega > 1 replaces it by the over-relaxed
+ * polar(G' + (omega - 1)(G' - G)), G the gate before the update, which
+ * reaches a given fidelity in fewer sweeps but no longer follows the
+ * reference's gate trajectory. */
+#define QMX_OPT_SWEEP_RELAXATION 6
 int qmx_set_option(qmx_handle* h, int option, int value);
 
 /* Phase timers (no reference counterpart: instrumentation for benchmarks).

@@ -657,12 +657,23 @@ def circuit_state(num_sites: int, gate_list) -> np.ndarray:
     return psi
 
 
-def sweep_once(target: np.ndarray, num_sites: int, gate_list, polar: str = "lapack"):
+def relaxed_update(g_new: np.ndarray, g_old: np.ndarray, omega: float) -> np.ndarray:
+    """Over-relaxed gate update (no reference counterpart; the reference's
+    update is omega = 1): polar(G' + (omega - 1)(G' - G)), with G' the plain
+    update conj(polar(E)) and G the gate before it."""
+    if omega == 1.0:
+        return g_new
+    return polar_unitary(g_new + (omega - 1.0) * (g_new - g_old))
+
+
+def sweep_once(target: np.ndarray, num_sites: int, gate_list, polar: str = "lapack",
+               omega: float = 1.0):
     """One ``_sweep_unitary_layers`` pass (sequential.py:400-507).
 
     phi <- G^dag phi; E = <T'| phi>_rest; G' = conj(polar(E)); T' <- G'^dag T'.
     Returns the updated gate list and the overlap <T|C'|0> after the last
-    update, sum_{o,i} G'[o,i] E[o,i].
+    update, sum_{o,i} G'[o,i] E[o,i].  ``omega`` != 1 replaces G' by
+    ``relaxed_update`` (not part of the reference).
     """
     gl = list(gate_list)
     phi = circuit_state(num_sites, gl)
@@ -673,17 +684,18 @@ def sweep_once(target: np.ndarray, num_sites: int, gate_list, polar: str = "lapa
         phi = apply_gate(phi, num_sites, g.conj().T, site, width)
         env = environment(tp, phi, num_sites, site, width)
         pu = polar_unitary(env) if polar == "lapack" else polar_unitary_jacobi(env)
-        g_new = np.conj(pu)
+        g_new = relaxed_update(np.conj(pu), g, omega)
         tp = apply_gate(tp, num_sites, g_new.conj().T, site, width)
         gl[idx] = (site, width, g_new)
         overlap = np.sum(g_new * env)
     return gl, overlap
 
 
-def optimize_gate_list(target, num_sites, gate_list, num_sweeps, trace=None, polar="lapack"):
+def optimize_gate_list(target, num_sites, gate_list, num_sweeps, trace=None, polar="lapack",
+                       omega: float = 1.0):
     """``_optimize_unitary_layers`` (sequential.py:509-541)."""
     for _ in range(num_sweeps):
-        gate_list, ov = sweep_once(target, num_sites, gate_list, polar)
+        gate_list, ov = sweep_once(target, num_sites, gate_list, polar, omega)
         if trace is not None:
             trace.append(ov)
     return gate_list

@@ -201,6 +201,7 @@ QMX_OPT_SPLIT_SWEEPS = 2
 QMX_OPT_WINDOW_BLOCKS_PER_CU = 3
 QMX_OPT_FWD_BLOCKS_PER_CU = 4
 QMX_OPT_STAGED_WINDOWS = 5
+QMX_OPT_SWEEP_RELAXATION = 6
 
 
 def set_option(option: int, value: int) -> None:
@@ -230,6 +231,15 @@ def set_split_sweeps(enable: bool) -> None:
     """Run batches of >= 2 states as two interleaved half-batches (default on)."""
     h = handle()
     check(load().qmx_set_option(h, QMX_OPT_SPLIT_SWEEPS, 1 if enable else 0), h)
+
+
+def set_sweep_relaxation(omega: float) -> None:
+    """Gate update of the sweeps: omega = 1 (default) is the reference's
+    G' = conj(polar(E)); 1 < omega < 1.95 the over-relaxed
+    polar(G' + (omega - 1)(G' - G)), which reaches a given fidelity in fewer
+    sweeps but leaves the reference's gate trajectory."""
+    h = handle()
+    check(load().qmx_set_option(h, QMX_OPT_SWEEP_RELAXATION, int(round(1000 * omega))), h)
 
 
 def set_sweep_mode(mode: str) -> None:

@@ -35,6 +35,7 @@ struct qmx_handle {
   int win_blocks_per_cu = 16;           // QMX_OPT_WINDOW_BLOCKS_PER_CU
   int fwd_blocks_per_cu = 4;            // QMX_OPT_FWD_BLOCKS_PER_CU
   int staged_windows = 1;               // QMX_OPT_STAGED_WINDOWS (0, 1, 2)
+  int relax_milli = 1000;               // QMX_OPT_SWEEP_RELAXATION (omega * 1000)
   hipStream_t aux[2] = {nullptr, nullptr};   // the two half-batch jobs
   hipEvent_t pass_ev[2] = {nullptr, nullptr};
   hipEvent_t join_ev[2] = {nullptr, nullptr};
@@ -834,6 +835,7 @@ void job_window_finalize(qmx_handle* h, SweepJob<S>& J, const std::vector<Window
   fa.part = J.L.part;
   fa.overlap = J.overlap ? J.overlap + s : nullptr;
   fa.overlap_stride = J.nsweeps;
+  fa.omega_m1 = (h->relax_milli - 1000) * 1e-3;
   fa.pstride = 1;
   if (fa.nblk > kWinMaxBlocks) {
     // large grids (few large states): pre-sum groups of 32 partials with a
@@ -866,6 +868,7 @@ void job_step_sweep(qmx_handle* h, SweepJob<S>& J, int s) {
     a.counter = J.L.counter;
     a.overlap = J.overlap ? J.overlap + s : nullptr;
     a.overlap_stride = J.nsweeps;
+    a.omega_m1 = (h->relax_milli - 1000) * 1e-3;
     launch_sweep_step(a, nblk, J.batch, J.st);
   }
 }
@@ -1123,6 +1126,11 @@ int qmx_set_option(qmx_handle* h, int option, int value) {
   if (option == QMX_OPT_FWD_BLOCKS_PER_CU) {
     if (value < 1 || value > 16) return fail(h, QMX_ERR_INVALID, "forward blocks per CU: 1..16");
     h->fwd_blocks_per_cu = value;
+    return QMX_OK;
+  }
+  if (option == QMX_OPT_SWEEP_RELAXATION) {
+    if (value < 1000 || value > 1950) return fail(h, QMX_ERR_INVALID, "sweep relaxation: 1000..1950 (omega * 1000)");
+    h->relax_milli = value;
     return QMX_OK;
   }
   if (option == QMX_OPT_SPLIT_SWEEPS) {

@@ -343,6 +343,33 @@ QMX_DEV void polar4(int d, const double2* e, double2* out, double* nuc) {
     }
 }
 
+// Over-relaxed local update (no reference counterpart).  g holds the plain
+// update conj(polar(E)) of a gate whose value before the update was g_old;
+// with omega_m1 = omega - 1 its d x d block becomes
+//   polar(g + omega_m1 (g - g_old)),
+// a step past the local optimum along the direction of the update, projected
+// back onto the unitaries (omega = 1 leaves g as it is).  Returns the
+// overlap after the update, |sum_{o,i} g[o][i] E[o][i]| over the block.
+QMX_DEV double relax_update(int d, double omega_m1, const double2* g_old, double2* g, const double2* e) {
+  double2 m[16], u[16];
+#pragma unroll
+  for (int k = 0; k < 16; ++k) {
+    const bool in = (k / 4) < d && (k % 4) < d;
+    m[k] = in ? cadd(g[k], cscale(csub(g[k], g_old[k]), omega_m1)) : cz();
+  }
+  double unused = 0.0;
+  polar4(d, m, u, &unused);
+  double2 ov = cz();
+#pragma unroll
+  for (int k = 0; k < 16; ++k) {
+    if ((k / 4) < d && (k % 4) < d) {
+      g[k] = u[k];
+      ov = cfma(u[k], e[k], ov);
+    }
+  }
+  return cabs_(ov);
+}
+
 // --------------------------------------------- Householder null space -----
 // Trailing columns of the full Householder QR of rows^T (n x m, n <= 4,
 // m < n), LAPACK zgeqr2 + zung2r conventions.  rows is m x n (stride 4).

@@ -58,6 +58,7 @@ struct SweepArgs {
   int* counter;
   double* overlap;     // overlap[b * overlap_stride], may be nullptr
   int overlap_stride;
+  double omega_m1;     // over-relaxation of the gate update minus one (0: plain)
 };
 
 // Host-computed tile geometry.  A tile is a set P of t bits (gate bits plus
@@ -136,6 +137,7 @@ struct WinFinArgs {
   const double* part;
   double* overlap;
   int overlap_stride;
+  double omega_m1;     // over-relaxation of the gate update minus one (0: plain)
 };
 
 struct Chain2Args {

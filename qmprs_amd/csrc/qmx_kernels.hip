@@ -532,8 +532,14 @@ __global__ __launch_bounds__(256) void sweep_step_kernel(SweepArgs<S> a) {
       polar4(d, em, u, &nuc);
       for (int j = 0; j < 16; ++j) {
         int o = j / 4, i = j % 4;
-        a.gates[gic * 16 + j] = (o < d && i < d) ? cconj(u[j]) : cz();
+        u[j] = (o < d && i < d) ? cconj(u[j]) : cz();
       }
+      if (a.omega_m1 != 0.0) {
+        double2 old[16];
+        for (int j = 0; j < 16; ++j) old[j] = a.gates[gic * 16 + j];
+        nuc = relax_update(d, a.omega_m1, old, u, em);
+      }
+      for (int j = 0; j < 16; ++j) a.gates[gic * 16 + j] = u[j];
       if (a.overlap) a.overlap[(size_t)b * a.overlap_stride] = nuc;
       a.counter[b] = 0;
     }

@@ -637,7 +637,10 @@ __global__ __launch_bounds__(256) void window_finalize_kernel(WinFinArgs a) {
       double nuc = 0.0;
       polar4(d, e, u, &nuc);
 #pragma unroll
-      for (int k = 0; k < 16; ++k) gnew[k] = cconj(u[k]);
+      for (int k = 0; k < 16; ++k) u[k] = cconj(u[k]);
+      if (a.omega_m1 != 0.0) nuc = relax_update(d, a.omega_m1, gm[j], u, e);
+#pragma unroll
+      for (int k = 0; k < 16; ++k) gnew[k] = u[k];
       if (a.s0 + j == kb - 1 && a.overlap) a.overlap[(size_t)b * a.overlap_stride] = nuc;
     }
     __syncthreads();

@@ -14,7 +14,7 @@ __all__ = ["Sequential"]
 
 import torch
 
-from qmprs_amd import engine
+from qmprs_amd import _lib, engine
 from qmprs_amd.primitives.mps import MPS, UnitaryLayer
 from qmprs_amd.synthesis.mps_encoding.base import MPSEncoder
 
@@ -28,7 +28,11 @@ class Sequential(MPSEncoder):
     of the sweep's dense states;
     ``schedule`` ("DallOall" default, the reference's method; "IterDiOi" or
     "IterDiOall", the iterative variants its notebook names) -- see
-    ``qmprs_amd.engine.schedule_layers``.
+    ``qmprs_amd.engine.schedule_layers``;
+    ``sweep_relaxation`` (1.0 default, the reference's gate update; 1 < omega
+    < 1.95) -- over-relaxed sweep updates polar(G' + (omega - 1)(G' - G)),
+    which reach a given fidelity in fewer sweeps but leave the reference's
+    gate trajectory.
     """
 
     def __init__(self, circuit_framework) -> None:
@@ -133,6 +137,18 @@ class Sequential(MPSEncoder):
         if schedule not in engine.SCHEDULES:
             raise ValueError(f"schedule must be one of {engine.SCHEDULES}")
         sweep_dtype = kwargs.get("sweep_dtype", torch.complex64)
+        omega = kwargs.get("sweep_relaxation", 1.0)
+        if isinstance(omega, bool) or not isinstance(omega, (int, float)) or not 1.0 <= omega <= 1.95:
+            raise ValueError("sweep_relaxation must be a number in [1, 1.95].")
+        if omega == 1.0:
+            return self._encode(mps, num_layers, num_sweeps, schedule, sweep_dtype)
+        _lib.set_sweep_relaxation(float(omega))
+        try:
+            return self._encode(mps, num_layers, num_sweeps, schedule, sweep_dtype)
+        finally:
+            _lib.set_sweep_relaxation(1.0)
+
+    def _encode(self, mps: MPS, num_layers: int, num_sweeps: int, schedule: str, sweep_dtype):
         if schedule != "DallOall":
             return self._scheduled_circuit(mps, num_layers, num_sweeps, schedule, sweep_dtype)
         return self._sequential_unitary_circuit(mps, num_layers, num_sweeps, sweep_dtype)

@@ -253,6 +253,49 @@ def test_sweep_matches_oracle(gpu, dtype, n, layers, sweeps, seed):
         assert serr < 1e-5
 
 
+@pytest.mark.parametrize("n,layers,sweeps,seed", [(6, 3, 3, 1), (9, 3, 3, 2)])
+def test_relaxed_sweep_matches_oracle(gpu, n, layers, sweeps, seed):
+    """Over-relaxed gate update (omega = 1.6; an option with no reference
+    counterpart) on the per-gate path (N < 8) and the windowed path (N >= 8)
+    against the oracle's relaxed sweep, complex128."""
+    from qmprs_amd import _lib
+    eng = _engine()
+    omega = 1.6
+    sv = O.ket(O.random_uniform_state(n, seed))
+    ref_t, _ = O.chi_truncate(sv, n, 2 ** n)
+    ref_layers = O.get_unitary_layers(ref_t, n, layers)
+    gl0 = O.layer_gate_list(ref_layers)
+    trace = []
+    gl = O.optimize_gate_list(ref_t, n, gl0, sweeps, trace, polar="jacobi", omega=omega)
+    ref_state = O.circuit_state(n, gl)
+    ref_fid = abs(np.vdot(sv, ref_state))
+    g, w = eng.layers_to_arrays(ref_layers, n)
+    gates = torch.from_numpy(g).to(gpu).unsqueeze(0).contiguous()
+    widths = torch.from_numpy(w).to(gpu).unsqueeze(0).contiguous()
+    used = torch.tensor([g.shape[0]], dtype=torch.int32, device=gpu)
+    target = torch.from_numpy(ref_t).to(gpu).unsqueeze(0).contiguous()
+    psi_ref = torch.from_numpy(sv).to(gpu).unsqueeze(0).contiguous()
+    _lib.set_sweep_relaxation(omega)
+    try:
+        res = eng.sweep(target, gates, widths, used, sweeps, psi_ref=psi_ref, circuit_out=True)
+        torch.cuda.synchronize()
+    finally:
+        _lib.set_sweep_relaxation(1.0)
+    fid = abs(res.fidelity[0, -1].item())
+    circ = res.circuit[0].cpu().numpy().astype(np.complex128)
+    got = _gate_list_from_device(gates[0].cpu().numpy(), widths[0].cpu().numpy(), g.shape[0])
+    gdiff = max(np.linalg.norm(a[2] - b[2]) for a, b in zip(got, gl))
+    serr = np.linalg.norm(circ - ref_state)
+    print(f"n={n} omega={omega} fid {fid:.12f} ref {ref_fid:.12f} gate err {gdiff:.3e} state err {serr:.3e}")
+    for _, _, gm in got:
+        d = gm.shape[0]
+        assert np.abs(gm @ gm.conj().T - np.eye(d)).max() < 1e-12
+    np.testing.assert_allclose(res.overlap[0].cpu().numpy(), np.abs(trace), rtol=1e-10)
+    assert abs(fid - ref_fid) < 1e-10
+    assert gdiff < 1e-8
+    assert serr < 1e-9
+
+
 @pytest.mark.parametrize("n,chi,layers,sweeps,seed,sdtype", [
     (6, 64, 4, 3, 0, torch.complex128), (8, 8, 5, 4, 1, torch.complex64),
     (10, 512, 5, 3, 2, torch.complex64), (12, 16, 4, 2, 3, torch.complex64)])
@@ -549,6 +592,34 @@ def test_schedule_batch_early_exit_and_sequential_api(gpu):
     assert abs(got - enc.last_info["fidelity"]) < 1e-9
     with pytest.raises(ValueError):
         enc.prepare_state(rnd, 8, num_layers=2, schedule="Brickwall")
+
+
+def test_sequential_sweep_relaxation(gpu):
+    """``sweep_relaxation`` through the Sequential API: the circuit is the
+    oracle's over-relaxed encoding (complex128 sweeps), the option is reset
+    afterwards (a plain encode follows the reference's update again), and
+    values outside [1, 1.95] are rejected."""
+    from qmprs_amd import _lib
+    from qmprs_amd.circuit import GateCircuit
+    from qmprs_amd.synthesis.mps_encoding import Sequential
+    n, chi, layers, sweeps = 9, 16, 3, 3
+    sv = O.ket(O.random_mps_state(n, 24, 5))
+    enc = Sequential(circuit_framework=GateCircuit)
+    target, _ = O.chi_truncate(sv, n, chi)
+    gl0 = O.layer_gate_list(O.get_unitary_layers(target, n, layers))
+    for omega in (1.7, 1.0):
+        circ = enc.prepare_state(sv, chi, num_layers=layers, num_sweeps=sweeps,
+                                 sweep_relaxation=omega, sweep_dtype=torch.complex128)
+        gl = O.optimize_gate_list(target, n, gl0, sweeps, polar="jacobi", omega=omega)
+        ref = O.circuit_state(n, gl)
+        err = np.linalg.norm(circ.get_statevector() - ref)
+        print(f"omega {omega}: state err {err:.3e}")
+        assert err < 1e-9
+    for bad in (0.9, 2.0, True, "1.5"):
+        with pytest.raises(ValueError):
+            enc.prepare_state(sv, chi, num_layers=1, sweep_relaxation=bad)
+    with pytest.raises(ValueError):
+        _lib.set_sweep_relaxation(2.5)
 
 
 def test_large_grid_window_sweep_equals_step_sweep(gpu):
