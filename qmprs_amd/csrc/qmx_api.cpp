@@ -1064,6 +1064,16 @@ void qmx_set_error_text(qmx_handle* h, const char* msg) {
   if (h) h->err = msg ? msg : "";
 }
 int qmx_handle_device(const qmx_handle* h) { return h ? h->device : 0; }
+// the qmx_set_option settings of `from`, for handles that act on its behalf
+void qmx_copy_options(const qmx_handle* from, qmx_handle* to) {
+  if (!from || !to) return;
+  to->sweep_mode = from->sweep_mode;
+  to->split_sweeps = from->split_sweeps;
+  to->win_blocks_per_cu = from->win_blocks_per_cu;
+  to->fwd_blocks_per_cu = from->fwd_blocks_per_cu;
+  to->staged_windows = from->staged_windows;
+  to->relax_milli = from->relax_milli;
+}
 
 // =================================================================== C-ABI ==
 extern "C" {

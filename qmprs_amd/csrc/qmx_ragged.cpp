@@ -29,6 +29,7 @@
 // handle internals shared with qmx_api.cpp
 void qmx_set_error_text(qmx_handle* h, const char* msg);
 int qmx_handle_device(const qmx_handle* h);
+void qmx_copy_options(const qmx_handle* from, qmx_handle* to);
 
 namespace {
 
@@ -151,6 +152,7 @@ int qmx_encode_ragged(qmx_handle* h, int count, const qmx_ragged_state* states, 
     if (hipSetDevice(device) != hipSuccess) return fail(QMX_ERR_HIP, "hipSetDevice");
     qmx_handle* wh = nullptr;
     if (qmx_create(device, &wh) != QMX_OK) return fail(QMX_ERR_HIP, "worker handle");
+    qmx_copy_options(h, wh);   // the caller's sweep settings apply to every bucket
     hipStream_t ws = nullptr;
     if (hipStreamCreateWithFlags(&ws, hipStreamNonBlocking) != hipSuccess ||
         hipStreamWaitEvent(ws, start, 0) != hipSuccess) {

@@ -408,24 +408,36 @@ def test_split_sweeps_equal_single_stream(gpu, batch):
     assert (one.fidelity - two.fidelity).abs().max().item() < 1e-12
 
 
-def test_encode_ragged_equals_bucket_encodes(gpu):
+@pytest.mark.parametrize("omega", [1.0, 1.7])
+def test_encode_ragged_equals_bucket_encodes(gpu, omega):
     """qmx_encode_ragged: states of different (N, chi), shuffled, give what
-    qmx_encode gives on each (N, chi) bucket (bit for bit: same batches)."""
+    qmx_encode gives on each (N, chi) bucket (bit for bit: same batches);
+    the caller handle's sweep relaxation reaches the bucket workers."""
+    from qmprs_amd import _lib
     eng = _engine()
     spec = [(9, 8), (11, 16), (9, 8), (12, 8), (11, 16), (10, 4), (9, 8)]
     svs = [O.random_mps_state(n, 2 * chi, 70 + i).astype(np.complex64) for i, (n, chi) in enumerate(spec)]
     L, S = 3, 2
-    res = eng.encode_ragged(svs, [c for _, c in spec], L, S, max_concurrency=3)
     buckets = {}
     for i, (n, chi) in enumerate(spec):
         buckets.setdefault((n, chi), []).append(i)
-    for (n, chi), idx in buckets.items():
-        ref = eng.encode(np.stack([svs[i] for i in idx]), chi, L, S)
+    _lib.set_sweep_relaxation(omega)
+    try:
+        res = eng.encode_ragged(svs, [c for _, c in spec], L, S, max_concurrency=3)
+        refs = {key: eng.encode(np.stack([svs[i] for i in idx]), key[1], L, S)
+                for key, idx in buckets.items()}
+    finally:
+        _lib.set_sweep_relaxation(1.0)
+    for key, idx in buckets.items():
+        ref = refs[key]
         for k, i in enumerate(idx):
             assert res[i].layers_used == int(ref.layers_used[k])
             assert torch.equal(res[i].gates, ref.gates[k])
             assert torch.equal(res[i].widths, ref.widths[k])
             assert torch.equal(res[i].fidelity, ref.fidelity[k])
+    if omega != 1.0:   # and the relaxed gates differ from the plain ones
+        plain = eng.encode(np.stack([svs[i] for i in buckets[(11, 16)]]), 16, L, S)
+        assert not torch.equal(plain.gates, refs[(11, 16)].gates)
 
 
 def test_encode_grouped_one_group_equals_encode(gpu):
