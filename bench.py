@@ -163,12 +163,17 @@ def relaxed_leg(sv_dev, plain_res, wl, args) -> dict:
         for _ in range(args.warmup):
             res = engine.encode(sv_dev, chi, L, S, sweep_dtype=torch.complex64, goal=goal)
         torch.cuda.synchronize()
+        # phase timers on, as in the plain leg's timed steps (same overhead)
+        _lib.profile_enable(True)
+        _lib.profile_read()
         t0 = time.perf_counter()
         for _ in range(args.steps):
             res = engine.encode(sv_dev, chi, L, S, sweep_dtype=torch.complex64, goal=goal)
         torch.cuda.synchronize()
         t1 = time.perf_counter()
+        prof = _lib.profile_read()
     finally:
+        _lib.profile_enable(False)
         _lib.set_sweep_relaxation(1.0)
     fid = res.fidelity[:, -1].abs()
     return {
@@ -180,6 +185,8 @@ def relaxed_leg(sv_dev, plain_res, wl, args) -> dict:
         "mean_sweeps": float(res.sweeps_done.double().mean()),
         "max_sweeps": int(res.sweeps_done.max()),
         "fidelity_mean": float(fid.mean()),
+        "phases_ms_per_step": {p: {"ms": round(v[0] / args.steps, 2), "launches": int(v[1] / args.steps)}
+                               for p, v in prof.items()},
     }
 
 
