@@ -478,7 +478,10 @@ QMX_DEV double4 jgrid_rotation(double al, double be, double2 g, double tol, bool
   double c = 1.0, sn = 0.0;
   double2 em = make_double2(1.0, 0.0);
   rotated = false;
-  if (ag > tol && ag > 1e-15 * sqrt(fabs(al * be))) {
+  // one comparison against the larger threshold (the same test as
+  // ag > tol && ag > 1e-15 sqrt|al be|): al and be are used on every path,
+  // so their loads are not sunk behind the block loads that follow them
+  if (ag > fmax(tol, 1e-15 * sqrt(fabs(al * be)))) {
     const double tau = (be - al) / (2.0 * ag);
     const double t = (tau >= 0 ? 1.0 : -1.0) / (fabs(tau) + sqrt(1.0 + tau * tau));
     c = 1.0 / sqrt(1.0 + t * t);
@@ -548,24 +551,28 @@ __global__ __launch_bounds__(256) void jgrid_round_kernel(const double2* ain_all
   const double2* of = off_in + (size_t)b * max_pairs;
   int p0, p1;
   circle_pair_rr(n, rr, P, p0, p1);
+  static_assert(kJgridQPerThread == 1, "one column pair per thread");
+  const int Q = Qh;
+  int q0, q1;
+  circle_pair_rr(n, rr, Q, q0, q1);
+  // load order: the rotation inputs (small, cached) first, then the block of
+  // A; the rotations only wait for the first loads and are computed while
+  // the block is in flight (loads complete in issue order)
+  const double dp0 = dg[p0], dp1 = dg[p1], dq0 = dg[q0], dq1 = dg[q1];
+  const double2 oP = of[P], oQ = of[Q];
+  const double2 a00 = a[(size_t)p0 * ld + q0], a01 = a[(size_t)p0 * ld + q1];
+  const double2 a10 = a[(size_t)p1 * ld + q0], a11 = a[(size_t)p1 * ld + q1];
   bool rotated;
-  const double4 RP = jgrid_rotation(dg[p0], dg[p1], of[P], tol, rotated);
+  const double4 RP = jgrid_rotation(dp0, dp1, oP, tol, rotated);
+  const double4 RQ = jgrid_rotation(dq0, dq1, oQ, tol, rotated);
   if (Qh == 0) log_all[(size_t)b * log_stride + (size_t)r * max_pairs + P] = RP;
   const double2 K00 = make_double2(RP.x, 0.0), K01 = make_double2(RP.y, 0.0);
   const double2 K10 = cscale(make_double2(RP.z, RP.w), -RP.y);
   const double2 K11 = cscale(make_double2(RP.z, RP.w), RP.x);
-#pragma unroll
-  for (int h = 0; h < kJgridQPerThread; ++h) {
-    const int Q = Qh + h * qs;
-    if (Q >= np) break;
-    int q0, q1;
-    circle_pair_rr(n, rr, Q, q0, q1);
-    const double4 RQ = jgrid_rotation(dg[q0], dg[q1], of[Q], tol, rotated);
+  {
     const double2 J00 = make_double2(RQ.x, 0.0), J01 = make_double2(RQ.y, 0.0);
     const double2 J10 = cscale(make_double2(RQ.z, RQ.w), -RQ.y);
     const double2 J11 = cscale(make_double2(RQ.z, RQ.w), RQ.x);
-    const double2 a00 = a[(size_t)p0 * ld + q0], a01 = a[(size_t)p0 * ld + q1];
-    const double2 a10 = a[(size_t)p1 * ld + q0], a11 = a[(size_t)p1 * ld + q1];
     const double2 b00 = cadd(cmul(a00, J00), cmul(a01, J10));
     const double2 b01 = cadd(cmul(a00, J01), cmul(a01, J11));
     const double2 b10 = cadd(cmul(a10, J00), cmul(a11, J10));
