@@ -293,7 +293,9 @@ __global__ __launch_bounds__(1024) void jacobi_lds_sweep_kernel(double2* a_all, 
       const double ag = cabs_(g);
       double c = 1.0, sn = 0.0;
       double2 em = make_double2(1.0, 0.0);
-      if (ag > abs_tol && ag > 1e-15 * sqrt(fabs(al * be))) {
+      // one comparison against the larger threshold: the diagonal reads are
+      // issued together with the off-diagonal one
+      if (ag > fmax(abs_tol, 1e-15 * sqrt(fabs(al * be)))) {
         const double tau = (be - al) / (2.0 * ag);
         const double t = (tau >= 0 ? 1.0 : -1.0) / (fabs(tau) + sqrt(1.0 + tau * tau));
         c = 1.0 / sqrt(1.0 + t * t);
