@@ -50,8 +50,10 @@ int qmx_destroy(qmx_handle* h);
 const char* qmx_last_error(const qmx_handle* h);
 
 /* Tuning options (no reference counterpart).
- * QMX_OPT_SWEEP_MODE: QMX_SWEEP_AUTO (windowed passes for N >= 8, else one
- * pass per gate), QMX_SWEEP_STEP (one pass per gate) or QMX_SWEEP_WINDOW. */
+ * QMX_OPT_SWEEP_MODE: QMX_SWEEP_AUTO (windowed passes for N >= 10, else one
+ * pass per gate), QMX_SWEEP_STEP (one pass per gate) or QMX_SWEEP_WINDOW
+ * (windowed passes; below N = 10, where a tile holds too few rest indices for
+ * the pass kernel's four waves, it also runs one pass per gate). */
 #define QMX_OPT_SWEEP_MODE 1
 #define QMX_SWEEP_AUTO 0
 #define QMX_SWEEP_STEP 1
@@ -80,6 +82,15 @@ const char* qmx_last_error(const qmx_handle* h);
  * reaches a given fidelity in fewer sweeps but no longer follows the
  * reference's gate trajectory. */
 #define QMX_OPT_SWEEP_RELAXATION 6
+/* Failure injection (host-forced error paths for tests; every kernel still
+ * runs to its normal end):
+ * QMX_OPT_JACOBI_SWEEP_CAP: at most this many sweeps in the chi-truncation's
+ * Jacobi eigensolvers (0, the default: their own limits of 30 / 40 / 80); a
+ * state that has not converged then fails with QMX_ERR_NO_CONVERGENCE.
+ * QMX_OPT_FAIL_UNITARITY: 1 makes every generated gate fail the unitarity
+ * check of mps.py:837-839 (QMX_ERR_NOT_UNITARY); 0 (default) restores it. */
+#define QMX_OPT_JACOBI_SWEEP_CAP 7
+#define QMX_OPT_FAIL_UNITARITY 8
 int qmx_set_option(qmx_handle* h, int option, int value);
 
 /* Phase timers (no reference counterpart: instrumentation for benchmarks).

@@ -202,6 +202,8 @@ QMX_OPT_WINDOW_BLOCKS_PER_CU = 3
 QMX_OPT_FWD_BLOCKS_PER_CU = 4
 QMX_OPT_STAGED_WINDOWS = 5
 QMX_OPT_SWEEP_RELAXATION = 6
+QMX_OPT_JACOBI_SWEEP_CAP = 7
+QMX_OPT_FAIL_UNITARITY = 8
 
 
 def set_option(option: int, value: int) -> None:
@@ -243,6 +245,7 @@ def set_sweep_relaxation(omega: float) -> None:
 
 
 def set_sweep_mode(mode: str) -> None:
-    """'auto' (windowed passes for N >= 8), 'step' (one pass per gate) or 'window'."""
+    """'auto' (windowed passes for N >= 10), 'step' (one pass per gate) or
+    'window' (windowed passes; below N = 10 it also runs one pass per gate)."""
     h = handle()
     check(load().qmx_set_option(h, QMX_OPT_SWEEP_MODE, SWEEP_MODES[mode]), h)

@@ -36,6 +36,8 @@ struct qmx_handle {
   int fwd_blocks_per_cu = 4;            // QMX_OPT_FWD_BLOCKS_PER_CU
   int staged_windows = 1;               // QMX_OPT_STAGED_WINDOWS (0, 1, 2)
   int relax_milli = 1000;               // QMX_OPT_SWEEP_RELAXATION (omega * 1000)
+  int jacobi_sweep_cap = 0;             // QMX_OPT_JACOBI_SWEEP_CAP (0: the solvers' own limits)
+  bool fail_unitarity = false;          // QMX_OPT_FAIL_UNITARITY (failure injection)
   hipStream_t aux[2] = {nullptr, nullptr};   // the two half-batch jobs
   hipEvent_t pass_ev[2] = {nullptr, nullptr};
   hipEvent_t join_ev[2] = {nullptr, nullptr};
@@ -255,6 +257,7 @@ constexpr int kJacobiSweeps = 30;   // sweep launches of the LDS Jacobi per bond
 // sweeps of the grid Jacobi (2r > 512): up to 34 were needed on the Gram
 // matrices of N=26 random-MPS states of bond 1024 at chi = 512
 constexpr int kJacobiGridSweeps = 80;
+constexpr int kJacobiGlobalSweeps = 40;   // one-block Jacobi on global memory (2r <= 512)
 constexpr int kWinMaxBlocks = 256;
 constexpr int kWinMaxBlocksLarge = 1024;
 
@@ -492,15 +495,17 @@ int run_chi(qmx_handle* h, int n, int batch, int chi, double cutoff, const void*
     int nchunks = std::max(1, std::min(L.nchunks_max, rows / 256));
     launch_gram(y, len, L.ncur, i, nmax, L.part, nchunks, batch, st);
     launch_gram_reduce(L.part, nchunks, L.ncur, nmax, L.a, nm2, batch, st);
+    // sweep limits of the three eigensolvers; QMX_OPT_JACOBI_SWEEP_CAP lowers them
+    auto cap = [&](int limit) { return h->jacobi_sweep_cap > 0 ? std::min(limit, h->jacobi_sweep_cap) : limit; };
     if (L.jgrid) {
       std::vector<int> conv(batch);
-      if (launch_jacobi_grid(L.a, L.v, nm2, L.ncur, nmax, batch, L.jgrid, conv.data(), kJacobiGridSweeps,
+      if (launch_jacobi_grid(L.a, L.v, nm2, L.ncur, nmax, batch, L.jgrid, conv.data(), cap(kJacobiGridSweeps),
                              L.error, st) != 0)
         return fail(h, QMX_ERR_HIP, "grid Jacobi");
     } else if (L.jlog && jacobi_lds_setup(nmax) == hipSuccess) {
-      launch_jacobi_lds(L.a, L.v, nm2, L.ncur, nmax, batch, L.jlog, L.jstate, kJacobiSweeps, L.error, st);
+      launch_jacobi_lds(L.a, L.v, nm2, L.ncur, nmax, batch, L.jlog, L.jstate, cap(kJacobiSweeps), L.error, st);
     } else {
-      launch_jacobi(L.a, L.v, nm2, L.ncur, nmax, batch, L.error, st);
+      launch_jacobi(L.a, L.v, nm2, L.ncur, nmax, batch, cap(kJacobiGlobalSweeps), L.error, st);
     }
     double2* wi = L.w + static_cast<size_t>(i) * batch * nm2;
     launch_select(L.a, L.v, wi, nm2, L.ncur, nmax, chi, cutoff, L.kk + i, n + 1,
@@ -562,6 +567,7 @@ int run_generate_layer(qmx_handle* h, int n, int batch, int lmax, int layer, dou
   r.bonds = bonds;
   r.skip = skip;
   r.error = L.error;
+  r.unitary_atol = h->fail_unitarity ? -1.0 : 1e-8;
   launch_ran_gates(r, batch, st);
   QMX_LAUNCHED(h, "generate layer");
   return QMX_OK;
@@ -1141,6 +1147,16 @@ int qmx_set_option(qmx_handle* h, int option, int value) {
   if (option == QMX_OPT_SWEEP_RELAXATION) {
     if (value < 1000 || value > 1950) return fail(h, QMX_ERR_INVALID, "sweep relaxation: 1000..1950 (omega * 1000)");
     h->relax_milli = value;
+    return QMX_OK;
+  }
+  if (option == QMX_OPT_JACOBI_SWEEP_CAP) {
+    if (value < 0 || value > 1000) return fail(h, QMX_ERR_INVALID, "Jacobi sweep cap: 0..1000 (0: no cap)");
+    h->jacobi_sweep_cap = value;
+    return QMX_OK;
+  }
+  if (option == QMX_OPT_FAIL_UNITARITY) {
+    if (value != 0 && value != 1) return fail(h, QMX_ERR_INVALID, "fail unitarity: 0 or 1");
+    h->fail_unitarity = value != 0;
     return QMX_OK;
   }
   if (option == QMX_OPT_SPLIT_SWEEPS) {

@@ -167,6 +167,7 @@ struct RanArgs {
   int* bonds;          // optional [B][lmax][n]
   const int* skip;
   int* error;
+  double unitary_atol;  // allclose atol of the unitarity check (1e-8; < 0 fails every gate)
 };
 
 struct Amp0Args {
@@ -259,7 +260,7 @@ hipError_t jacobi_lds_setup(int ncols_max);
 void launch_jacobi_lds(double2* a, double2* v, size_t stride, const int* ncols_dev, int ncols_max,
                        int batch, double* log, int* state, int max_sweeps, int* error, hipStream_t st);
 void launch_jacobi(double2* a, double2* v, size_t stride, const int* ncols_dev, int ncols_max,
-                   int batch, int* error, hipStream_t);
+                   int batch, int max_sweeps, int* error, hipStream_t);
 void launch_select(const double2* a, const double2* v, double2* w, size_t stride,
                    const int* ncols_dev, int ncols_max, int chi, double cutoff, int* k_out,
                    int kstride, double* spectra, size_t spec_stride, int batch, hipStream_t);

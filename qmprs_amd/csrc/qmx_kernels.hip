@@ -808,7 +808,8 @@ __global__ __launch_bounds__(64) void ran_gates_kernel(RanArgs a) {
     }
     width = 2;
   }
-  // unitarity: allclose(M M^H, I, rtol=1e-5, atol=1e-8)
+  // unitarity: allclose(M M^H, I, rtol=1e-5, atol=1e-8); a negative atol
+  // (failure injection, set from the host) fails every gate
   const int d = width == 2 ? 4 : 2;
   bool ok = true;
   for (int i = 0; i < d; ++i)
@@ -817,7 +818,7 @@ __global__ __launch_bounds__(64) void ran_gates_kernel(RanArgs a) {
       for (int kk = 0; kk < d; ++kk) s = cfma(m[i * 4 + kk], cconj(m[j * 4 + kk]), s);
       double ref = (i == j) ? 1.0 : 0.0;
       double diff = sqrt((s.x - ref) * (s.x - ref) + s.y * s.y);
-      if (!(diff <= 1e-8 + 1e-5 * fabs(ref))) ok = false;
+      if (!(diff <= a.unitary_atol + 1e-5 * fabs(ref)) || a.unitary_atol < 0.0) ok = false;
     }
   if (!ok) atomicOr(a.error, kErrNotUnitary);
   size_t gi = ((size_t)b * a.lmax + a.layer) * n + site;

@@ -93,7 +93,8 @@ constexpr int kJacobiMaxPairs = 512;
 constexpr double kJacobiAbsTol = 0.25 * 2.220446049250313e-16;
 
 __global__ __launch_bounds__(512) void jacobi_kernel(double2* a_all, double2* v_all, size_t stride,
-                                                     const int* ncols_dev, int ld, int* error) {
+                                                     const int* ncols_dev, int ld, int max_sweeps,
+                                                     int* error) {
   __shared__ int pp[kJacobiMaxPairs], qq[kJacobiMaxPairs];
   __shared__ double jc[kJacobiMaxPairs], js[kJacobiMaxPairs];
   __shared__ double2 je[kJacobiMaxPairs];
@@ -118,7 +119,7 @@ __global__ __launch_bounds__(512) void jacobi_kernel(double2* a_all, double2* v_
   const int np = n / 2;
   const double abs_tol = kJacobiAbsTol * s_scale + 1e-300;
   bool converged = false;
-  for (int sweep = 0; sweep < 40; ++sweep) {
+  for (int sweep = 0; sweep < max_sweeps; ++sweep) {
     __syncthreads();
     if (tid == 0) any_rot = 0;
     __syncthreads();
@@ -851,9 +852,9 @@ void launch_gram_reduce(const double2* part, int nchunks, const int* ncols_dev, 
 }
 
 void launch_jacobi(double2* a, double2* v, size_t stride, const int* ncols_dev, int ncols_max,
-                   int batch, int* error, hipStream_t st) {
+                   int batch, int max_sweeps, int* error, hipStream_t st) {
   hipLaunchKernelGGL(jacobi_kernel, dim3(batch), dim3(512), 0, st, a, v, stride, ncols_dev,
-                     ncols_max, error);
+                     ncols_max, max_sweeps, error);
 }
 
 constexpr int kJgridRowsLds = 65536;   // LDS bytes of V rows per block in jgrid_rows_kernel (4 rows of 1024: all CUs busy; 8 or 2 rows measured slower)

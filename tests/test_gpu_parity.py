@@ -253,11 +253,12 @@ def test_sweep_matches_oracle(gpu, dtype, n, layers, sweeps, seed):
         assert serr < 1e-5
 
 
-@pytest.mark.parametrize("n,layers,sweeps,seed", [(6, 3, 3, 1), (9, 3, 3, 2)])
+@pytest.mark.parametrize("n,layers,sweeps,seed", [(6, 3, 3, 1), (10, 3, 3, 2), (11, 3, 2, 3)])
 def test_relaxed_sweep_matches_oracle(gpu, n, layers, sweeps, seed):
     """Over-relaxed gate update (omega = 1.6; an option with no reference
-    counterpart) on the per-gate path (N < 8) and the windowed path (N >= 8)
-    against the oracle's relaxed sweep, complex128."""
+    counterpart) on the per-gate path (N = 6 < 10) and the windowed path
+    (N = 10, 11: window_finalize_kernel's relaxed update) against the
+    oracle's relaxed sweep, complex128."""
     from qmprs_amd import _lib
     eng = _engine()
     omega = 1.6
@@ -333,10 +334,11 @@ def test_encode_batch_equals_single(gpu):
 
 
 @pytest.mark.parametrize("dtype,n,layers,sweeps,seed", [
-    (torch.complex128, 8, 3, 2, 5), (torch.complex128, 11, 4, 2, 6), (torch.complex64, 13, 3, 2, 7)])
+    (torch.complex128, 10, 3, 2, 5), (torch.complex128, 11, 4, 2, 6), (torch.complex64, 13, 3, 2, 7)])
 def test_window_sweep_equals_step_sweep(gpu, dtype, n, layers, sweeps, seed):
     """The windowed sweep (one HBM pass per gate window, 16x16 MFMA
-    environment) is the gate-by-gate sweep regrouped: same gates, same state."""
+    environment; N >= 10) is the gate-by-gate sweep regrouped: same gates,
+    same state."""
     eng = _engine()
     from qmprs_amd import _lib
     sv = O.ket(O.random_mps_state(n, 16, seed))
@@ -614,7 +616,7 @@ def test_sequential_sweep_relaxation(gpu):
     from qmprs_amd import _lib
     from qmprs_amd.circuit import GateCircuit
     from qmprs_amd.synthesis.mps_encoding import Sequential
-    n, chi, layers, sweeps = 9, 16, 3, 3
+    n, chi, layers, sweeps = 10, 16, 3, 3    # N >= 10: the windowed sweep
     sv = O.ket(O.random_mps_state(n, 24, 5))
     enc = Sequential(circuit_framework=GateCircuit)
     target, _ = O.chi_truncate(sv, n, chi)
