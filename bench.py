@@ -30,7 +30,8 @@ summed launch durations, measured with HIP events recorded by the library
 on the stream the kernel is launched on, launch by launch, over the timed
 steps.  ``traffic`` comes from two rocprofv3 counter runs (FETCH_SIZE, then
 WRITE_SIZE) of a one-sweep probe of this same code and batch, calibrated by
-the same counters on a 1 GiB device copy (null when rocprofv3 is absent).
+the same counters on a kernel that reads and writes 1 GiB (null when
+rocprofv3 is absent).
 ``cpu_baseline`` times the FP64 NumPy oracle (a port of the reference
 algorithm) on the host cores: chi-truncation + all layers + one sweep per
 state, the other sweeps scaled from the timed one, median over 3 states.
@@ -135,16 +136,17 @@ def _algorithmic_bytes(n: int, windows: int, state_sweeps: float) -> float:
 
 
 def traffic_probe(args) -> int:
-    """Counter-run body: a 1 GiB device copy (calibration), then one encode of
+    """Counter-run body: a 1 GiB elementwise copy (calibration), then one encode of
     the bench batch with one sweep.  Run under rocprofv3 --pmc by
     ``measure_traffic``."""
     import torch
     from qmprs_amd import _lib, engine
     wl = WORKLOAD
     dev = _lib.require_gpu()
-    src = torch.empty(1 << 30, dtype=torch.uint8, device=dev)
+    # calibration: one elementwise kernel that reads 1 GiB and writes 1 GiB
+    src = torch.ones(1 << 28, dtype=torch.float32, device=dev)
     dst = torch.empty_like(src)
-    dst.copy_(src)
+    torch.neg(src, out=dst)
     torch.cuda.synchronize()
     del src, dst
     svs = _states(wl["num_qubits"], wl["generator_bond"], args.batch, seed0=1000)
@@ -165,7 +167,7 @@ def _counter_means(csv_dir: str) -> dict:
             for row in csv.DictReader(fh):
                 name = row.get("Kernel_Name", "")
                 key = ("window_pass" if "window_pass_kernel" in name else
-                       "copy" if "copyBuffer" in name else None)
+                       "copy" if "neg_kernel" in name else None)
                 if key is None:
                     continue
                 v = float(row.get("Counter_Value", "nan"))
@@ -177,7 +179,8 @@ def _counter_means(csv_dir: str) -> dict:
 def measure_traffic(args, n: int, states_per_launch: int) -> dict | None:
     """HBM bytes per window-pass launch from two rocprofv3 counter runs of
     ``traffic_probe`` (FETCH_SIZE, then WRITE_SIZE; one counter per run),
-    each scaled by what the same counter reads on a 1 GiB copy."""
+    each scaled by what the same counter reads on the probe's 1 GiB
+    elementwise copy (torch.neg of 2^28 floats: 1 GiB read, 1 GiB written)."""
     prof = shutil.which("rocprofv3")
     if prof is None:
         return None
