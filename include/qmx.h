@@ -181,6 +181,22 @@ int qmx_sweep(qmx_handle* h, int num_qubits, int batch, int num_layers, const in
               const double* goal, int* sweeps_done, void* circuit_out, void* workspace,
               size_t workspace_bytes, void* stream);
 
+/* qmx_sweep without a workspace argument.  No counterpart in the reference:
+ * for callers that cannot allocate device memory (code under HIP graph
+ * capture, tight serving loops, harnesses that forbid hipMalloc).  The
+ * scratch qmx_sweep takes from its workspace comes from a library-owned
+ * array of QMX_NOALLOC_SCRATCH_BYTES (64 MiB) per device, held from library
+ * load to process end; problems whose qmx_sweep_workspace() exceeds it fail
+ * with QMX_ERR_WORKSPACE.  The sweep runs on `stream` alone (no internal
+ * streams or events).  Nothing is allocated; every call on one GPU shares
+ * that GPU's scratch, so calls that overlap on one GPU race.  Results equal
+ * qmx_sweep's with QMX_OPT_SPLIT_SWEEPS = 0 bit for bit. */
+#define QMX_NOALLOC_SCRATCH_BYTES (64u << 20)
+int qmx_sweep_noalloc(qmx_handle* h, int num_qubits, int batch, int num_layers, const int* layers_used,
+                      int num_sweeps, const void* target, int storage_dtype, void* gates,
+                      const int* widths, const void* psi_ref, void* fidelity, double* overlap,
+                      const double* goal, int* sweeps_done, void* circuit_out, void* stream);
+
 /* C|0..0> of a batch of gate lists on the GPU: the state a circuit backend's
  * get_statevector() returns for the emitted circuit (used by the reference's
  * tests, test_sequential_encoding.py:64-67), and the forward layer

@@ -26,6 +26,8 @@ QMX_ERR_BASIS = 6
 QMX_C64 = 0
 QMX_C128 = 1
 
+NOALLOC_SCRATCH_BYTES = 64 << 20   # QMX_NOALLOC_SCRATCH_BYTES
+
 PHASES = ("chi", "layers", "rebuild", "sweep", "window_pass")
 
 QMX_OPT_SWEEP_MODE = 1
@@ -86,6 +88,8 @@ SIGNATURES = {
     "qmx_sweep_workspace": (_SZ, [_INT, _INT, _INT]),
     "qmx_sweep": (_INT, [_VP, _INT, _INT, _INT, _VP, _INT, _VP, _INT, _VP, _VP, _VP, _VP, _VP, _VP,
                          _VP, _VP, _VP, _SZ, _VP]),
+    "qmx_sweep_noalloc": (_INT, [_VP, _INT, _INT, _INT, _VP, _INT, _VP, _INT, _VP, _VP, _VP, _VP, _VP, _VP,
+                                 _VP, _VP, _VP]),
     "qmx_circuit_state": (_INT, [_VP, _INT, _INT, _INT, _VP, _VP, _VP, _INT, _VP, _VP, _VP, _VP, _SZ, _VP]),
     "qmx_forward_plan": (_INT, [_INT, _INT, _VP, _SZ, _VP, _SZ, ctypes.POINTER(_INT), ctypes.POINTER(_INT)]),
     "qmx_encode_workspace": (_SZ, [ctypes.POINTER(EncodeConfig)]),

@@ -42,6 +42,8 @@ struct qmx_handle {
   hipEvent_t pass_ev[2] = {nullptr, nullptr};
   hipEvent_t join_ev[2] = {nullptr, nullptr};
   hipEvent_t call_ev = nullptr;
+  void* scratch = nullptr;              // this device's qmx_sweep_noalloc scratch (qmx_scratch.hip)
+  hipError_t scratch_status = hipSuccess;
 };
 
 namespace {
@@ -1098,6 +1100,16 @@ int qmx_create(int device, qmx_handle** out) {
   qmx_handle* h = new qmx_handle();
   h->device = device;
   h->cus = prop.multiProcessorCount > 0 ? prop.multiProcessorCount : 256;
+  // the allocation-free entry point's scratch: its device address, looked up
+  // with the handle's device current (the caller's current device restored)
+  int prev = 0;
+  if (hipGetDevice(&prev) == hipSuccess && hipSetDevice(device) == hipSuccess) {
+    h->scratch_status = scratch_address(device, &h->scratch);
+    (void)hipSetDevice(prev);
+  } else {
+    h->scratch_status = hipErrorInvalidDevice;
+  }
+  if (h->scratch_status != hipSuccess) h->scratch = nullptr;
   *out = h;
   return QMX_OK;
 }
@@ -1356,6 +1368,29 @@ int qmx_sweep(qmx_handle* h, int num_qubits, int batch, int num_layers, const in
                                  static_cast<const double2*>(target), static_cast<double2*>(gates), widths,
                                  static_cast<const double2*>(psi_ref), static_cast<double2*>(fidelity),
                                  overlap, goal, sweeps_done, static_cast<double2*>(circuit_out), c, st);
+}
+
+int qmx_sweep_noalloc(qmx_handle* h, int num_qubits, int batch, int num_layers, const int* layers_used,
+                      int num_sweeps, const void* target, int storage_dtype, void* gates,
+                      const int* widths, const void* psi_ref, void* fidelity, double* overlap,
+                      const double* goal, int* sweeps_done, void* circuit_out, void* stream) {
+  QMX_CALL(check_common(h, num_qubits, batch));
+  if (storage_dtype != QMX_C64 && storage_dtype != QMX_C128) return fail(h, QMX_ERR_INVALID, "storage_dtype");
+  const size_t need = qmx_sweep_workspace(num_qubits, batch, storage_dtype);
+  if (need > kNoAllocScratchBytes)
+    return fail(h, QMX_ERR_WORKSPACE,
+                "qmx_sweep_noalloc: this problem needs " + std::to_string(need) +
+                    " bytes of scratch (qmx_sweep_workspace); the library's scratch holds " +
+                    std::to_string(kNoAllocScratchBytes) + " bytes per device");
+  if (!h->scratch) return hip_fail(h, h->scratch_status, "qmx_sweep_noalloc: scratch address");
+  // one stream, the caller's: no internal streams or events are created
+  const bool split = h->split_sweeps;
+  h->split_sweeps = false;
+  const int rc = qmx_sweep(h, num_qubits, batch, num_layers, layers_used, num_sweeps, target, storage_dtype,
+                           gates, widths, psi_ref, fidelity, overlap, goal, sweeps_done, circuit_out,
+                           h->scratch, kNoAllocScratchBytes, stream);
+  h->split_sweeps = split;
+  return rc;
 }
 
 int qmx_circuit_state(qmx_handle* h, int num_qubits, int batch, int num_layers, const int* layers_used,

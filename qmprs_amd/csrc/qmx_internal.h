@@ -20,6 +20,11 @@ constexpr int kWinTileBits = 11;     // window pass tile: 2^11 amplitudes
 constexpr int kMaxWindow = 8;        // gates per window
 constexpr int kWinMinQubits = 10;    // windowed sweep: 4 waves x k-steps of 4 rest indices
 
+// allocation-free entry point: library-owned scratch per device (qmx_scratch.hip)
+constexpr size_t kNoAllocScratchBytes = size_t(64) << 20;
+constexpr int kMaxDevices = 64;
+hipError_t scratch_address(int device, void** out);
+
 // error bits accumulated on the device
 constexpr int kErrNotUnitary = 1;
 constexpr int kErrBasis = 2;

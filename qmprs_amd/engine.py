@@ -152,6 +152,28 @@ def sweep(target: torch.Tensor, gates: torch.Tensor, widths: torch.Tensor, layer
     return SweepResult(fid, ov[:, :num_sweeps], done, circ)
 
 
+def sweep_noalloc(target: torch.Tensor, gates: torch.Tensor, widths: torch.Tensor,
+                  layers_used: torch.Tensor, num_sweeps: int, fidelity: torch.Tensor | None = None,
+                  overlap: torch.Tensor | None = None, psi_ref: torch.Tensor | None = None,
+                  goal: torch.Tensor | None = None, sweeps_done: torch.Tensor | None = None,
+                  circuit_out: torch.Tensor | None = None) -> None:
+    """``sweep`` through qmx_sweep_noalloc: the library allocates nothing and
+    takes its scratch from its own per-device array (problems up to
+    qmx_sweep_workspace() <= 64 MiB); the caller passes every output buffer
+    (fidelity (B, S+1) complex128 with psi_ref, overlap (B, S) float64,
+    sweeps_done (B,) int32 with goal, circuit_out like target).  Overlapping
+    calls on one GPU race on that scratch."""
+    b, length = target.shape
+    n = num_qubits_of(length)
+    num_layers = gates.shape[1]
+    lib, h = _lib.load(), _lib.handle()
+    st = lib.qmx_sweep_noalloc(h, n, b, num_layers, _lib.ptr(layers_used), num_sweeps, _lib.ptr(target),
+                               _DTYPES[target.dtype], _lib.ptr(gates), _lib.ptr(widths), _lib.ptr(psi_ref),
+                               _lib.ptr(fidelity), _lib.ptr(overlap), _lib.ptr(goal), _lib.ptr(sweeps_done),
+                               _lib.ptr(circuit_out), _lib.stream_ptr())
+    _lib.check(st, h)
+
+
 def circuit_state(gates: torch.Tensor, widths: torch.Tensor, layers_used: torch.Tensor,
                   dtype=torch.complex128, psi_ref: torch.Tensor | None = None):
     """C|0..0> of each state's gate list on the GPU (qmx_circuit_state);
