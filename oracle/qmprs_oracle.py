@@ -173,9 +173,21 @@ def dominant_subspace(gram: np.ndarray, max_bond: int | None, cutoff: float,
         w = canonical_basis(vec[:, :k])
     elif gauge == "lapack":
         w = vec[:, :k]
+    elif gauge.startswith("phases:"):
+        # the eigenvectors with random phases: the freedom an SVD leaves on a
+        # bond with distinct singular values (measures gauge sensitivity)
+        rng = np.random.default_rng([int(gauge.split(":", 1)[1]), n_calls_for_seed(gram)])
+        w = vec[:, :k] * np.exp(2j * np.pi * rng.random(k))
     else:
         raise ValueError(f"unknown gauge {gauge!r}")
     return w, lam
+
+
+def n_calls_for_seed(gram: np.ndarray) -> int:
+    """A per-call seed component for the random-phase gauge: derived from the
+    matrix itself, so every bond of every layer draws different phases while
+    a run stays reproducible."""
+    return int(abs(hash(np.round(gram, 6).tobytes())) % (2 ** 31))
 
 
 # --------------------------------------------------------------------------
