@@ -125,6 +125,36 @@ int qmx_chi_truncate(qmx_handle* h, int num_qubits, int batch, int bond_dimensio
                      const void* statevectors, int sv_dtype, void* psi_out, void* target_out,
                      double* spectra, void* workspace, size_t workspace_bytes, void* stream);
 
+/* Input preprocessing of prepare_state (base.py:99-102), the quick Ket
+ * operations the reference applies before building the MPS (quick is not
+ * available here; the rules below are the ones assumed, see the oracle's
+ * change_indexing / compress_amplitudes):
+ *   index_type QMX_INDEX_ROW keeps the order; QMX_INDEX_SNAKE (N >= 3)
+ *   reverses the second half of the amplitudes (the amplitude vector viewed
+ *   as two rows, the odd row reversed);
+ *   compression_percentage > 0 then sets the int(pct / 100 * 2^N) amplitudes
+ *   of smallest magnitude to zero (equal magnitudes in index order);
+ *   percentages that would zero every amplitude are QMX_ERR_INVALID.
+ *   statevectors, out  device, B x 2^N of sv_dtype (out may equal the input
+ *                      only when index_type is QMX_INDEX_ROW)
+ * No normalisation: qmx_chi_truncate normalises, as Ket does. */
+#define QMX_INDEX_ROW 0
+#define QMX_INDEX_SNAKE 1
+size_t qmx_prepare_input_workspace(int num_qubits, int batch);
+int qmx_prepare_input(qmx_handle* h, int num_qubits, int batch, const void* statevectors, int sv_dtype,
+                      int index_type, double compression_percentage, void* out, void* workspace,
+                      size_t workspace_bytes, void* stream);
+
+/* Dense state of an MPS given by its site tensors: the MPS(mps=...) entry
+ * (mps.py:183-197, with MPS.to_statevector mps.py:251-270).
+ *   bond_dims  host, num_sites + 1 ints, bond_dims[0] = bond_dims[N] = 1
+ *   tensors    device, complex128 'lpr' site tensors A_i (bond_dims[i] x 2 x
+ *              bond_dims[i+1]) stored one after the other
+ *   state_out  device, 2^N complex128 (site 0 = most significant bit) */
+size_t qmx_mps_to_state_workspace(int num_sites, const int* bond_dims);
+int qmx_mps_to_state(qmx_handle* h, int num_sites, const int* bond_dims, const void* tensors, void* state_out,
+                     void* workspace, size_t workspace_bytes, void* stream);
+
 /* (b)+(c) one Ran layer from a normalised dense state.
  * Replaces MPS.generate_bond_D_unitary_layer (mps.py:849-891) with
  * generate_unitary_layer (mps.py:746-847): bond-2 compression in right form,

@@ -102,6 +102,38 @@ def random_mps_state(num_qubits: int, bond: int, seed: int) -> np.ndarray:
     return psi / np.linalg.norm(psi)
 
 
+def change_indexing(statevector, index_type: str) -> np.ndarray:
+    """quick ``Ket.change_indexing`` (base.py:99) [Q]: "row" keeps the order;
+    "snake" (N >= 3) views the amplitudes as a matrix of two rows and
+    reverses the odd row, i.e. the second half of the vector is reversed.
+    Parity unpinned: quick is not installed, this states the assumed rule."""
+    sv = np.asarray(statevector).reshape(-1).copy()
+    if index_type == "row":
+        return sv
+    if index_type != "snake":
+        raise ValueError("Index type not supported.")
+    if sv.size >= 8:
+        m = sv.reshape(2, -1)
+        m[1::2, :] = m[1::2, ::-1]
+        sv = m.reshape(-1)
+    return sv
+
+
+def compress_amplitudes(statevector, compression_percentage: float) -> np.ndarray:
+    """quick ``Ket.compress`` (base.py:101-102) [Q]: the int(pct / 100 * 2^N)
+    amplitudes of smallest magnitude are set to zero; equal magnitudes are
+    taken in index order (a stable sort; quick's own tie order is that of
+    numpy's default argsort).  Parity unpinned, as ``change_indexing``."""
+    sv = np.asarray(statevector).reshape(-1).copy()
+    cutoff = int((compression_percentage / 100.0) * sv.size)
+    if cutoff <= 0:
+        return sv
+    mag2 = sv.real.astype(np.float64) ** 2 + sv.imag.astype(np.float64) ** 2
+    order = np.argsort(mag2, kind="stable")
+    sv[order[:cutoff]] = 0
+    return sv
+
+
 # --------------------------------------------------------------------------
 # Truncation rules
 # --------------------------------------------------------------------------

@@ -28,6 +28,9 @@ QMX_C128 = 1
 
 NOALLOC_SCRATCH_BYTES = 64 << 20   # QMX_NOALLOC_SCRATCH_BYTES
 
+QMX_INDEX_ROW = 0
+QMX_INDEX_SNAKE = 1
+
 PHASES = ("chi", "layers", "rebuild", "sweep", "window_pass")
 
 QMX_OPT_SWEEP_MODE = 1
@@ -90,6 +93,10 @@ SIGNATURES = {
                          _VP, _VP, _VP, _SZ, _VP]),
     "qmx_sweep_noalloc": (_INT, [_VP, _INT, _INT, _INT, _VP, _INT, _VP, _INT, _VP, _VP, _VP, _VP, _VP, _VP,
                                  _VP, _VP, _VP]),
+    "qmx_prepare_input_workspace": (_SZ, [_INT, _INT]),
+    "qmx_prepare_input": (_INT, [_VP, _INT, _INT, _VP, _INT, _INT, _DBL, _VP, _VP, _SZ, _VP]),
+    "qmx_mps_to_state_workspace": (_SZ, [_INT, ctypes.POINTER(_INT)]),
+    "qmx_mps_to_state": (_INT, [_VP, _INT, ctypes.POINTER(_INT), _VP, _VP, _VP, _SZ, _VP]),
     "qmx_circuit_state": (_INT, [_VP, _INT, _INT, _INT, _VP, _VP, _VP, _INT, _VP, _VP, _VP, _VP, _SZ, _VP]),
     "qmx_forward_plan": (_INT, [_INT, _INT, _VP, _SZ, _VP, _SZ, ctypes.POINTER(_INT), ctypes.POINTER(_INT)]),
     "qmx_encode_workspace": (_SZ, [ctypes.POINTER(EncodeConfig)]),

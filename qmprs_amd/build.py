@@ -18,7 +18,8 @@ PKG = Path(__file__).resolve().parent
 ROOT = PKG.parent
 SRC = PKG / "csrc"
 OUT = PKG / "libqmx.so"
-SOURCES = ["qmx_kernels.hip", "qmx_linalg.hip", "qmx_window.hip", "qmx_mlp.hip", "qmx_scratch.hip", "qmx_api.cpp",
+SOURCES = ["qmx_kernels.hip", "qmx_linalg.hip", "qmx_window.hip", "qmx_mlp.hip", "qmx_scratch.hip", "qmx_prep.hip",
+           "qmx_api.cpp",
            "qmx_ragged.cpp"]
 ARCH = os.environ.get("QMX_OFFLOAD_ARCH", "gfx950")
 

@@ -8,6 +8,7 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <cmath>
 #include <cstring>
 #include <string>
 #include <vector>
@@ -1368,6 +1369,101 @@ int qmx_sweep(qmx_handle* h, int num_qubits, int batch, int num_layers, const in
                                  static_cast<const double2*>(target), static_cast<double2*>(gates), widths,
                                  static_cast<const double2*>(psi_ref), static_cast<double2*>(fidelity),
                                  overlap, goal, sweeps_done, static_cast<double2*>(circuit_out), c, st);
+}
+
+size_t qmx_prepare_input_workspace(int num_qubits, int batch) {
+  if (num_qubits < 2 || num_qubits > kMaxQubits || batch < 1 || batch > 65535) return 0;
+  return prep_input_bytes(num_qubits, batch);
+}
+
+int qmx_prepare_input(qmx_handle* h, int num_qubits, int batch, const void* statevectors, int sv_dtype,
+                      int index_type, double compression_percentage, void* out, void* workspace,
+                      size_t workspace_bytes, void* stream) {
+  QMX_CALL(check_common(h, num_qubits, batch));
+  if (!statevectors || !out) return fail(h, QMX_ERR_INVALID, "null pointer");
+  if (sv_dtype != QMX_C64 && sv_dtype != QMX_C128) return fail(h, QMX_ERR_INVALID, "dtype");
+  if (index_type != QMX_INDEX_ROW && index_type != QMX_INDEX_SNAKE)
+    return fail(h, QMX_ERR_INVALID, "Index type not supported.");
+  if (!std::isfinite(compression_percentage)) return fail(h, QMX_ERR_INVALID, "compression_percentage");
+  const size_t len = state_len(num_qubits);
+  // int((pct / 100) * 2^N) amplitudes, as the reference computes it; only
+  // for pct > 0 (base.py:101)
+  size_t cutoff = 0;
+  if (compression_percentage > 0.0) {
+    const double x = (compression_percentage / 100.0) * static_cast<double>(len);
+    if (x >= static_cast<double>(len))
+      return fail(h, QMX_ERR_INVALID, "compression_percentage would set every amplitude to zero");
+    cutoff = static_cast<size_t>(x);
+  }
+  if (cutoff > 0 && (!workspace || workspace_bytes < prep_input_bytes(num_qubits, batch)))
+    return fail(h, QMX_ERR_WORKSPACE, "workspace too small");
+  QMX_HIP(h, hipSetDevice(h->device));
+  hipStream_t st = static_cast<hipStream_t>(stream);
+  const int snake = index_type == QMX_INDEX_SNAKE ? 1 : 0;
+  if (sv_dtype == QMX_C64)
+    launch_prep_input(static_cast<const float2*>(statevectors), static_cast<float2*>(out), num_qubits, batch,
+                      snake, cutoff, workspace, st);
+  else
+    launch_prep_input(static_cast<const double2*>(statevectors), static_cast<double2*>(out), num_qubits, batch,
+                      snake, cutoff, workspace, st);
+  QMX_LAUNCHED(h, "input preprocessing");
+  return QMX_OK;
+}
+
+namespace {
+bool mps_bonds_valid(int num_sites, const int* bond_dims) {
+  if (num_sites < 2 || num_sites > kMaxQubits || !bond_dims) return false;
+  if (bond_dims[0] != 1 || bond_dims[num_sites] != 1) return false;
+  for (int i = 1; i < num_sites; ++i)
+    if (bond_dims[i] < 1 || bond_dims[i] > (1 << 20)) return false;
+  return true;
+}
+// largest intermediate X_i (2^i rows x bond_dims[i] columns), i = 1 .. N-1
+size_t mps_chain_elems(int num_sites, const int* bond_dims) {
+  size_t mx = 1;
+  for (int i = 1; i < num_sites; ++i) mx = std::max(mx, ((size_t)1 << i) * (size_t)bond_dims[i]);
+  return mx;
+}
+}  // namespace
+
+size_t qmx_mps_to_state_workspace(int num_sites, const int* bond_dims) {
+  if (!mps_bonds_valid(num_sites, bond_dims)) return 0;
+  return 2 * mps_chain_elems(num_sites, bond_dims) * sizeof(double2) + 512;
+}
+
+int qmx_mps_to_state(qmx_handle* h, int num_sites, const int* bond_dims, const void* tensors, void* state_out,
+                     void* workspace, size_t workspace_bytes, void* stream) {
+  if (!h) return QMX_ERR_INVALID;
+  if (!mps_bonds_valid(num_sites, bond_dims))
+    return fail(h, QMX_ERR_INVALID, "bond dimensions: num_sites + 1 entries >= 1, first and last 1");
+  if (!tensors || !state_out || !workspace) return fail(h, QMX_ERR_INVALID, "null pointer");
+  if (workspace_bytes < qmx_mps_to_state_workspace(num_sites, bond_dims))
+    return fail(h, QMX_ERR_WORKSPACE, "workspace too small");
+  QMX_HIP(h, hipSetDevice(h->device));
+  hipStream_t st = static_cast<hipStream_t>(stream);
+  Carver c(workspace);
+  const size_t mx = mps_chain_elems(num_sites, bond_dims);
+  double2* buf[2] = {c.take<double2>(mx), c.take<double2>(mx)};
+  const double2* site = static_cast<const double2*>(tensors);
+  // X_1 = A_0 viewed as (2 x r_1): the site tensor itself
+  const double2* x = site;
+  site += 2 * (size_t)bond_dims[1];
+  for (int i = 1; i < num_sites; ++i) {
+    const int kd = bond_dims[i], nc = 2 * bond_dims[i + 1];
+    double2* y = (i == num_sites - 1) ? static_cast<double2*>(state_out) : buf[i & 1];
+    // X_{i+1} (2^i x 2 r_{i+1}) = X_i (2^i x r_i) A_i (r_i x 2 r_{i+1}), in row
+    // chunks of at most 2^21 rows (the GEMM grid's row dimension)
+    const int chunk_log2 = std::min(i, 21);
+    const size_t rows = (size_t)1 << i, chunk = (size_t)1 << chunk_log2;
+    for (size_t r0 = 0; r0 < rows; r0 += chunk) {
+      GemmDims d{chunk_log2, nullptr, nullptr, 0, kd, nc, nc};
+      launch_gemm(x + r0 * kd, 0, site, 0, y + r0 * nc, 0, d, 0, 1, st);
+    }
+    QMX_LAUNCHED(h, "MPS contraction step");
+    site += (size_t)kd * nc;
+    x = y;
+  }
+  return QMX_OK;
 }
 
 int qmx_sweep_noalloc(qmx_handle* h, int num_qubits, int batch, int num_layers, const int* layers_used,

@@ -287,4 +287,10 @@ void launch_gemm(const double2* a, size_t a_stride, const double2* bm, size_t b_
                  double2* c, size_t c_stride, const GemmDims& d, int conj_b, int batch,
                  hipStream_t);
 
+// input preprocessing (qmx_prep.hip): re-index (snake) and zero the `cutoff`
+// smallest-magnitude amplitudes of each state
+size_t prep_input_bytes(int n, int batch);
+template <typename S>
+void launch_prep_input(const S* in, S* out, int n, int batch, int snake, size_t cutoff, void* ws, hipStream_t);
+
 }  // namespace qmx

@@ -49,6 +49,55 @@ def as_device_states(statevectors, dtype=None) -> torch.Tensor:
     return t.to(dev).contiguous()
 
 
+INDEX_TYPES = {"row": _lib.QMX_INDEX_ROW, "snake": _lib.QMX_INDEX_SNAKE}
+
+
+def prepare_input(statevectors, index_type: str = "row", compression_percentage: float = 0.0) -> torch.Tensor:
+    """quick Ket re-indexing and compression of prepare_state's input
+    (base.py:99-102) on the GPU (qmx_prepare_input): a new (B, 2^N) tensor of
+    the input's complex dtype."""
+    if index_type not in INDEX_TYPES:
+        raise ValueError("Index type not supported.")
+    sv = as_device_states(statevectors)
+    b, length = sv.shape
+    n = num_qubits_of(length)
+    lib, h = _lib.load(), _lib.handle()
+    out = torch.empty_like(sv)
+    ws = _lib.workspace(lib.qmx_prepare_input_workspace(n, b), sv.device)
+    st = lib.qmx_prepare_input(h, n, b, _lib.ptr(sv), _DTYPES[sv.dtype], INDEX_TYPES[index_type],
+                               float(compression_percentage), _lib.ptr(out), _lib.ptr(ws), ws.numel(),
+                               _lib.stream_ptr())
+    _lib.check(st, h)
+    return out
+
+
+def mps_to_state(tensors) -> torch.Tensor:
+    """Dense state (1, 2^N) complex128 of 'lpr' site tensors (l, 2, r) with
+    end bonds 1 (qmx_mps_to_state: the MPS(mps=...) entry, mps.py:183-197)."""
+    ts = [np.asarray(a, dtype=np.complex128) for a in tensors]
+    n = len(ts)
+    if n < 2 or any(a.ndim != 3 for a in ts):
+        raise ValueError("an MPS of at least 2 site tensors of shape (l, 2, r) is needed")
+    if any(a.shape[1] != 2 for a in ts):
+        raise ValueError("Only supports MPS with physical dimension of 2.")
+    bonds = [ts[0].shape[0]] + [a.shape[2] for a in ts]
+    if bonds[0] != 1 or bonds[-1] != 1 or any(ts[i].shape[0] != bonds[i] for i in range(n)):
+        raise ValueError("site tensor bond dimensions do not match (end bonds must be 1)")
+    dev = _lib.require_gpu()
+    flat = torch.from_numpy(np.concatenate([a.reshape(-1) for a in ts])).to(dev)
+    bd = (ctypes.c_int * (n + 1))(*bonds)
+    lib, h = _lib.load(), _lib.handle()
+    need = lib.qmx_mps_to_state_workspace(n, bd)
+    if need == 0:
+        raise ValueError("invalid bond dimensions")
+    ws = _lib.workspace(need, dev)
+    out = torch.empty((1, 1 << n), dtype=torch.complex128, device=dev)
+    st = lib.qmx_mps_to_state(h, n, bd, _lib.ptr(flat), _lib.ptr(out), _lib.ptr(ws), ws.numel(),
+                              _lib.stream_ptr())
+    _lib.check(st, h)
+    return out
+
+
 def chi_truncate(states: torch.Tensor, bond_dimension: int, cutoff: float = CUTOFF,
                  spectra: bool = False):
     """(a)+(b): normalised inputs and the chi-MPS states (complex128)."""

@@ -37,14 +37,20 @@ UnitaryBlock: TypeAlias = tuple[int, int, list[Tensor]]
 UnitaryLayer: TypeAlias = list[UnitaryBlock]
 
 
-def _normalised(statevector) -> np.ndarray:
-    """quick ``Ket``: flat complex128 unit vector (base.py:96-97)."""
-    sv = np.asarray(getattr(statevector, "data", statevector), dtype=np.complex128).reshape(-1)
-    engine.num_qubits_of(sv.size)
-    nrm = np.linalg.norm(sv)
-    if nrm == 0:
+def _input_states(statevector) -> torch.Tensor:
+    """The statevector as a (1, 2^N) device tensor (complex64 stays complex64;
+    quick ``Ket`` normalisation, base.py:96-97, happens in qmx_chi_truncate)."""
+    if isinstance(statevector, torch.Tensor):
+        sv = statevector.reshape(-1)
+    else:
+        sv = np.asarray(getattr(statevector, "data", statevector)).reshape(-1)
+        if not np.iscomplexobj(sv):
+            sv = sv.astype(np.complex128)
+    engine.num_qubits_of(sv.shape[0])
+    states = engine.as_device_states(sv)
+    if float(torch.linalg.vector_norm(states)) == 0.0:
         raise ValueError("The statevector must be non-zero.")
-    return sv / nrm
+    return states
 
 
 class MPS:
@@ -63,14 +69,13 @@ class MPS:
                 "`bond_dimension` must be an integer greater than 0. "
                 f"Received {bond_dimension}.")
         if statevector is not None and mps is None:
-            sv = _normalised(statevector)
-            if sv.size == 2:
+            states = _input_states(statevector)
+            if states.shape[1] == 2:
                 raise ValueError(f"The statevector must have at least 2 qubits. Received 1.")
-            self.statevector = sv
-            states = engine.as_device_states(sv)
             psi, target, _ = engine.chi_truncate(states, bond_dimension)
             self._psi = psi
             self._state = target
+            self.statevector = psi[0].cpu().numpy()
         elif mps is not None and statevector is None:
             if isinstance(mps, (str, bytes)) or not hasattr(mps, "__len__"):
                 raise TypeError(
@@ -81,13 +86,8 @@ class MPS:
             tensors = [np.asarray(a, dtype=np.complex128) for a in mps]
             if any(a.ndim != 3 or a.shape[1] != 2 for a in tensors):
                 raise ValueError("Only supports MPS with physical dimension of 2.")
-            # contract on the GPU (MPS.to_statevector, mps.py:183-197)
-            dev = engine.as_device_states(np.zeros(2)).device
-            dense = torch.from_numpy(tensors[0]).to(dev).reshape(-1, tensors[0].shape[-1])
-            for a in tensors[1:]:
-                at = torch.from_numpy(a).to(dev)
-                dense = (dense @ at.reshape(a.shape[0], -1)).reshape(-1, a.shape[-1])
-            dense = dense.reshape(1, -1).contiguous()
+            # contracted on the GPU (qmx_mps_to_state; MPS.to_statevector, mps.py:183-197)
+            dense = engine.mps_to_state(tensors)
             nrm = torch.linalg.vector_norm(dense)
             if float(nrm) == 0.0:
                 raise ValueError("The MPS must represent a non-zero state.")

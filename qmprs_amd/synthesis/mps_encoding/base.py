@@ -8,7 +8,9 @@ from abc import ABC, abstractmethod
 from typing import Literal
 
 import numpy as np
+import torch
 
+from qmprs_amd import engine
 from qmprs_amd.primitives import MPS
 
 
@@ -25,13 +27,19 @@ class MPSEncoder(ABC):
 
     def prepare_state(self, statevector, bond_dimension: int, compression_percentage: float = 0.0,
                       index_type: Literal["row", "snake"] = "row", **kwargs):
-        """base.py:58-106: statevector -> MPS (chi-truncated on the GPU) -> circuit."""
-        if index_type != "row":
-            raise ValueError("Only index_type='row' is supported.")
-        if compression_percentage > 0.0:
-            raise ValueError("compression_percentage > 0 is not supported.")
-        if hasattr(statevector, "data") and not isinstance(statevector, np.ndarray):
+        """base.py:58-106: statevector -> re-indexing / compression (quick Ket
+        semantics, on the GPU) -> MPS (chi-truncated on the GPU) -> circuit.
+
+        ``index_type="snake"`` reverses the second half of the amplitudes
+        (N >= 3) and ``compression_percentage`` > 0 zeroes the
+        int(pct / 100 * 2^N) smallest-magnitude amplitudes, the rules assumed
+        for quick's ``Ket.change_indexing`` / ``Ket.compress``."""
+        if index_type not in ("row", "snake"):
+            raise ValueError("Index type not supported.")
+        if hasattr(statevector, "data") and not isinstance(statevector, (np.ndarray, torch.Tensor)):
             statevector = statevector.data
+        if index_type == "snake" or compression_percentage > 0.0:
+            statevector = engine.prepare_input(statevector, index_type, compression_percentage)
         mps = MPS(statevector, bond_dimension=bond_dimension)
         return self.prepare_mps(mps, **kwargs)
 

@@ -141,10 +141,8 @@ def test_prepare_state_argument_validation():
     from qmprs_amd.synthesis.mps_encoding import Sequential
     enc = Sequential(GateCircuit)
     sv = np.ones(8) / np.sqrt(8)
-    with pytest.raises(ValueError):
-        enc.prepare_state(sv, 4, index_type="snake")
-    with pytest.raises(ValueError):
-        enc.prepare_state(sv, 4, compression_percentage=10.0)
+    with pytest.raises(ValueError, match="Index type"):
+        enc.prepare_state(sv, 4, index_type="column")
 
 
 def test_mps_argument_validation():

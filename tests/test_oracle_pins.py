@@ -491,3 +491,39 @@ def test_relaxed_sweeps_reach_plain_fidelity_sooner():
             reached = s
             break
     assert reached is not None and reached <= 21, reached
+
+
+def test_snake_reindexing_closed_form():
+    """change_indexing("snake") [Q]: N >= 3 reverses the second half (an
+    involution); N < 3 and "row" leave the order."""
+    v = np.arange(16) + 0j
+    np.testing.assert_array_equal(O.change_indexing(v, "snake").real,
+                                  [0, 1, 2, 3, 4, 5, 6, 7, 15, 14, 13, 12, 11, 10, 9, 8])
+    w = np.random.default_rng(0).standard_normal(64) + 0j
+    np.testing.assert_array_equal(O.change_indexing(O.change_indexing(w, "snake"), "snake"), w)
+    np.testing.assert_array_equal(O.change_indexing(np.arange(4) + 0j, "snake"), np.arange(4))
+    np.testing.assert_array_equal(O.change_indexing(w, "row"), w)
+    with pytest.raises(ValueError):
+        O.change_indexing(w, "column")
+
+
+@pytest.mark.parametrize("pct", [0.0, 12.5, 50.0, 99.0])
+def test_compression_zeroes_the_smallest(pct):
+    """compress_amplitudes [Q]: exactly int(pct/100 * 2^N) amplitudes become
+    zero, every zeroed magnitude is <= every kept one, equal magnitudes are
+    taken in index order, the kept amplitudes are unchanged."""
+    rng = np.random.default_rng(int(pct))
+    v = rng.integers(0, 5, 256) + 1j * rng.integers(0, 3, 256)
+    out = O.compress_amplitudes(v, pct)
+    k = int(pct / 100 * v.size)
+    zeroed = np.nonzero((out == 0) & (v != 0))[0]
+    was_zero = np.count_nonzero(v == 0)
+    assert np.count_nonzero(out == 0) == max(k, was_zero)
+    kept = out != 0
+    np.testing.assert_array_equal(out[kept], v[kept])
+    if k:
+        # the first k of (magnitude, index) order are exactly the zeroed ones
+        order = np.lexsort((np.arange(v.size), np.abs(v)))
+        assert np.all(out[order[:k]] == 0) and np.all(out[order[k:]] == v[order[k:]])
+        if kept.any() and zeroed.size:
+            assert np.abs(v[zeroed]).max() <= np.abs(v[kept]).min()
