@@ -214,35 +214,28 @@ def measure_traffic(args, n: int, states_per_launch: int) -> dict | None:
                      f"{windows} window passes per half-batch"}
 
 
-# learned warm start: training pairs from the plain method on N=16, chi=32,
-# 8 layers (416 states x 8 layers x 15 two-qubit slots = 49920 pairs)
+# learned warm start: trajectory pairs from the plain method on N=16, chi=32,
+# 8 layers, 30 sweeps (416 states x 8 layers x 15 two-qubit slots x 11 sweeps
+# = 549120 pairs; 32 held-out states for the held-out loss)
 WARM_TRAIN = {"num_qubits": 16, "bond_dimension": 32, "num_layers": 8, "num_sweeps": 30,
-              "generator_bond": 48, "states": 416, "epochs": 20, "batch": 512, "lr": 1e-3,
+              "generator_bond": 48, "states": 416, "held_out": 32, "epochs": 10, "batch": 512, "lr": 1e-3,
               "width": 512}
 
 
-def _fp64_check(res, psi128, goal) -> dict:
-    """FP64 re-evaluation of each state's final gates (qmx_circuit_state in
-    complex128): |<psi|C|0>| against the goal."""
-    from qmprs_amd import engine
-    _, f = engine.circuit_state(res.gates, res.widths, res.layers_used, psi_ref=psi128)
-    f = f.abs()
-    return {"goal_reached": float((f >= goal).double().mean()),
-            "fidelity_mean": float(f.mean()), "fidelity_min": float(f.min())}
-
-
 def warm_start_leg(svs, psi128, goal, wl, args) -> dict:
-    """Train the warm-start MLP on the GPU (untimed), then time the
-    warm-started encode of the same batch against the FP64 goal."""
+    """Train the trajectory network on the GPU (untimed), then time the
+    encode with learned jumps of the same batch against the FP64 goal."""
     import torch
     from qmprs_amd import learned
     tw = WARM_TRAIN
     t0 = time.perf_counter()
     train_sv = _states(tw["num_qubits"], tw["generator_bond"], tw["states"], seed0=5000)
-    data = learned.make_training_set(train_sv, tw["bond_dimension"], tw["num_layers"], tw["num_sweeps"])
+    held_sv = _states(tw["num_qubits"], tw["generator_bond"], tw["held_out"], seed0=9000)
+    data = learned.make_trajectory_set(train_sv, tw["bond_dimension"], tw["num_layers"], tw["num_sweeps"])
+    held = learned.make_trajectory_set(held_sv, tw["bond_dimension"], tw["num_layers"], tw["num_sweeps"])
     t1 = time.perf_counter()
     net = learned.WarmStartMLP(width=tw["width"], seed=0)
-    hist = learned.train(net, data, epochs=tw["epochs"], batch=tw["batch"], lr=tw["lr"])
+    hist = learned.train(net, data, epochs=tw["epochs"], batch=tw["batch"], lr=tw["lr"], valid=learned.TRAJ_OUT)
     torch.cuda.synchronize()
     t2 = time.perf_counter()
     chi, L = wl["bond_dimension"], wl["num_layers"]
@@ -259,12 +252,16 @@ def warm_start_leg(svs, psi128, goal, wl, args) -> dict:
     out = {
         "value": round(len(svs) * steps / (t4 - t3), 3), "unit": "state-preps/s", "steps": steps,
         "ms_per_step": round(1000.0 * (t4 - t3) / steps, 2),
+        "method": f"plain complex64 sweeps with a learned jump every {learned.JUMP_EVERY} sweeps "
+                  f"(prediction {learned.TRAJ_K} sweeps ahead), kept per state if the fidelity does not drop",
         "mean_sweeps": float(res.sweeps.float().mean()), "max_sweeps": int(res.sweeps.max()),
-        "proposals_accepted": float(res.accepted.float().mean()),
+        "jumps_kept_per_state": float(res.accepted.float().mean()), "jump_points": res.tried,
         "train": {"pairs": int(data.weights.sum().item()), "config": {k: tw[k] for k in (
             "num_qubits", "bond_dimension", "num_layers", "num_sweeps", "states", "epochs", "width")},
             "data_s": round(t1 - t0, 1), "train_s": round(t2 - t1, 1),
-            "loss_first_epoch": hist[0], "loss_last_epoch": hist[-1]},
+            "loss_first_epoch": hist[0], "loss_last_epoch": hist[-1],
+            "held_out_mse": learned.evaluate_loss(net, held, learned.TRAJ_OUT),
+            "held_out_mse_zero_predictor": learned.zero_loss(held, learned.TRAJ_OUT)},
     }
     out.update(_fp64_check(res, psi128, goal))
     return out

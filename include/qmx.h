@@ -382,6 +382,30 @@ int qmx_gate_expmap(qmx_handle* h, int num_qubits, int batch, int num_layers, co
                     const float* coeffs, int ld, void* gates, const int* widths, const int* skip,
                     void* stream);
 
+/* Trajectory extrapolation (the learned warm start's jumps; no reference
+ * counterpart).  Rows as above.  For the gates after sweeps s-2 (a), s-1 (b)
+ * and s (c): d1 = c(Ga^H Gb), d2 = c(Gb^H Gc), row_scale = |d2| (floor
+ * 1e-12); e2 = d2 / |d2| and e1 = the part of d1 orthogonal to e2,
+ * normalised (zero when d1 is parallel to d2 to 1e-6).
+ *   qmx_traj_features: BF16 rows (ld >= 35) = [d1 / scale (15), d2 / scale
+ *                      (15), log10(scale) / 8, sweep_frac, layer / L, site / N,
+ *                      1, 0 ...]; row_weight as above; row_scale
+ *   qmx_traj_targets:  FP32 rows (ld >= 2) = [t.e2, t.e1, 0 ...] with t =
+ *                      c(Gc^H Gd) / scale, Gd the gates K sweeps after s
+ *   qmx_traj_jump:     G <- G exp(i scale sum_k (c_0 e2_k + c_1 e1_k) l_k),
+ *                      the basis from (Ga, Gb, G), on live rows of states with
+ *                      skip[b] == 0 (skip may be NULL) */
+int qmx_traj_features(qmx_handle* h, int num_qubits, int batch, int num_layers, const int* layers_used,
+                      const void* gates_a, const void* gates_b, const void* gates_c, const int* widths,
+                      float sweep_frac, void* features, int ld, float* row_weight, float* row_scale,
+                      void* stream);
+int qmx_traj_targets(qmx_handle* h, int num_qubits, int batch, int num_layers, const int* layers_used,
+                     const void* gates_a, const void* gates_b, const void* gates_c, const void* gates_d,
+                     const int* widths, float* targets, int ld, void* stream);
+int qmx_traj_jump(qmx_handle* h, int num_qubits, int batch, int num_layers, const int* layers_used,
+                  const void* gates_a, const void* gates_b, const float* coeffs, int ld, void* gates,
+                  const int* widths, const int* skip, void* stream);
+
 #ifdef __cplusplus
 }
 #endif

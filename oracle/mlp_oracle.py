@@ -149,3 +149,57 @@ def features(g0: np.ndarray, g1: np.ndarray, layer: int, num_layers: int, site: 
     f[48] = site / num_sites
     f[49] = 1.0
     return f
+
+
+# ---------------------------------------------- trajectory extrapolation --
+
+TRAJ_FEATURES = 35
+
+
+def traj_features(ga: np.ndarray, gb: np.ndarray, gc: np.ndarray, layer: int, num_layers: int,
+                  site: int, num_sites: int, sweep_frac: float):
+    """Feature row (FP64 before BF16 rounding) and row scale of one two-qubit
+    gate slot from its gates after sweeps s-2, s-1, s: the updates
+    d1 = c(Ga^H Gb), d2 = c(Gb^H Gc) in units of |d2| (computed in FP64,
+    rounded to FP32 as the kernel keeps it)."""
+    d1 = log_coeffs(ga.conj().T @ gb)
+    d2 = log_coeffs(gb.conj().T @ gc)
+    scale = np.float32(max(np.linalg.norm(d2), 1e-12))
+    f = np.zeros(64)
+    f[:15] = d1 / float(scale)
+    f[15:30] = d2 / float(scale)
+    f[30] = np.log10(float(scale)) / 8
+    f[31] = sweep_frac
+    f[32] = layer / num_layers
+    f[33] = site / num_sites
+    f[34] = 1.0
+    return f, float(scale)
+
+
+def traj_basis(ga, gb, gc):
+    """(scale, e1, e2): e2 = d2 / |d2|, e1 = the part of d1 orthogonal to e2,
+    normalised (zero when below 1e-6 of scale)."""
+    d1 = log_coeffs(ga.conj().T @ gb)
+    d2 = log_coeffs(gb.conj().T @ gc)
+    n2 = np.linalg.norm(d2)
+    sc = max(n2, 1e-12)
+    e2 = d2 / n2 if n2 > 0 else np.zeros(15)
+    e1 = d1 - (d1 @ e2) * e2
+    n1 = np.linalg.norm(e1)
+    e1 = e1 / n1 if n1 > 1e-6 * sc else np.zeros(15)
+    return sc, e1, e2
+
+
+def traj_target(ga, gb, gc, gd) -> np.ndarray:
+    """Label: c(Gc^H Gd) / scale on (e2, e1)."""
+    sc, e1, e2 = traj_basis(ga, gb, gc)
+    t = log_coeffs(gc.conj().T @ gd) / sc
+    return np.array([t @ e2, t @ e1])
+
+
+def traj_jump(ga, gb, g, coef) -> np.ndarray:
+    """G exp(i scale (c_0 e2 + c_1 e1) . l) with the basis from (Ga, Gb, G)."""
+    sc, e1, e2 = traj_basis(ga, gb, g)
+    c = np.asarray(coef, dtype=np.float64)[:2]
+    cs = (sc * (c[0] * e2 + c[1] * e1)).astype(np.float32).astype(np.float64)
+    return g @ exp_coeffs(cs)

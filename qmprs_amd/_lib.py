@@ -121,6 +121,9 @@ SIGNATURES = {
     "qmx_mse_loss": (_INT, [_VP, _INT, _INT, _INT, _VP, _VP, _VP, _FLT, _VP, _VP, _VP, _SZ, _VP]),
     "qmx_gate_features": (_INT, [_VP, _INT, _INT, _INT, _VP, _VP, _VP, _VP, _VP, _INT, _VP, _VP]),
     "qmx_gate_log": (_INT, [_VP, _INT, _INT, _INT, _VP, _VP, _VP, _VP, _VP, _INT, _VP]),
+    "qmx_traj_features": (_INT, [_VP, _INT, _INT, _INT, _VP, _VP, _VP, _VP, _VP, _FLT, _VP, _INT, _VP, _VP, _VP]),
+    "qmx_traj_targets": (_INT, [_VP, _INT, _INT, _INT, _VP, _VP, _VP, _VP, _VP, _VP, _VP, _INT, _VP]),
+    "qmx_traj_jump": (_INT, [_VP, _INT, _INT, _INT, _VP, _VP, _VP, _VP, _INT, _VP, _VP, _VP, _VP]),
     "qmx_gate_expmap": (_INT, [_VP, _INT, _INT, _INT, _VP, _VP, _INT, _VP, _VP, _VP, _VP]),
 }
 

@@ -1889,6 +1889,50 @@ int qmx_gate_log(qmx_handle* h, int num_qubits, int batch, int num_layers, const
   return QMX_OK;
 }
 
+int qmx_traj_features(qmx_handle* h, int num_qubits, int batch, int num_layers, const int* layers_used,
+                      const void* gates_a, const void* gates_b, const void* gates_c, const int* widths,
+                      float sweep_frac, void* features, int ld, float* row_weight, float* row_scale,
+                      void* stream) {
+  if (!h) return QMX_ERR_INVALID;
+  GateRowArgs r{};
+  QMX_CALL(gate_rows_check(h, num_qubits, batch, num_layers, r, layers_used, widths));
+  if (!gates_a || !gates_b || !gates_c || !features || !row_scale || ld < kTrajFeatures)
+    return fail(h, QMX_ERR_INVALID, "traj_features: bad arguments");
+  QMX_HIP(h, hipSetDevice(h->device));
+  launch_traj_features(r, static_cast<const double2*>(gates_a), static_cast<const double2*>(gates_b),
+                       static_cast<const double2*>(gates_c), static_cast<uint16_t*>(features), ld, row_weight,
+                       row_scale, sweep_frac, static_cast<hipStream_t>(stream));
+  QMX_LAUNCHED(h, "trajectory features");
+  return QMX_OK;
+}
+int qmx_traj_targets(qmx_handle* h, int num_qubits, int batch, int num_layers, const int* layers_used,
+                     const void* gates_a, const void* gates_b, const void* gates_c, const void* gates_d,
+                     const int* widths, float* targets, int ld, void* stream) {
+  if (!h) return QMX_ERR_INVALID;
+  GateRowArgs r{};
+  QMX_CALL(gate_rows_check(h, num_qubits, batch, num_layers, r, layers_used, widths));
+  if (!gates_a || !gates_b || !gates_c || !gates_d || !targets || ld < 2)
+    return fail(h, QMX_ERR_INVALID, "traj_targets: bad arguments");
+  QMX_HIP(h, hipSetDevice(h->device));
+  launch_traj_targets(r, static_cast<const double2*>(gates_a), static_cast<const double2*>(gates_b),
+                      static_cast<const double2*>(gates_c), static_cast<const double2*>(gates_d), targets, ld,
+                      static_cast<hipStream_t>(stream));
+  QMX_LAUNCHED(h, "trajectory targets");
+  return QMX_OK;
+}
+int qmx_traj_jump(qmx_handle* h, int num_qubits, int batch, int num_layers, const int* layers_used,
+                  const void* gates_a, const void* gates_b, const float* coeffs, int ld, void* gates,
+                  const int* widths, const int* skip, void* stream) {
+  if (!h) return QMX_ERR_INVALID;
+  GateRowArgs r{};
+  QMX_CALL(gate_rows_check(h, num_qubits, batch, num_layers, r, layers_used, widths));
+  if (!gates_a || !gates_b || !coeffs || !gates || ld < 2) return fail(h, QMX_ERR_INVALID, "traj_jump: bad arguments");
+  QMX_HIP(h, hipSetDevice(h->device));
+  launch_traj_jump(r, static_cast<const double2*>(gates_a), static_cast<const double2*>(gates_b), coeffs, ld,
+                   static_cast<double2*>(gates), skip, static_cast<hipStream_t>(stream));
+  QMX_LAUNCHED(h, "trajectory jump");
+  return QMX_OK;
+}
 int qmx_gate_expmap(qmx_handle* h, int num_qubits, int batch, int num_layers, const int* layers_used,
                     const float* coeffs, int ld, void* gates, const int* widths, const int* skip,
                     void* stream) {

@@ -225,6 +225,13 @@ void launch_gate_features(const GateRowArgs&, const double2* ga, const double2* 
                           float* weight, hipStream_t);
 void launch_gate_log(const GateRowArgs&, const double2* ga, const double2* gb, float* out, int ld, hipStream_t);
 void launch_gate_expmap(const GateRowArgs&, const float* coef, int ld, double2* g, const int* skip, hipStream_t);
+constexpr int kTrajFeatures = 35;   // used columns of a trajectory feature row
+void launch_traj_features(const GateRowArgs&, const double2* ga, const double2* gb, const double2* gc,
+                          uint16_t* feat, int ld, float* weight, float* scale, float sweep_frac, hipStream_t);
+void launch_traj_targets(const GateRowArgs&, const double2* ga, const double2* gb, const double2* gc,
+                         const double2* gd, float* out, int ld, hipStream_t);
+void launch_traj_jump(const GateRowArgs&, const double2* ga, const double2* gb, const float* coef, int ld,
+                      double2* g, const int* skip, hipStream_t);
 
 template <typename S> void launch_apply_pass(const PassArgs<S>&, int nblk, int batch, hipStream_t);
 template <typename S> void launch_sweep_step(const SweepArgs<S>&, int nblk, int batch, hipStream_t);

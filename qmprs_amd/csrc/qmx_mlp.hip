@@ -335,6 +335,151 @@ __global__ __launch_bounds__(64) void gate_features_kernel(GateRowArgs a, const 
 }
 
 // labels: c(Ga^H Gb) as FP32 rows of width ld (zeros beyond 15)
+// ---- trajectory extrapolation (learned warm start) ----------------------
+// su(4) coefficients c(A^H B) of the relative rotation of one gate slot
+QMX_DEV void slot_log(const double2* A, const double2* B, double* cf) {
+  double2 u[16], h[16];
+#pragma unroll
+  for (int r = 0; r < 4; ++r)
+#pragma unroll
+    for (int c = 0; c < 4; ++c) {
+      double2 s = cz();
+#pragma unroll
+      for (int k = 0; k < 4; ++k) s = cfmac(A[k * 4 + r], B[k * 4 + c], s);
+      u[r * 4 + c] = s;
+    }
+  su4_log(u, h);
+  su4_coeffs(h, cf);
+}
+
+// Trajectory features of each two-qubit gate slot at sweep s, from the gates
+// after sweeps s-2 (a), s-1 (b) and s (c): the last two updates d1 = c(Ga^H
+// Gb), d2 = c(Gb^H Gc) in units of |d2| (the row scale, floored at 1e-12):
+// [d1 / scale (15), d2 / scale (15), log10(scale) / 8, sweep_frac, layer /
+// lmax, site / n, 1, 0 ...]
+__global__ __launch_bounds__(64) void traj_features_kernel(GateRowArgs a, const double2* ga, const double2* gb,
+                                                           const double2* gc, uint16_t* feat, int ld,
+                                                           float* weight, float* scale, float sweep_frac) {
+  const int row = blockIdx.x * blockDim.x + threadIdx.x;
+  if (row >= a.rows) return;
+  size_t gi;
+  const bool ok = gate_row(a, row, gi);
+  uint16_t* f = feat + (size_t)row * ld;
+  for (int c = 0; c < ld; ++c) f[c] = 0;
+  if (weight) weight[row] = ok ? 1.f : 0.f;
+  if (scale) scale[row] = 1.f;
+  if (!ok) return;
+  double d1[15], d2[15];
+  slot_log(ga + gi * 16, gb + gi * 16, d1);
+  slot_log(gb + gi * 16, gc + gi * 16, d2);
+  double nrm = 0.0;
+#pragma unroll
+  for (int k = 0; k < 15; ++k) nrm += d2[k] * d2[k];
+  const double sc = fmax(sqrt(nrm), 1e-12);
+  const float scf = (float)sc;
+  if (scale) scale[row] = scf;
+#pragma unroll
+  for (int k = 0; k < 15; ++k) {
+    f[k] = f2bf((float)(d1[k] / (double)scf));
+    f[15 + k] = f2bf((float)(d2[k] / (double)scf));
+  }
+  const int per = a.n - 1;
+  const int rem = row % (a.lmax * per);
+  f[30] = f2bf(log10f(scf) / 8.f);
+  f[31] = f2bf(sweep_frac);
+  f[32] = f2bf((float)(rem / per) / (float)a.lmax);
+  f[33] = f2bf((float)(rem % per) / (float)a.n);
+  f[34] = f2bf(1.f);
+}
+
+// Orthonormal basis of the span of a slot's last two updates (in units of
+// |d2|): e2 = d2 / |d2|, e1 = the part of d1 orthogonal to e2, normalised
+// (zero when d1 is parallel to d2 to 1e-6).  Returns |d2| (floor 1e-12).
+QMX_DEV double traj_basis(const double2* ga, const double2* gb, const double2* gc, double* e1, double* e2) {
+  double d1[15], d2[15];
+  slot_log(ga, gb, d1);
+  slot_log(gb, gc, d2);
+  double n2 = 0.0;
+#pragma unroll
+  for (int k = 0; k < 15; ++k) n2 += d2[k] * d2[k];
+  n2 = sqrt(n2);
+  const double sc = fmax(n2, 1e-12);
+  double dot = 0.0;
+#pragma unroll
+  for (int k = 0; k < 15; ++k) {
+    e2[k] = n2 > 0.0 ? d2[k] / n2 : 0.0;
+    dot += d1[k] * e2[k];
+  }
+  double n1 = 0.0;
+#pragma unroll
+  for (int k = 0; k < 15; ++k) {
+    e1[k] = d1[k] - dot * e2[k];
+    n1 += e1[k] * e1[k];
+  }
+  n1 = sqrt(n1);
+  const bool keep = n1 > 1e-6 * sc;
+#pragma unroll
+  for (int k = 0; k < 15; ++k) e1[k] = keep ? e1[k] / n1 : 0.0;
+  return sc;
+}
+
+// targets: the update c(Gc^H Gd) in units of |d2|, projected on (e2, e1):
+// [t.e2, t.e1, 0 ...] (FP32, zero-padded to ld)
+__global__ __launch_bounds__(64) void traj_target_kernel(GateRowArgs a, const double2* ga, const double2* gb,
+                                                         const double2* gc, const double2* gd, float* out, int ld) {
+  const int row = blockIdx.x * blockDim.x + threadIdx.x;
+  if (row >= a.rows) return;
+  size_t gi;
+  const bool ok = gate_row(a, row, gi);
+  float* o = out + (size_t)row * ld;
+  for (int c = 0; c < ld; ++c) o[c] = 0.f;
+  if (!ok) return;
+  double e1[15], e2[15], t[15];
+  const double sc = traj_basis(ga + gi * 16, gb + gi * 16, gc + gi * 16, e1, e2);
+  slot_log(gc + gi * 16, gd + gi * 16, t);
+  double p2 = 0.0, p1 = 0.0;
+#pragma unroll
+  for (int k = 0; k < 15; ++k) {
+    p2 += t[k] * e2[k];
+    p1 += t[k] * e1[k];
+  }
+  o[0] = (float)(p2 / sc);
+  o[1] = (float)(p1 / sc);
+}
+
+// jump: G <- G exp(i |d2| sum_k (coef0 e2_k + coef1 e1_k) l_k) on live rows of
+// states with skip == 0, the basis taken from (Ga, Gb, G)
+__global__ __launch_bounds__(64) void traj_jump_kernel(GateRowArgs a, const double2* ga, const double2* gb,
+                                                       const float* coef, int ld, double2* g, const int* skip) {
+  const int row = blockIdx.x * blockDim.x + threadIdx.x;
+  if (row >= a.rows) return;
+  size_t gi;
+  if (!gate_row(a, row, gi)) return;
+  const int b = row / (a.lmax * (a.n - 1));
+  if (skip && skip[b]) return;
+  double2* G = g + gi * 16;
+  double e1[15], e2[15];
+  const double sc = traj_basis(ga + gi * 16, gb + gi * 16, G, e1, e2);
+  const double c0 = coef[(size_t)row * ld], c1 = coef[(size_t)row * ld + 1];
+  float cs[15];
+#pragma unroll
+  for (int k = 0; k < 15; ++k) cs[k] = (float)(sc * (c0 * e2[k] + c1 * e1[k]));
+  double2 h[16], e[16], out[16];
+  su4_matrix(cs, h);
+  su4_exp(h, e);
+#pragma unroll
+  for (int r = 0; r < 4; ++r)
+#pragma unroll
+    for (int c = 0; c < 4; ++c) {
+      double2 s = cz();
+#pragma unroll
+      for (int k = 0; k < 4; ++k) s = cfma(G[r * 4 + k], e[k * 4 + c], s);
+      out[r * 4 + c] = s;
+    }
+#pragma unroll
+  for (int i = 0; i < 16; ++i) G[i] = out[i];
+}
+
 __global__ __launch_bounds__(64) void gate_log_kernel(GateRowArgs a, const double2* ga, const double2* gb, float* out, int ld) {
   const int row = blockIdx.x * blockDim.x + threadIdx.x;
   if (row >= a.rows) return;
@@ -419,6 +564,19 @@ void launch_gate_features(const GateRowArgs& a, const double2* ga, const double2
 void launch_gate_log(const GateRowArgs& a, const double2* ga, const double2* gb, float* out, int ld,
                      hipStream_t st) {
   hipLaunchKernelGGL(gate_log_kernel, dim3((a.rows + 63) / 64), dim3(64), 0, st, a, ga, gb, out, ld);
+}
+void launch_traj_features(const GateRowArgs& a, const double2* ga, const double2* gb, const double2* gc,
+                          uint16_t* feat, int ld, float* weight, float* scale, float sweep_frac, hipStream_t st) {
+  hipLaunchKernelGGL(traj_features_kernel, dim3((a.rows + 63) / 64), dim3(64), 0, st, a, ga, gb, gc, feat, ld,
+                     weight, scale, sweep_frac);
+}
+void launch_traj_targets(const GateRowArgs& a, const double2* ga, const double2* gb, const double2* gc,
+                         const double2* gd, float* out, int ld, hipStream_t st) {
+  hipLaunchKernelGGL(traj_target_kernel, dim3((a.rows + 63) / 64), dim3(64), 0, st, a, ga, gb, gc, gd, out, ld);
+}
+void launch_traj_jump(const GateRowArgs& a, const double2* ga, const double2* gb, const float* coef, int ld,
+                      double2* g, const int* skip, hipStream_t st) {
+  hipLaunchKernelGGL(traj_jump_kernel, dim3((a.rows + 63) / 64), dim3(64), 0, st, a, ga, gb, coef, ld, g, skip);
 }
 void launch_gate_expmap(const GateRowArgs& a, const float* coef, int ld, double2* g, const int* skip,
                         hipStream_t st) {

@@ -1,20 +1,27 @@
 """Learned warm start for the environment sweeps (no reference counterpart).
 
-A small network proposes starting gates for the sweeps of
-``Sequential._optimize_unitary_layers`` (sequential.py:509-541).  Data,
-training and inference all run on the GPU through libqmx:
+The plain sweeps of ``Sequential._optimize_unitary_layers``
+(sequential.py:509-541) converge linearly: after the first few sweeps every
+gate keeps moving along a slowly turning direction, and the fidelity creeps
+up for tens of sweeps.  A small network, trained on trajectories of the plain
+method itself, predicts from a gate slot's last two updates where that gate
+will be K sweeps later; the encoder jumps there every few sweeps and keeps a
+state's jump only if its fidelity does not drop.  The plain sweeps remain the
+fallback and decide the final fidelity.  Everything runs on the GPU through
+libqmx:
 
-* training pairs come from the plain method itself: for each state the Ran
-  gates G0 (after layer extraction), the gates G1 after one sweep and the
-  gates GS after S sweeps; the features of a two-qubit gate slot are
-  ``qmx_gate_features(G0, G1)`` (the su(4) coefficients of G0^H G1, G1 itself,
-  its position) and the label is ``qmx_gate_log(G1, GS)`` (15 su(4)
-  coefficients);
+* features (``qmx_traj_features``): the su(4) coefficients of the updates
+  G_{s-2}^H G_{s-1} and G_{s-1}^H G_s of each two-qubit gate slot, in units
+  of the last update's norm, plus that norm's log, the sweep index and the
+  slot's position; label (``qmx_traj_targets``): c(G_s^H G_{s+K}) in the same
+  units, projected on that basis;
 * the model is a 3-layer ReLU MLP with BF16 weights and FP32 accumulation
   (``qmx_mlp_forward`` / ``qmx_mlp_backward`` on MFMA, ``qmx_mlp_adam``);
-* at inference the proposal G1 exp(i sum c_k l_k) replaces G1 for a state
-  only if its fidelity is at least that of G1; otherwise the state keeps the
-  plain gates.  The sweeps then continue as usual and decide the final fidelity.
+* the network predicts two numbers per slot, the coefficients of the jump on
+  an orthonormal basis (e2, e1) of the slot's last two updates (so that every
+  jump moves along the sweep's own directions: predictions of all 15 su(4)
+  coefficients, with errors in the steep directions, lowered the fidelity);
+  the jump is ``qmx_traj_jump``: G_s exp(i |d2| (c_0 e2 + c_1 e1) . l).
 """
 
 from __future__ import annotations
@@ -173,62 +180,112 @@ class TrainingSet:
     weights: torch.Tensor    # (rows,) 1 for live gate slots
 
 
-def plain_trajectory(statevectors, bond_dimension: int, num_layers: int, num_sweeps: int,
-                     sweep_dtype=torch.complex64):
-    """Plain method on a batch: returns (psi, target, widths, used, G0, G1, GS,
-    fidelity trace (B, S + 1))."""
+REF_SWEEPS = 30     # sweep count of the reference path (feature sweep_frac = s / REF_SWEEPS)
+TRAJ_K = 6          # sweeps ahead the network predicts
+JUMP_EVERY = 6      # a jump after sweeps 6, 12, 18, ...
+
+
+def traj_features(ga, gb, gc, widths, layers_used, sweep_frac: float):
+    """(rows, IN_DIM) BF16 features, (rows,) live-row weights, (rows,) row scales."""
+    b, nl, n = widths.shape
+    rows = b * nl * (n - 1)
+    dev = ga.device
+    feat = torch.empty((rows, IN_DIM), dtype=torch.int16, device=dev)
+    w = torch.empty(rows, dtype=torch.float32, device=dev)
+    sc = torch.empty(rows, dtype=torch.float32, device=dev)
+    _call("qmx_traj_features", n, b, nl, _lib.ptr(layers_used), _lib.ptr(ga), _lib.ptr(gb), _lib.ptr(gc),
+          _lib.ptr(widths), _f(sweep_frac), _lib.ptr(feat), IN_DIM, _lib.ptr(w), _lib.ptr(sc), _lib.stream_ptr())
+    return feat, w, sc
+
+
+TRAJ_OUT = 2        # the network's outputs: coefficients on (e2, e1)
+
+
+def traj_targets(ga, gb, gc, gd, widths, layers_used, ld: int = OUT_DIM):
+    """(rows, ld) FP32 labels: the update Gc -> Gd in units of |d2|, on the
+    orthonormal basis (e2, e1) of the slot's last two updates."""
+    b, nl, n = widths.shape
+    rows = b * nl * (n - 1)
+    out = torch.empty((rows, ld), dtype=torch.float32, device=gc.device)
+    _call("qmx_traj_targets", n, b, nl, _lib.ptr(layers_used), _lib.ptr(ga), _lib.ptr(gb), _lib.ptr(gc),
+          _lib.ptr(gd), _lib.ptr(widths), _lib.ptr(out), ld, _lib.stream_ptr())
+    return out
+
+
+def traj_jump(ga, gb, gates, widths, layers_used, coef, skip=None) -> None:
+    """gates <- gates exp(i |d2| (c_0 e2 + c_1 e1) . l) in place on live
+    two-qubit slots (basis from ga, gb, gates)."""
+    b, nl, n = widths.shape
+    _call("qmx_traj_jump", n, b, nl, _lib.ptr(layers_used), _lib.ptr(ga), _lib.ptr(gb), _lib.ptr(coef.contiguous()),
+          coef.shape[1], _lib.ptr(gates), _lib.ptr(widths), _lib.ptr(skip), _lib.stream_ptr())
+
+
+def plain_snapshots(statevectors, bond_dimension: int, num_layers: int, num_sweeps: int,
+                    sweep_dtype=torch.complex64):
+    """The plain method on a batch, one sweep at a time (the same gates as one
+    call of num_sweeps sweeps): (prepared, [gates after sweep 0..S], fidelity
+    (B, S + 1))."""
     p = engine.encode_prepare(statevectors, bond_dimension, num_layers, sweep_dtype=sweep_dtype)
-    psi, tgt, gates, widths, used = p.psi, p.target, p.gates, p.widths, p.layers_used
-    g0 = gates.clone()
-    r1 = engine.sweep(tgt, gates, widths, used, 1, psi_ref=psi)
-    g1 = gates.clone()
-    rs = engine.sweep(tgt, gates, widths, used, num_sweeps - 1, psi_ref=psi)
-    fid = torch.cat([r1.fidelity[:, :1], rs.fidelity], dim=1).abs()
-    return psi, tgt, widths, used, g0, g1, gates, fid
+    gates = p.gates
+    snaps = [gates.clone()]
+    fids = []
+    for _ in range(num_sweeps):
+        r = engine.sweep(p.target, gates, p.widths, p.layers_used, 1, psi_ref=p.psi)
+        fids.append(r.fidelity[:, :1])
+        snaps.append(gates.clone())
+    fids.append(r.fidelity[:, -1:])
+    return p, snaps, torch.cat(fids, dim=1).abs()
 
 
-def make_training_set(statevectors, bond_dimension: int, num_layers: int, num_sweeps: int,
-                      chunk: int = 64) -> TrainingSet:
+def make_trajectory_set(statevectors, bond_dimension: int, num_layers: int, num_sweeps: int = REF_SWEEPS,
+                        sweeps=tuple(range(4, 25, 2)), k: int = TRAJ_K, chunk: int = 64) -> TrainingSet:
+    """Training pairs from plain trajectories: for each listed sweep s the
+    features of (G_{s-2}, G_{s-1}, G_s) and the label G_s^H G_{s+k}."""
     feats, labels, weights = [], [], []
     for i in range(0, len(statevectors), chunk):
-        _, _, widths, used, g0, g1, gs, _ = plain_trajectory(statevectors[i:i + chunk], bond_dimension,
-                                                             num_layers, num_sweeps)
-        f, w = gate_features(g0, g1, widths, used)
-        feats.append(f)
-        weights.append(w)
-        labels.append(gate_log(g1, gs, widths, used))
+        p, snaps, _ = plain_snapshots(statevectors[i:i + chunk], bond_dimension, num_layers, num_sweeps)
+        for s in sweeps:
+            f, w, sc = traj_features(snaps[s - 2], snaps[s - 1], snaps[s], p.widths, p.layers_used,
+                                     s / REF_SWEEPS)
+            feats.append(f)
+            weights.append(w)
+            labels.append(traj_targets(snaps[s - 2], snaps[s - 1], snaps[s], snaps[min(s + k, num_sweeps)],
+                                       p.widths, p.layers_used))
     return TrainingSet(torch.cat(feats), torch.cat(labels), torch.cat(weights))
 
 
 def train(net: WarmStartMLP, data: TrainingSet, epochs: int = 20, batch: int = 512, lr: float = 1e-3,
-          seed: int = 0) -> list[float]:
-    """Minibatch Adam on the weighted MSE; returns the mean loss per epoch."""
+          seed: int = 0, valid: int = VALID_OUT) -> list[float]:
+    """Minibatch Adam on the weighted MSE over the first ``valid`` output
+    columns; returns the mean loss per epoch."""
     live = torch.nonzero(data.weights > 0).flatten()
     g = torch.Generator(device="cpu").manual_seed(seed)
     history = []
     for _ in range(epochs):
         order = live[torch.randperm(live.numel(), generator=g).to(live.device)]
-        total, count = 0.0, 0
         losses = []
         for s in range(0, order.numel() - batch + 1, batch):
             idx = order[s:s + batch]
             x = data.features[idx].contiguous()
             t = data.labels[idx].contiguous()
             y = net.forward(x)
-            loss, dy = mse_loss(y, t, None)
+            loss, dy = mse_loss(y, t, None, valid)
             grads = net.backward(x, dy)
             net.adam(grads, lr)
             losses.append(loss)
-        if losses:
-            total = torch.cat(losses).mean().item()
-            count = len(losses)
-        history.append(total if count else float("nan"))
+        history.append(torch.cat(losses).mean().item() if losses else float("nan"))
     return history
 
 
-def evaluate_loss(net: WarmStartMLP, data: TrainingSet) -> float:
+def evaluate_loss(net: WarmStartMLP, data: TrainingSet, valid: int = VALID_OUT) -> float:
     y = net.forward(data.features)
-    loss, _ = mse_loss(y, data.labels, data.weights)
+    loss, _ = mse_loss(y, data.labels, data.weights, valid)
+    return float(loss.item())
+
+
+def zero_loss(data: TrainingSet, valid: int = VALID_OUT) -> float:
+    """The weighted MSE of predicting no jump (all-zero output)."""
+    loss, _ = mse_loss(torch.zeros_like(data.labels), data.labels, data.weights, valid)
     return float(loss.item())
 
 
@@ -241,38 +298,55 @@ class WarmResult:
     layers_used: torch.Tensor
     fidelity: torch.Tensor      # (B,) final |<psi|C|0>|
     sweeps: torch.Tensor        # (B,) sweeps run in total
-    accepted: torch.Tensor      # (B,) 1 where the proposal replaced the plain gates
-    fid_plain_1: torch.Tensor   # (B,) fidelity after the first (plain) sweep
-    fid_proposal: torch.Tensor  # (B,) fidelity of the proposal
+    accepted: torch.Tensor      # (B,) jumps kept
+    tried: int                  # jump points reached
 
 
 def encode_warm(statevectors, bond_dimension: int, num_layers: int, num_sweeps: int, net: WarmStartMLP,
-                goal: torch.Tensor | None = None, sweep_dtype=torch.complex64) -> WarmResult:
-    """prepare_state with a learned warm start after the first sweep.
-
-    Sweeps stop per state once |<psi|C|0>| >= goal[b] (when goal is given) or
-    after num_sweeps in total."""
+                goal: torch.Tensor | None = None, sweep_dtype=torch.complex64, first: int = JUMP_EVERY,
+                every: int = JUMP_EVERY) -> WarmResult:
+    """prepare_state with learned jumps: plain sweeps, and after sweeps
+    first, first + every, ... the network's jump for every state still short of
+    its goal, kept where it does not lower the state's fidelity.  Sweeps stop
+    per state at |<psi|C|0>| >= goal[b] (when goal is given) or after
+    num_sweeps in total."""
     if num_sweeps < 1:
         raise ValueError("the warm start needs num_sweeps >= 1")
     p = engine.encode_prepare(statevectors, bond_dimension, num_layers, sweep_dtype=sweep_dtype)
     psi, tgt, gates, widths, used = p.psi, p.target, p.gates, p.widths, p.layers_used
-    g0 = gates.clone()
-    r1 = engine.sweep(tgt, gates, widths, used, 1, psi_ref=psi)
-    f1 = r1.fidelity[:, -1].abs()
-    feat, _ = gate_features(g0, gates, widths, used)
-    coef = net.forward(feat)
-    prop = gates.clone()
-    gate_expmap(prop, widths, used, coef)
-    rp = engine.sweep(tgt, prop, widths, used, 0, psi_ref=psi)
-    fp = rp.fidelity[:, -1].abs()
-    accept = fp >= f1
-    gates = torch.where(accept.view(-1, 1, 1, 1, 1), prop, gates).contiguous()
-    rest = num_sweeps - 1
-    if rest > 0:
-        res = engine.sweep(tgt, gates, widths, used, rest, psi_ref=psi, goal=goal)
-        done = res.sweeps_done
-        final = res.fidelity[:, -1].abs()
-    else:
-        done = torch.zeros_like(used)
-        final = torch.where(accept, fp, f1)
-    return WarmResult(gates, widths, used, final, done + 1, accept.to(torch.int32), f1, fp)
+    b = gates.shape[0]
+    done = torch.zeros(b, dtype=torch.int32, device=gates.device)
+    accepted = torch.zeros(b, dtype=torch.int32, device=gates.device)
+
+    def run(k):
+        nonlocal done
+        r = engine.sweep(tgt, gates, widths, used, k, psi_ref=psi, goal=goal)
+        done = done + (r.sweeps_done if goal is not None else k)
+        return r.fidelity[:, -1].abs()
+
+    s, tried, fid = 0, 0, None
+    j = first
+    while j <= num_sweeps and j >= 2:
+        if j - 2 > s:
+            run(j - 2 - s)
+        ga = gates.clone()
+        run(1)
+        gb = gates.clone()
+        fid = run(1)
+        s = j
+        active = (fid < goal) if goal is not None else torch.ones_like(fid, dtype=torch.bool)
+        if not bool(active.any()):
+            break
+        tried += 1
+        feat, _, _ = traj_features(ga, gb, gates, widths, used, s / REF_SWEEPS)
+        coef = net.forward(feat)
+        prop = gates.clone()
+        traj_jump(ga, gb, prop, widths, used, coef, skip=(~active).to(torch.int32).contiguous())
+        _, fp = engine.circuit_state(prop, widths, used, dtype=sweep_dtype, psi_ref=psi)
+        keep = active & (fp.abs() >= fid)
+        gates.copy_(torch.where(keep.view(-1, 1, 1, 1, 1), prop, gates))
+        accepted += keep.to(torch.int32)
+        j += every
+    if num_sweeps > s:
+        fid = run(num_sweeps - s)
+    return WarmResult(gates, widths, used, fid, done, accepted, tried)

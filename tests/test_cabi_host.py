@@ -335,3 +335,17 @@ def test_forward_plan_rejects_bad_sizes():
     assert lib.qmx_forward_plan(31, 2, None, 0, None, 0, ctypes.byref(nr), ctypes.byref(ns)) != 0
     st = lib.qmx_forward_plan(10, 2, None, 0, None, 0, ctypes.byref(nr), ctypes.byref(ns))
     assert st == mod.QMX_ERR_WORKSPACE and nr.value == 20 and ns.value >= 1
+
+
+def test_every_package_module_imports():
+    """Every module of the package and bench.py import without a GPU (the GPU
+    is touched only by calls)."""
+    import importlib
+    import pkgutil
+
+    import qmprs_amd
+    for m in pkgutil.walk_packages(qmprs_amd.__path__, "qmprs_amd."):
+        if m.name.endswith("libqmx"):     # the C library itself (loaded by ctypes)
+            continue
+        importlib.import_module(m.name)
+    importlib.import_module("bench")
