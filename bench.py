@@ -222,6 +222,16 @@ WARM_TRAIN = {"num_qubits": 16, "bond_dimension": 32, "num_layers": 8, "num_swee
               "width": 512}
 
 
+def _fp64_check(res, psi128, goal) -> dict:
+    """FP64 re-evaluation of each state's final gates (qmx_circuit_state in
+    complex128): |<psi|C|0>| against the goal."""
+    from qmprs_amd import engine
+    _, f = engine.circuit_state(res.gates, res.widths, res.layers_used, psi_ref=psi128)
+    f = f.abs()
+    return {"goal_reached": float((f >= goal).double().mean()),
+            "fidelity_mean": float(f.mean()), "fidelity_min": float(f.min())}
+
+
 def warm_start_leg(svs, psi128, goal, wl, args) -> dict:
     """Train the trajectory network on the GPU (untimed), then time the
     encode with learned jumps of the same batch against the FP64 goal."""
