@@ -262,8 +262,9 @@ def warm_start_leg(svs, psi128, goal, wl, args) -> dict:
     out = {
         "value": round(len(svs) * steps / (t4 - t3), 3), "unit": "state-preps/s", "steps": steps,
         "ms_per_step": round(1000.0 * (t4 - t3) / steps, 2),
-        "method": f"plain complex64 sweeps with a learned jump every {learned.JUMP_EVERY} sweeps "
-                  f"(prediction {learned.TRAJ_K} sweeps ahead), kept per state if the fidelity does not drop",
+        "method": f"plain complex64 sweeps with a learned jump after sweep {learned.JUMP_FIRST} and every "
+                  f"{learned.JUMP_EVERY} sweeps after it (each predicting {learned.TRAJ_K} sweeps ahead), kept "
+                  f"per state if the fidelity does not drop",
         "mean_sweeps": float(res.sweeps.float().mean()), "max_sweeps": int(res.sweeps.max()),
         "jumps_kept_per_state": float(res.accepted.float().mean()), "jump_points": res.tried,
         "train": {"pairs": int(data.weights.sum().item()), "config": {k: tw[k] for k in (

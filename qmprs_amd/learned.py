@@ -182,7 +182,9 @@ class TrainingSet:
 
 REF_SWEEPS = 30     # sweep count of the reference path (feature sweep_frac = s / REF_SWEEPS)
 TRAJ_K = 6          # sweeps ahead the network predicts
-JUMP_EVERY = 6      # a jump after sweeps 6, 12, 18, ...
+JUMP_FIRST = 6      # the first jump after sweep 6, ...
+JUMP_EVERY = 4      # ... then after every 4 further sweeps (sweeps to goal at N=20: 6/6 20.97,
+                    # 4/4 19.78, 6/4 19.55, 5/5 20.24, 8/8 23.26; 6/4 fastest per step)
 
 
 def traj_features(ga, gb, gc, widths, layers_used, sweep_frac: float):
@@ -303,7 +305,7 @@ class WarmResult:
 
 
 def encode_warm(statevectors, bond_dimension: int, num_layers: int, num_sweeps: int, net: WarmStartMLP,
-                goal: torch.Tensor | None = None, sweep_dtype=torch.complex64, first: int = JUMP_EVERY,
+                goal: torch.Tensor | None = None, sweep_dtype=torch.complex64, first: int = JUMP_FIRST,
                 every: int = JUMP_EVERY) -> WarmResult:
     """prepare_state with learned jumps: plain sweeps, and after sweeps
     first, first + every, ... the network's jump for every state still short of
