@@ -201,8 +201,118 @@ QMX_DEV int canonical_basis4(int k, const double2* v, double2* w, double tol) {
 }
 
 // ------------------------------------------------------- polar (d <= 4) ----
+// In-place inverse of a 4 x 4 complex matrix m by Gauss-Jordan elimination
+// with partial (row) pivoting: row swaps are recorded and undone as column
+// swaps at the end.  False when a pivot is zero or the result is not finite.
+// Compile-time indices only (swaps are selects).
+QMX_DEV bool inv4_inplace(double2* m) {
+  int perm[4];
+#pragma unroll
+  for (int k = 0; k < 4; ++k) {
+    int p = k;
+    double best = cabs2(m[k * 4 + k]);
+#pragma unroll
+    for (int i = k + 1; i < 4; ++i) {
+      const double v = cabs2(m[i * 4 + k]);
+      if (v > best) {
+        best = v;
+        p = i;
+      }
+    }
+    if (!(best > 0.0)) return false;
+    perm[k] = p;
+#pragma unroll
+    for (int i = k + 1; i < 4; ++i) {
+      if (p == i) {
+#pragma unroll
+        for (int j = 0; j < 4; ++j) {
+          const double2 t = m[k * 4 + j];
+          m[k * 4 + j] = m[i * 4 + j];
+          m[i * 4 + j] = t;
+        }
+      }
+    }
+    const double ib = 1.0 / best;
+    const double2 ip = make_double2(m[k * 4 + k].x * ib, -m[k * 4 + k].y * ib);   // 1 / pivot
+    m[k * 4 + k] = make_double2(1.0, 0.0);
+#pragma unroll
+    for (int j = 0; j < 4; ++j) m[k * 4 + j] = cmul(m[k * 4 + j], ip);
+#pragma unroll
+    for (int i = 0; i < 4; ++i) {
+      if (i != k) {
+        const double2 f = m[i * 4 + k];
+        m[i * 4 + k] = cz();
+#pragma unroll
+        for (int j = 0; j < 4; ++j) m[i * 4 + j] = csub(m[i * 4 + j], cmul(f, m[k * 4 + j]));
+      }
+    }
+  }
+#pragma unroll
+  for (int k = 3; k >= 0; --k) {
+#pragma unroll
+    for (int c = k + 1; c < 4; ++c) {
+      if (perm[k] == c) {
+#pragma unroll
+        for (int i = 0; i < 4; ++i) {
+          const double2 t = m[i * 4 + k];
+          m[i * 4 + k] = m[i * 4 + c];
+          m[i * 4 + c] = t;
+        }
+      }
+    }
+  }
+  double s = 0.0;
+#pragma unroll
+  for (int i = 0; i < 16; ++i) s += cabs2(m[i]);
+  return s < 1e300;   // false for inf / nan
+}
+
+// Unitary polar factor of a nonsingular 4 x 4 matrix by the scaled Newton
+// iteration (N. J. Higham, "Computing the polar decomposition -- with
+// applications", SIAM J. Sci. Stat. Comput. 7 (1986)), in place on u (which
+// holds the matrix on entry):
+//   X_0 = u,  X_{k+1} = (z X_k + X_k^{-H} / z) / 2,
+//   z = (||X_k^{-1}||_F / ||X_k||_F)^{1/2}  (z = 1 once ||X_{k+1} - X_k||_F < 1e-2),
+// which converges quadratically; it stops when ||X_{k+1} - X_k||_F <= 1e-8
+// (the last iterate is then accurate to rounding).  Returns false, with u
+// unspecified, when the matrix is ill-conditioned (||X_0||_F ||X_0^{-1}||_F >
+// 1e6), an inverse fails, or 40 iterations do not converge.
+QMX_DEV bool polar4_newton(double2* u) {
+  double2 xi[16];
+  bool scale = true;
+  for (int it = 0; it < 40; ++it) {
+#pragma unroll
+    for (int i = 0; i < 16; ++i) xi[i] = u[i];
+    if (!inv4_inplace(xi)) return false;
+    double nx = 0.0, ni = 0.0;
+#pragma unroll
+    for (int i = 0; i < 16; ++i) {
+      nx += cabs2(u[i]);
+      ni += cabs2(xi[i]);
+    }
+    if (it == 0 && nx * ni > 1e12) return false;
+    const double z = scale ? sqrt(sqrt(ni / nx)) : 1.0;
+    const double zi = 1.0 / z;
+    double dl = 0.0;
+    // X_{k+1}[i][j] = (z X[i][j] + conj(X^{-1}[j][i]) / z) / 2
+#pragma unroll
+    for (int i = 0; i < 4; ++i)
+#pragma unroll
+      for (int j = 0; j < 4; ++j) {
+        const double2 a = u[i * 4 + j], b = xi[j * 4 + i];
+        const double2 v = make_double2(0.5 * (z * a.x + zi * b.x), 0.5 * (z * a.y - zi * b.y));
+        dl += cabs2(csub(v, a));
+        u[i * 4 + j] = v;
+      }
+    if (dl < 1e-4) scale = false;
+    if (dl <= 1e-16) return true;
+  }
+  return false;
+}
+
 // Unitary polar factor U V^H of the d x d (d = 2 or 4) matrix e (row-major,
-// stride 4) by a one-sided (Hestenes) Jacobi SVD on a 4 x 4 working matrix (a
+// stride 4).  Well-conditioned inputs (see polar4_newton) take the Newton
+// iteration; the others (and any input it rejects) a one-sided (Hestenes) Jacobi SVD on a 4 x 4 working matrix (a
 // 2 x 2 problem is embedded as diag(E, |E| I), whose padded columns are
 // orthogonal to E's and never rotate).  Rotations: cyclic pairs p < q, when
 // |a_p^H a_q| > 1e-16 |a_p| |a_q|.  Pairs (u_j, v_j) with sigma_j >
@@ -211,7 +321,7 @@ QMX_DEV int canonical_basis4(int k, const double2* v, double2* w, double tol) {
 // against the columns already set has norm > 1/2, on both the U and V sides.
 // The result then depends only on the well-determined singular subspaces.
 // *nuc = sum of E's singular values.  Compile-time indices only.
-QMX_DEV void polar4(int d, const double2* e, double2* out, double* nuc) {
+QMX_DEV void polar4_jacobi(int d, const double2* e, double2* out, double* nuc) {
   double2 a[16], v[16];
   double fro = 0.0;
 #pragma unroll
@@ -343,6 +453,42 @@ QMX_DEV void polar4(int d, const double2* e, double2* out, double* nuc) {
     }
 }
 
+QMX_DEV void polar4(int d, const double2* e, double2* out, double* nuc) {
+  // the Newton iterate, in out: e, or diag(E, |E|_F I) for a 2 x 2 problem
+  double fro = 0.0;
+#pragma unroll
+  for (int i = 0; i < 16; ++i) {
+    const int r = i / 4, c = i % 4;
+    out[i] = (r < d && c < d) ? e[i] : cz();
+    fro += cabs2(out[i]);
+  }
+  if (d == 2) {
+    const double s = fro > 0.0 ? sqrt(fro) : 1.0;
+    out[10] = make_double2(s, 0.0);
+    out[15] = make_double2(s, 0.0);
+  }
+  // e is read again below (from LDS where the kernels keep it) instead of
+  // being held in registers through the iteration
+  asm volatile("" ::: "memory");
+  if (fro > 0.0 && polar4_newton(out)) {
+    asm volatile("" ::: "memory");
+    double tr = 0.0;   // Re tr(U^H E) = sum of the singular values
+#pragma unroll
+    for (int i = 0; i < 16; ++i) {
+      const int r = i / 4, c = i % 4;
+      if (r < d && c < d) {
+        tr += out[i].x * e[i].x + out[i].y * e[i].y;
+      } else {
+        out[i] = cz();
+      }
+    }
+    *nuc = tr;
+    return;
+  }
+  asm volatile("" ::: "memory");
+  polar4_jacobi(d, e, out, nuc);
+}
+
 // Over-relaxed local update (no reference counterpart).  g holds the plain
 // update conj(polar(E)) of a gate whose value before the update was g_old;
 // with omega_m1 = omega - 1 its d x d block becomes
@@ -350,23 +496,22 @@ QMX_DEV void polar4(int d, const double2* e, double2* out, double* nuc) {
 // a step past the local optimum along the direction of the update, projected
 // back onto the unitaries (omega = 1 leaves g as it is).  Returns the
 // overlap after the update, |sum_{o,i} g[o][i] E[o][i]| over the block.
-QMX_DEV double relax_update(int d, double omega_m1, const double2* g_old, double2* g, const double2* e) {
-  double2 m[16], u[16];
+QMX_DEV double relax_update(int d, double omega_m1, const double2* g_old, double2* g, const double2* e,
+                            double2* m) {
+  // m (16 entries of scratch, LDS in the kernels: the polar reads it from
+  // there rather than holding a second copy in registers)
 #pragma unroll
   for (int k = 0; k < 16; ++k) {
     const bool in = (k / 4) < d && (k % 4) < d;
     m[k] = in ? cadd(g[k], cscale(csub(g[k], g_old[k]), omega_m1)) : cz();
   }
   double unused = 0.0;
-  polar4(d, m, u, &unused);
+  polar4(d, m, g, &unused);
+  asm volatile("" ::: "memory");
   double2 ov = cz();
 #pragma unroll
-  for (int k = 0; k < 16; ++k) {
-    if ((k / 4) < d && (k % 4) < d) {
-      g[k] = u[k];
-      ov = cfma(u[k], e[k], ov);
-    }
-  }
+  for (int k = 0; k < 16; ++k)
+    if ((k / 4) < d && (k % 4) < d) ov = cfma(g[k], e[k], ov);
   return cabs_(ov);
 }
 

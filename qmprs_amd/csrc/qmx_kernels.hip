@@ -407,6 +407,7 @@ __global__ __launch_bounds__(256) void sweep_step_kernel(SweepArgs<S> a) {
   extern __shared__ double2 smem[];
   __shared__ Geom g;
   __shared__ double2 gc[16], gp[16];
+  __shared__ double2 s_em[16], s_rm[16];
   __shared__ int s_info[8];
   __shared__ double red[4 * 32];
   __shared__ double tot[32];
@@ -525,20 +526,18 @@ __global__ __launch_bounds__(256) void sweep_step_kernel(SweepArgs<S> a) {
   if (publish_partial<32>(red, pb, a.counter + b, gridDim.x, &s_flag)) {
     reduce_partials<32>(pb, gridDim.x, scratch, tot);
     if (threadIdx.x == 0) {
-      double2 em[16], u[16];
-      for (int j = 0; j < 16; ++j) em[j] = make_double2(tot[2 * j], tot[2 * j + 1]);
+      // E and the relaxed update's scratch in LDS: the polar solves read them
+      // from there
+      double2 u[16];
+      for (int j = 0; j < 16; ++j) s_em[j] = make_double2(tot[2 * j], tot[2 * j + 1]);
       int d = wc == 2 ? 4 : 2;
       double nuc = 0.0;
-      polar4(d, em, u, &nuc);
+      polar4(d, s_em, u, &nuc);
       for (int j = 0; j < 16; ++j) {
         int o = j / 4, i = j % 4;
         u[j] = (o < d && i < d) ? cconj(u[j]) : cz();
       }
-      if (a.omega_m1 != 0.0) {
-        double2 old[16];
-        for (int j = 0; j < 16; ++j) old[j] = a.gates[gic * 16 + j];
-        nuc = relax_update(d, a.omega_m1, old, u, em);
-      }
+      if (a.omega_m1 != 0.0) nuc = relax_update(d, a.omega_m1, a.gates + gic * 16, u, s_em, s_rm);
       for (int j = 0; j < 16; ++j) a.gates[gic * 16 + j] = u[j];
       if (a.overlap) a.overlap[(size_t)b * a.overlap_stride] = nuc;
       a.counter[b] = 0;

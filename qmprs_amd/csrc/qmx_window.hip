@@ -561,6 +561,7 @@ __global__ __launch_bounds__(256) void window_finalize_kernel(WinFinArgs a) {
   __shared__ int gw[kMaxWindow], gh[kMaxWindow], gl[kMaxWindow];
   __shared__ double2 gnew[16];
   __shared__ double2 es[16];
+  __shared__ double2 rm[16];
   __shared__ double part_sum[512];
   const int b = blockIdx.x;
   const int n = a.n;
@@ -630,15 +631,14 @@ __global__ __launch_bounds__(256) void window_finalize_kernel(WinFinArgs a) {
     }
     __syncthreads();
     if (threadIdx.x == 0) {
-      double2 e[16], u[16];
-#pragma unroll
-      for (int k = 0; k < 16; ++k) e[k] = es[k];
+      // E stays in LDS (es): the polar solves read it from there
+      double2 u[16];
       const int d = w == 2 ? 4 : 2;
       double nuc = 0.0;
-      polar4(d, e, u, &nuc);
+      polar4(d, es, u, &nuc);
 #pragma unroll
       for (int k = 0; k < 16; ++k) u[k] = cconj(u[k]);
-      if (a.omega_m1 != 0.0) nuc = relax_update(d, a.omega_m1, gm[j], u, e);
+      if (a.omega_m1 != 0.0) nuc = relax_update(d, a.omega_m1, gm[j], u, es, rm);
 #pragma unroll
       for (int k = 0; k < 16; ++k) gnew[k] = u[k];
       if (a.s0 + j == kb - 1 && a.overlap) a.overlap[(size_t)b * a.overlap_stride] = nuc;
