@@ -262,6 +262,9 @@ constexpr int kJacobiSweeps = 30;   // sweep launches of the LDS Jacobi per bond
 constexpr int kJacobiGridSweeps = 80;
 constexpr int kJacobiGlobalSweeps = 40;   // one-block Jacobi on global memory (2r <= 512)
 constexpr int kWinMaxBlocks = 256;
+// the finalize sums up to this many block partials per state itself; above
+// it, window_reduce_kernel pre-sums groups of 32
+constexpr int kWinFinalizePartials = 64;
 constexpr int kWinMaxBlocksLarge = 1024;
 
 // window-pass blocks per state: about win_blocks_per_cu resident blocks per CU
@@ -440,6 +443,7 @@ struct SweepLayout {
   int* dot_counter;
   int* skip;
   int* active;
+  int* list;   // live states after the last goal check
   int nblk_max;
 };
 
@@ -460,6 +464,7 @@ SweepLayout<S> sweep_layout(Carver& c, int n, int batch) {
   L.dot_counter = c.take<int>(batch);
   L.skip = c.take<int>(batch);
   L.active = c.take<int>(1);
+  L.list = c.take<int>(batch);
   return L;
 }
 
@@ -630,8 +635,11 @@ template <typename S>
 int run_fwd(qmx_handle* h, int n, int batch, int lmax, const std::vector<FwdPass>& passes, S* psi,
             const double2* gates, const int* widths, const int* layers, const int* skip,
             const double2* ref, double* dot_part, int* dot_counter, double2* dot_out,
-            int dot_stride, hipStream_t st) {
+            int dot_stride, hipStream_t st, const int* list = nullptr, int rows = 0) {
   if (passes.empty()) return fail(h, QMX_ERR_INVALID, "empty forward-pass plan");
+  // grid rows: the batch, or the `rows` live states of list
+  const int nrow = list ? rows : batch;
+  if (nrow < 1) return QMX_OK;
   const int t = std::min(n, kFwdTileBits);
   const uint64_t all = (1ull << n) - 1ull;
   uint64_t live = 0;
@@ -661,16 +669,17 @@ int run_fwd(qmx_handle* h, int n, int batch, int lmax, const std::vector<FwdPass
     a.widths = widths;
     a.layers = layers;
     a.skip = skip;
+    a.list = list;
     const long ntiles = 1l << __builtin_popcountll(a.tmask);
     const int nblk =
-        (int)std::max(1l, std::min(ntiles, ((long)h->fwd_blocks_per_cu * h->cus + batch - 1) / batch));
-    launch_fwd_pass(a, nblk, batch, st);
+        (int)std::max(1l, std::min(ntiles, ((long)h->fwd_blocks_per_cu * h->cus + nrow - 1) / nrow));
+    launch_fwd_pass(a, nblk, nrow, st);
     if (a.full) live = all;
   }
   // the overlap with ref is a separate coalesced pass: in the last circuit
   // pass the gate bits are the lowest ones, where a fused read of the
   // complex128 reference would be strided
-  if (ref) launch_overlap<S>(psi, ref, n, batch, dot_part, dot_counter, dot_out, dot_stride, skip, st);
+  if (ref) launch_overlap<S>(psi, ref, n, batch, dot_part, dot_counter, dot_out, dot_stride, skip, st, list, nrow);
   QMX_LAUNCHED(h, "forward passes");
   return QMX_OK;
 }
@@ -729,6 +738,12 @@ struct SweepJob {
   const int* skip = nullptr;
   bool use_goal = false;
   bool active = true;
+  // After a goal check with states done: the live states' indices (device)
+  // and their count; the sweep's launches then cover the live states only,
+  // with as many blocks per state as a full grid holds.  Otherwise null and
+  // the batch.
+  const int* list = nullptr;
+  int live = 0;
   std::vector<FwdPass> fwd_plan;   // forward passes of the circuit rebuild
 };
 
@@ -743,6 +758,8 @@ int job_begin(qmx_handle* h, SweepJob<S>& J) {
   if (J.sweeps_done) launch_fill_int(J.sweeps_done, J.goal ? 0 : J.nsweeps, J.batch, st);
   J.use_goal = J.goal && J.psi_ref && J.fidelity;
   J.skip = J.use_goal ? J.L.skip : nullptr;
+  J.list = nullptr;
+  J.live = J.batch;
   QMX_LAUNCHED(h, "sweep set-up");
   return QMX_OK;
 }
@@ -757,8 +774,10 @@ int job_rebuild(qmx_handle* h, SweepJob<S>& J, int s, bool final_state) {
   double2* dot = J.fidelity ? J.fidelity + s : nullptr;
   if (n >= 5 && h->sweep_mode != QMX_SWEEP_STEP) {
     if (J.fwd_plan.empty()) J.fwd_plan = plan_fwd(n, J.lmax * n);
+    const int* list = final_state ? nullptr : J.list;
     QMX_CALL(run_fwd<S>(h, n, J.batch, J.lmax, J.fwd_plan, J.L.phi, J.gates, J.widths, J.layers_used, skip,
-                        ref, J.L.dot_part, J.L.dot_counter, dot, J.nsweeps + 1, J.st));
+                        ref, J.L.dot_part, J.L.dot_counter, dot, J.nsweeps + 1, J.st, list,
+                        list ? J.live : J.batch));
   } else {
     launch_basis0(J.L.phi, n, J.batch, skip, J.st);
     QMX_CALL(run_passes<S>(h, n, J.batch, J.lmax, 1, 0, forward_circuit_passes(n, J.lmax), J.L.phi,
@@ -773,10 +792,13 @@ int job_goal(qmx_handle* h, SweepJob<S>& J, int s) {
   launch_fill_int(J.L.active, 0, 1, J.st);
   launch_goal_check(J.fidelity + s, J.nsweeps + 1, J.goal, J.L.skip, J.L.active, J.sweeps_done, J.batch,
                     J.st);
+  launch_live_list(J.L.skip, J.batch, J.L.list, J.st);
   int active = 0;
   QMX_HIP(h, hipMemcpyAsync(&active, J.L.active, sizeof(int), hipMemcpyDeviceToHost, J.st));
   QMX_HIP(h, hipStreamSynchronize(J.st));
   J.active = active != 0;
+  J.live = active;
+  J.list = active < J.batch ? J.L.list : nullptr;
   return QMX_OK;
 }
 
@@ -784,7 +806,7 @@ template <typename S>
 void job_window_pass(qmx_handle* h, SweepJob<S>& J, const std::vector<Window>& wins, size_t wi) {
   const int n = J.n;
   const Window& w = wins[wi];
-  const int wblk = window_blocks(h, n, J.batch);
+  const int wblk = window_blocks(h, n, J.live);
   WinPassArgs<S> pa{};
   pa.n = n;
   pa.lmax = J.lmax;
@@ -805,6 +827,7 @@ void job_window_pass(qmx_handle* h, SweepJob<S>& J, const std::vector<Window>& w
   pa.widths = J.widths;
   pa.layers = J.layers_used;
   pa.skip = J.skip;
+  pa.list = J.list;
   pa.part = J.L.part;
   // after a sweep's last window neither T' nor phi is read again (the next
   // sweep rebuilds phi from |0..0> and restarts T' from the target)
@@ -822,7 +845,7 @@ void job_window_pass(qmx_handle* h, SweepJob<S>& J, const std::vector<Window>& w
   const int min_run = h->staged_windows == 2 ? 3 : 7;
   pa.staged = (h->staged_windows && stride_bits >= 2 && low_run >= min_run && pa.t - low_run <= 8) ? 1 : 0;
   PhaseTimer pt(h, J.st, QMX_PHASE_WINDOW_PASS);
-  launch_window_pass(pa, wblk, J.batch, J.st);
+  launch_window_pass(pa, wblk, J.live, J.st);
   pt.stop(1);
 }
 
@@ -833,7 +856,7 @@ void job_window_finalize(qmx_handle* h, SweepJob<S>& J, const std::vector<Window
   WinFinArgs fa{};
   fa.n = n;
   fa.lmax = J.lmax;
-  fa.nblk = window_blocks(h, n, J.batch);
+  fa.nblk = window_blocks(h, n, J.live);
   fa.s0 = w.s0;
   fa.s1 = w.s1;
   fa.umask = w.umask;
@@ -841,18 +864,20 @@ void job_window_finalize(qmx_handle* h, SweepJob<S>& J, const std::vector<Window
   fa.widths = J.widths;
   fa.layers = J.layers_used;
   fa.skip = J.skip;
+  fa.list = J.list;
   fa.part = J.L.part;
   fa.overlap = J.overlap ? J.overlap + s : nullptr;
   fa.overlap_stride = J.nsweeps;
   fa.omega_m1 = (h->relax_milli - 1000) * 1e-3;
   fa.pstride = 1;
-  if (fa.nblk > kWinMaxBlocks) {
-    // large grids (few large states): pre-sum groups of 32 partials with a
-    // grid of blocks, so the one-block finalize reads nblk / 32 of them
-    launch_window_reduce(J.L.part, fa.nblk, 32, J.batch, J.st);
+  if (fa.nblk > kWinFinalizePartials) {
+    // large grids (few states, or few live ones): pre-sum groups of 32
+    // partials with a grid of blocks, so the one-block finalize reads
+    // nblk / 32 of them
+    launch_window_reduce(J.L.part, fa.nblk, 32, J.live, J.list, J.st);
     fa.pstride = 32;
   }
-  launch_window_finalize(fa, J.batch, J.st);
+  launch_window_finalize(fa, J.live, J.st);
 }
 
 template <typename S>

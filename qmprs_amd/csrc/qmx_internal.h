@@ -100,6 +100,7 @@ struct WinPassArgs {
   const int* widths;
   const int* layers;
   const int* skip;
+  const int* list;   // grid row -> state (live states only), or null: row = state
   double* part;        // [B][nblk][512]
   int staged;          // copy tiles through LDS in tile order (see window_pass_kernel)
   int store;           // write T' and phi back (0 for a sweep's last window: neither is read again)
@@ -128,6 +129,7 @@ struct FwdArgs {
   const int* widths;
   const int* layers;
   const int* skip;
+  const int* list;   // grid row -> state (live states only), or null: row = state
 };
 
 struct WinFinArgs {
@@ -139,6 +141,7 @@ struct WinFinArgs {
   const int* widths;
   const int* layers;
   const int* skip;
+  const int* list;   // grid row -> state (live states only), or null: row = state
   const double* part;
   double* overlap;
   int overlap_stride;
@@ -243,7 +246,8 @@ void launch_scale_convert(const S* x, D* y, int n, int batch, const double* norm
 void launch_fill_int(int* p, int v, int count, hipStream_t);
 template <typename S>
 void launch_overlap(const S* x, const double2* ref, int n, int batch, double* part, int* counter,
-                    double2* out, int out_stride, const int* skip, hipStream_t);
+                    double2* out, int out_stride, const int* skip, hipStream_t, const int* list = nullptr,
+                    int rows = 0);
 template <typename S> void launch_basis0(S* psi, int n, int batch, const int* skip, hipStream_t);
 void launch_chain2_step(const Chain2Args&, int nblk, int batch, hipStream_t);
 template <typename S> void launch_window_pass(const WinPassArgs<S>&, int nblk, int batch, hipStream_t);
@@ -251,7 +255,7 @@ template <typename S> void launch_fwd_pass(const FwdArgs<S>&, int nblk, int batc
 void launch_window_finalize(const WinFinArgs&, int batch, hipStream_t);
 // pre-sums groups of `group` window-pass partial blocks into the first slot of
 // each group (large grids, before the one-block finalize)
-void launch_window_reduce(double* part, int nblk, int group, int batch, hipStream_t st);
+void launch_window_reduce(double* part, int nblk, int group, int batch, const int* list, hipStream_t st);
 void launch_ran_gates(const RanArgs&, int batch, hipStream_t);
 void launch_amp0(const Amp0Args&, hipStream_t);
 
@@ -280,6 +284,8 @@ void launch_chain_dims(int* nn, int* ncur, const int* kk, int n, int step, int f
                        int batch, hipStream_t);
 void launch_goal_check(const double2* dot, int dot_stride, const double* goal, int* skip,
                        int* active, int* sweeps_done, int batch, hipStream_t);
+// list[0 .. live) = the states b < batch with skip[b] == 0, in increasing order
+void launch_live_list(const int* skip, int batch, int* list, hipStream_t);
 // C[b] (m x ncols) = A[b] (m x kdim) * op(B[b]); conj_b: op(B)[j][c] = conj(B[c][j])
 // with leading dimension ldb; dims read from device arrays (per state) when given.
 struct GemmDims {

@@ -244,12 +244,12 @@ __global__ __launch_bounds__(256) void norm2_kernel(const S* __restrict__ x, int
 template <typename S>
 __global__ __launch_bounds__(256) void overlap_kernel(const S* __restrict__ x, const double2* __restrict__ ref,
                                                       int n, double* part, int* counter, double2* out,
-                                                      int out_stride, const int* skip) {
+                                                      int out_stride, const int* skip, const int* list) {
   __shared__ double red[4 * 2];
   __shared__ double scratch[256];
   __shared__ double tot[2];
   __shared__ int s_flag;
-  const int b = blockIdx.y;
+  const int b = list ? list[blockIdx.y] : blockIdx.y;
   if (skip && skip[b]) return;
   const size_t len = (size_t)1 << n;
   const S* xb = x + (size_t)b * len;
@@ -865,19 +865,21 @@ void launch_norm2(const S* x, int n, int batch, double* part, int* counter, doub
 }
 template <typename S>
 void launch_overlap(const S* x, const double2* ref, int n, int batch, double* part, int* counter,
-                    double2* out, int out_stride, const int* skip, hipStream_t st) {
+                    double2* out, int out_stride, const int* skip, hipStream_t st, const int* list, int rows) {
   // whole chunks of >= 4096 amplitudes (len is a power of two), about 1024
-  // blocks over the batch
+  // blocks over the grid's rows (the batch, or the `rows` live states of list)
+  const int r = list ? rows : batch;
+  if (r < 1) return;
   const size_t len = (size_t)1 << n;
   size_t nblk = 1;
-  while (nblk * 2 * batch <= 1024 && (len / (nblk * 2)) >= 4096) nblk *= 2;
-  hipLaunchKernelGGL(overlap_kernel<S>, dim3((unsigned)nblk, batch), dim3(256), 0, st, x, ref, n, part, counter,
-                     out, out_stride, skip);
+  while (nblk * 2 * r <= 1024 && (len / (nblk * 2)) >= 4096) nblk *= 2;
+  hipLaunchKernelGGL(overlap_kernel<S>, dim3((unsigned)nblk, r), dim3(256), 0, st, x, ref, n, part, counter,
+                     out, out_stride, skip, list);
 }
 template void launch_overlap<float2>(const float2*, const double2*, int, int, double*, int*, double2*, int,
-                                     const int*, hipStream_t);
+                                     const int*, hipStream_t, const int*, int);
 template void launch_overlap<double2>(const double2*, const double2*, int, int, double*, int*, double2*, int,
-                                      const int*, hipStream_t);
+                                      const int*, hipStream_t, const int*, int);
 template void launch_norm2<float2>(const float2*, int, int, double*, int*, double*, int, hipStream_t);
 template void launch_norm2<double2>(const double2*, int, int, double*, int*, double*, int, hipStream_t);
 

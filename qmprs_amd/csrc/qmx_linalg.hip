@@ -1010,6 +1010,33 @@ void launch_goal_check(const double2* dot, int dot_stride, const double* goal, i
                      goal, skip, active, sweeps_done, batch);
 }
 
+// one block: chunks of 1024 states, each compacted by a block-wide exclusive
+// scan of the live flags
+__global__ __launch_bounds__(1024) void live_list_kernel(const int* skip, int batch, int* list) {
+  __shared__ int sc[1024];
+  int base = 0;
+  for (int b0 = 0; b0 < batch; b0 += 1024) {
+    const int b = b0 + (int)threadIdx.x;
+    const int live = (b < batch && !skip[b]) ? 1 : 0;
+    sc[threadIdx.x] = live;
+    __syncthreads();
+    for (int off = 1; off < 1024; off <<= 1) {
+      const int v = threadIdx.x >= (unsigned)off ? sc[threadIdx.x - off] : 0;
+      __syncthreads();
+      sc[threadIdx.x] += v;
+      __syncthreads();
+    }
+    if (live) list[base + sc[threadIdx.x] - 1] = b;
+    const int total = sc[1023];
+    __syncthreads();
+    base += total;
+  }
+}
+
+void launch_live_list(const int* skip, int batch, int* list, hipStream_t st) {
+  hipLaunchKernelGGL(live_list_kernel, dim3(1), dim3(1024), 0, st, skip, batch, list);
+}
+
 void launch_gemm(const double2* a, size_t a_stride, const double2* bm, size_t b_stride, double2* c,
                  size_t c_stride, const GemmDims& d, int conj_b, int batch, hipStream_t st) {
   size_t m = (size_t)1 << d.rows_log2;

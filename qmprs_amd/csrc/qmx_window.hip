@@ -259,7 +259,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(WPE))) void
   __shared__ int xol[2][16];
   __shared__ int rdep[8];
   __shared__ uint32_t hoff[STAGED ? 256 : 1];
-  const int b = blockIdx.y;
+  const int b = a.list ? a.list[blockIdx.y] : blockIdx.y;
   const int n = a.n;
   if (a.skip && a.skip[b]) return;
   const int kb = a.layers[b] * n;
@@ -455,7 +455,7 @@ __global__ __launch_bounds__(256) void fwd_pass_kernel(FwdArgs<S> a) {
   extern __shared__ char wsm[];
   __shared__ c2 gmat[kMaxFwdGates][16];
   __shared__ int gcode[kMaxFwdGates];
-  const int b = blockIdx.y;
+  const int b = a.list ? a.list[blockIdx.y] : blockIdx.y;
   const int n = a.n;
   if (a.skip && a.skip[b]) return;
   const int lb = a.layers[b];
@@ -563,7 +563,7 @@ __global__ __launch_bounds__(256) void window_finalize_kernel(WinFinArgs a) {
   __shared__ double2 es[16];
   __shared__ double2 rm[16];
   __shared__ double part_sum[512];
-  const int b = blockIdx.x;
+  const int b = a.list ? a.list[blockIdx.x] : blockIdx.x;
   const int n = a.n;
   if (a.skip && a.skip[b]) return;
   const int kb = a.layers[b] * n;
@@ -713,8 +713,8 @@ void launch_window_finalize(const WinFinArgs& a, int batch, hipStream_t st) {
 
 // block k of state b sums partial slots [k group, min((k + 1) group, nblk))
 // into slot k group (each thread owns elements e, e + 256 of every slot)
-__global__ __launch_bounds__(256) void window_reduce_kernel(double* part, int nblk, int group) {
-  double* pb = part + (size_t)blockIdx.y * nblk * 512;
+__global__ __launch_bounds__(256) void window_reduce_kernel(double* part, int nblk, int group, const int* list) {
+  double* pb = part + (size_t)(list ? list[blockIdx.y] : blockIdx.y) * nblk * 512;
   const int q0 = blockIdx.x * group, q1 = min(q0 + group, nblk);
   for (int e = threadIdx.x; e < 512; e += blockDim.x) {
     double s = 0.0;
@@ -727,9 +727,9 @@ __global__ __launch_bounds__(256) void window_reduce_kernel(double* part, int nb
   }
 }
 
-void launch_window_reduce(double* part, int nblk, int group, int batch, hipStream_t st) {
+void launch_window_reduce(double* part, int nblk, int group, int batch, const int* list, hipStream_t st) {
   hipLaunchKernelGGL(window_reduce_kernel, dim3((nblk + group - 1) / group, batch), dim3(256), 0, st, part,
-                     nblk, group);
+                     nblk, group, list);
 }
 
 }  // namespace qmx
