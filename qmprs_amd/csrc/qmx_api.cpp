@@ -1107,6 +1107,8 @@ void qmx_copy_options(const qmx_handle* from, qmx_handle* to) {
   to->fwd_blocks_per_cu = from->fwd_blocks_per_cu;
   to->staged_windows = from->staged_windows;
   to->relax_milli = from->relax_milli;
+  to->jacobi_sweep_cap = from->jacobi_sweep_cap;
+  to->fail_unitarity = from->fail_unitarity;
 }
 
 // =================================================================== C-ABI ==

@@ -97,7 +97,61 @@ size_t plan(int count, const qmx_ragged_state* st, int L, int S, int sv_dtype, i
   return off + 256;
 }
 
+// One bucket on stream ws: gather the inputs into the bucket's contiguous
+// batch (device-to-device copies), one grouped encode, scatter gates, widths,
+// layer counts and fidelities back to each state's buffers (only the encode
+// when scatter is false).  Returns a QMX status; msg names the failure.
+int issue_bucket(qmx_handle* wh, const Bucket& b, const qmx_ragged_state* states, char* base, int L, int S,
+                 int sv_dtype, int sweep_dtype, double thr, double cutoff, hipStream_t ws, bool scatter,
+                 std::string& msg) {
+  const int B = static_cast<int>(b.idx.size());
+  const size_t len = static_cast<size_t>(1) << b.n;
+  const size_t svb = len * sv_bytes(sv_dtype);
+  const size_t gb = static_cast<size_t>(L) * b.n * 16 * 16, wb = static_cast<size_t>(L) * b.n * sizeof(int);
+  const size_t fb = static_cast<size_t>(S + 1) * 16;
+  for (int k = 0; k < B; ++k)
+    if (hipMemcpyAsync(base + b.off_in + k * svb, states[b.idx[k]].statevector, svb, hipMemcpyDeviceToDevice,
+                       ws) != hipSuccess) {
+      msg = "gather";
+      return QMX_ERR_HIP;
+    }
+  qmx_encode_config cfg{b.n, B, b.gchi.back(), L, S, sv_dtype, sweep_dtype, thr, cutoff};
+  const int rc = qmx_encode_grouped(wh, &cfg, b.gfirst.data(), b.gcount.data(), b.gchi.data(),
+                                    static_cast<int>(b.gchi.size()), base + b.off_in, base + b.off_gates,
+                                    reinterpret_cast<int*>(base + b.off_widths),
+                                    reinterpret_cast<int*>(base + b.off_used), base + b.off_fid, base + b.off_ws,
+                                    b.ws_bytes, ws);
+  if (rc != QMX_OK) {
+    msg = std::string("bucket N=") + std::to_string(b.n) + ": " + qmx_last_error(wh);
+    return rc;
+  }
+  if (!scatter) return QMX_OK;
+  for (int k = 0; k < B; ++k) {
+    const qmx_ragged_state& s = states[b.idx[k]];
+    const bool ok =
+        hipMemcpyAsync(s.gates, base + b.off_gates + k * gb, gb, hipMemcpyDeviceToDevice, ws) == hipSuccess &&
+        hipMemcpyAsync(s.widths, base + b.off_widths + k * wb, wb, hipMemcpyDeviceToDevice, ws) == hipSuccess &&
+        hipMemcpyAsync(s.layers_used, base + b.off_used + k * sizeof(int), sizeof(int), hipMemcpyDeviceToDevice,
+                       ws) == hipSuccess &&
+        hipMemcpyAsync(s.fidelity, base + b.off_fid + k * fb, fb, hipMemcpyDeviceToDevice, ws) == hipSuccess;
+    if (!ok) {
+      msg = "scatter";
+      return QMX_ERR_HIP;
+    }
+  }
+  return QMX_OK;
+}
+
+bool states_valid(int count, const qmx_ragged_state* states) {
+  for (int i = 0; i < count; ++i)
+    if (!states[i].statevector || !states[i].gates || !states[i].widths || !states[i].layers_used ||
+        !states[i].fidelity)
+      return false;
+  return true;
+}
+
 }  // namespace
+
 
 extern "C" {
 
@@ -117,10 +171,7 @@ int qmx_encode_ragged(qmx_handle* h, int count, const qmx_ragged_state* states, 
   std::vector<Bucket> buckets;
   const size_t need = plan(count, states, num_layers, num_sweeps, sv_dtype, sweep_dtype, buckets);
   if (need == 0) return QMX_ERR_INVALID;
-  for (int i = 0; i < count; ++i)
-    if (!states[i].statevector || !states[i].gates || !states[i].widths || !states[i].layers_used ||
-        !states[i].fidelity)
-      return QMX_ERR_INVALID;
+  if (!states_valid(count, states)) return QMX_ERR_INVALID;
   if (!workspace || workspace_bytes < need) return QMX_ERR_WORKSPACE;
   const int device = qmx_handle_device(h);
   if (hipSetDevice(device) != hipSuccess) return QMX_ERR_HIP;
@@ -162,41 +213,11 @@ int qmx_encode_ragged(qmx_handle* h, int count, const qmx_ragged_state* states, 
     streams[w] = ws;
     for (size_t bi = next.fetch_add(1); bi < buckets.size() && status.load() == QMX_OK;
          bi = next.fetch_add(1)) {
-      const Bucket& b = buckets[bi];
-      const int B = static_cast<int>(b.idx.size());
-      const size_t len = static_cast<size_t>(1) << b.n;
-      const size_t svb = len * sv_bytes(sv_dtype);
-      const size_t gb = static_cast<size_t>(L) * b.n * 16 * 16, wb = static_cast<size_t>(L) * b.n * sizeof(int);
-      const size_t fb = static_cast<size_t>(S + 1) * 16;
-      bool ok = true;
-      for (int k = 0; k < B && ok; ++k)
-        ok = hipMemcpyAsync(base + b.off_in + k * svb, states[b.idx[k]].statevector, svb,
-                            hipMemcpyDeviceToDevice, ws) == hipSuccess;
-      if (!ok) {
-        fail(QMX_ERR_HIP, "gather");
-        break;
-      }
-      qmx_encode_config cfg{b.n, B, b.gchi.back(), L, S, sv_dtype, sweep_dtype, fidelity_threshold, cutoff};
-      int rc = qmx_encode_grouped(wh, &cfg, b.gfirst.data(), b.gcount.data(), b.gchi.data(),
-                                  static_cast<int>(b.gchi.size()), base + b.off_in, base + b.off_gates,
-                                  reinterpret_cast<int*>(base + b.off_widths), reinterpret_cast<int*>(base + b.off_used),
-                                  base + b.off_fid, base + b.off_ws, b.ws_bytes, ws);
+      std::string msg;
+      const int rc = issue_bucket(wh, buckets[bi], states, base, L, S, sv_dtype, sweep_dtype, fidelity_threshold,
+                                  cutoff, ws, true, msg);
       if (rc != QMX_OK) {
-        fail(rc, std::string("bucket N=") + std::to_string(b.n) + ": " + qmx_last_error(wh));
-        (void)hipStreamSynchronize(ws);
-        qmx_destroy(wh);
-        return;
-      }
-      for (int k = 0; k < B && ok; ++k) {
-        const qmx_ragged_state& s = states[b.idx[k]];
-        ok = hipMemcpyAsync(s.gates, base + b.off_gates + k * gb, gb, hipMemcpyDeviceToDevice, ws) == hipSuccess &&
-             hipMemcpyAsync(s.widths, base + b.off_widths + k * wb, wb, hipMemcpyDeviceToDevice, ws) == hipSuccess &&
-             hipMemcpyAsync(s.layers_used, base + b.off_used + k * sizeof(int), sizeof(int),
-                            hipMemcpyDeviceToDevice, ws) == hipSuccess &&
-             hipMemcpyAsync(s.fidelity, base + b.off_fid + k * fb, fb, hipMemcpyDeviceToDevice, ws) == hipSuccess;
-      }
-      if (!ok) {
-        fail(QMX_ERR_HIP, "scatter");
+        fail(rc, msg);
         break;
       }
     }
@@ -225,5 +246,6 @@ int qmx_encode_ragged(qmx_handle* h, int count, const qmx_ragged_state* states, 
   if (rc != QMX_OK) qmx_set_error_text(h, err_msg.c_str());
   return rc;
 }
+
 
 }  // extern "C"

@@ -1,0 +1,108 @@
+"""Captured ragged encode (qmx_encode_ragged_graph) against the eager one.
+
+The graph holds the whole ragged encode (buckets of equal N as branches on
+several streams, tuned launch options that do not change results); every
+replay must equal qmx_encode_ragged bit for bit, also with eager GPU work and
+synchronisations between replays and with new inputs written into the
+captured buffers.  Device errors surface through qmx_graph_sync.
+"""
+
+from __future__ import annotations
+
+import numpy as np
+import pytest
+import torch
+
+from oracle import qmprs_oracle as O
+
+pytestmark = [
+    pytest.mark.gpu,
+    pytest.mark.skip(reason="the captured ragged encode (qmx_encode_ragged_graph) is not in the library: "
+                            "its first GPU run ended in a host segmentation fault inside the capture call "
+                            "whose cause was not found, so the path was taken out"),
+]
+
+MIX = [(10, 8), (12, 16), (11, 8), (13, 16), (10, 16), (12, 8), (11, 16), (13, 8), (10, 8)]
+L, S = 3, 3
+
+
+def _states(seed0):
+    return [O.random_mps_state(n, 12, seed0 + i).astype(np.complex64) for i, (n, _) in enumerate(MIX)]
+
+
+def _equal(a, b):
+    for x, y in zip(a, b):
+        assert torch.equal(x.gates, y.gates)
+        assert torch.equal(x.widths, y.widths)
+        assert x.layers_used == y.layers_used
+        assert torch.equal(x.fidelity, y.fidelity)
+
+
+@pytest.mark.parametrize("autotune", [False, True], ids=["fixed", "autotuned"])
+def test_graph_replays_equal_eager(gpu, autotune):
+    from qmprs_amd import engine
+    chis = [c for _, c in MIX]
+    svs = _states(300)
+    print(f"eager ragged encode of {len(svs)} states, then capture (autotune={autotune})", flush=True)
+    eager = engine.encode_ragged(svs, chis, L, S)
+    torch.cuda.synchronize()
+    g = engine.RaggedGraph(svs, chis, L, S, autotune=autotune)
+    print(g.describe(), flush=True)
+    assert "streams" in g.describe()
+    for rep in range(3):
+        print(f"replay {rep}", flush=True)
+        got = g.run()
+        _equal(got, eager)
+        # eager work between replays: another ragged encode and a torch reduction
+        other = engine.encode_ragged(_states(900 + rep), chis, L, S)
+        torch.cuda.synchronize()
+        assert all(abs(r.fidelity[-1]) > 0 for r in other)
+        _ = torch.randn(1 << 20, device="cuda").abs().sum().item()
+    # new inputs through the same graph
+    new = _states(700)
+    print("replay with new inputs", flush=True)
+    got = g.run(new)
+    _equal(got, engine.encode_ragged(new, chis, L, S))
+    g.close()
+
+
+def test_graph_results_against_oracle(gpu):
+    """One replay against the FP64 oracle (complex128 sweeps), per state."""
+    from qmprs_amd import engine
+    mix = [(10, 8), (12, 16), (11, 8)]
+    svs = [O.random_mps_state(n, 12, 40 + i) for i, (n, _) in enumerate(mix)]
+    g = engine.RaggedGraph(svs, [c for _, c in mix], L, 2, sweep_dtype=torch.complex128, autotune=False)
+    res = g.run()
+    for sv, (n, chi), r in zip(svs, mix, res):
+        ref = O.prepare_state(sv, chi, num_layers=L, num_sweeps=2, polar="jacobi")
+        fid = abs(r.fidelity[-1].item())
+        print(f"N={n} chi={chi}: graph {fid:.12f} oracle {ref['fidelity']:.12f}")
+        assert r.layers_used == len(ref["layers"])
+        assert abs(fid - ref["fidelity"]) < 1e-9
+    g.close()
+
+
+def test_graph_device_error_reported_at_sync(gpu):
+    """A device error inside the graph (the unitarity check forced to fail from
+    the host) is collected on the device and raised by sync(), as ValueError."""
+    from qmprs_amd import _lib, engine
+    svs = _states(500)[:3]
+    chis = [c for _, c in MIX[:3]]
+    _lib.set_option(_lib.QMX_OPT_FAIL_UNITARITY, 1)
+    try:
+        g = engine.RaggedGraph(svs, chis, L, S, autotune=False)
+    finally:
+        _lib.set_option(_lib.QMX_OPT_FAIL_UNITARITY, 0)
+    g.launch()
+    with pytest.raises(ValueError, match="unitar"):
+        g.sync()
+    g.close()
+
+
+def test_graph_refuses_large_chi_truncation(gpu):
+    """chi > 256 needs the grid Jacobi, which reads convergence on the host:
+    capture is refused with ValueError (nothing is launched)."""
+    from qmprs_amd import engine
+    sv = O.random_mps_state(19, 4, 3).astype(np.complex64)
+    with pytest.raises(ValueError, match="captured"):
+        engine.RaggedGraph([sv], [300], 1, 0, autotune=False)
