@@ -236,7 +236,7 @@ ChiLayout chi_layout(Carver& c, int n, int batch, int chi, bool need_psi) {
       L.jlog = c.take<double>(jacobi_lds_log_doubles(L.nmax) * batch);
       L.jstate = c.take<int>(2 * static_cast<size_t>(batch));
     }
-    if (L.nmax > kJacobiBlockMax) L.jgrid = c.take<char>(jacobi_grid_bytes(L.nmax, batch));
+    if (L.nmax > kJacobiBlockMax) L.jgrid = c.take<char>(jacobi_block_bytes(L.nmax, batch));
   }
   L.nn = c.take<int>(static_cast<size_t>(batch) * (n + 1));
   L.kk = c.take<int>(static_cast<size_t>(batch) * (n + 1));
@@ -507,9 +507,9 @@ int run_chi(qmx_handle* h, int n, int batch, int chi, double cutoff, const void*
     auto cap = [&](int limit) { return h->jacobi_sweep_cap > 0 ? std::min(limit, h->jacobi_sweep_cap) : limit; };
     if (L.jgrid) {
       std::vector<int> conv(batch);
-      if (launch_jacobi_grid(L.a, L.v, nm2, L.ncur, nmax, batch, L.jgrid, conv.data(), cap(kJacobiGridSweeps),
-                             L.error, st) != 0)
-        return fail(h, QMX_ERR_HIP, "grid Jacobi");
+      if (launch_jacobi_block(L.a, L.v, nm2, L.ncur, nmax, batch, L.jgrid, conv.data(), cap(kJacobiGridSweeps),
+                              L.error, st) != 0)
+        return fail(h, QMX_ERR_HIP, "block Jacobi");
     } else if (L.jlog && jacobi_lds_setup(nmax) == hipSuccess) {
       launch_jacobi_lds(L.a, L.v, nm2, L.ncur, nmax, batch, L.jlog, L.jstate, cap(kJacobiSweeps), L.error, st);
     } else {

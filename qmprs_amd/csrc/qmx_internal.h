@@ -267,6 +267,11 @@ void launch_gram_reduce(const double2* part, int nchunks, const int* ncols_dev, 
 // grid-wide Jacobi for matrices above 512 x 512 (qmx_linalg.hip); returns -1
 // on a HIP failure; synchronises the stream once per sweep
 size_t jacobi_grid_bytes(int ncols_max, int batch);
+// block Jacobi for n > 512 (subproblems of 32 in LDS, block rotations applied
+// as small GEMMs); same interface as launch_jacobi_grid
+size_t jacobi_block_bytes(int ncols_max, int batch);
+int launch_jacobi_block(double2* a, double2* v, size_t stride, const int* ncols_dev, int ncols_max, int batch,
+                        void* ws, int* conv_host_buf, int max_sweeps, int* error, hipStream_t st);
 int launch_jacobi_grid(double2* a, double2* v, size_t stride, const int* ncols_dev, int ncols_max, int batch,
                        void* ws, int* conv_host_buf, int max_sweeps, int* error, hipStream_t st);
 // LDS-resident Jacobi for matrices up to 128 x 128 (qmx_linalg.hip)

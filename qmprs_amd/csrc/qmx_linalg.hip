@@ -713,6 +713,239 @@ __global__ void jgrid_sweep_end_kernel(int* conv, int* rot, const double2* acur_
   for (int j = threadIdx.x; j < n; j += blockDim.x) a[(size_t)j * ld + j] = ac[(size_t)j * ld + j];
 }
 
+// ----------------------------------------------------- block Jacobi -------
+// For Gram matrices too large for one workgroup (n > 512, chi > 256): a
+// two-sided block Jacobi over blocks of kBjBs columns.  The matrix is copied
+// into an npad x npad buffer (npad = n rounded up to 2 kBjBs, zero padding:
+// padded indices are never rotated, as every off-diagonal entry they have is
+// zero), and the nb = npad / kBjBs blocks are paired by the same circle
+// method as the scalar solvers: nb - 1 rounds per sweep, nb / 2 disjoint block
+// pairs per round.  Per round:
+//   solve: each pair's 2 kBjBs x 2 kBjBs Hermitian submatrix gets one cyclic
+//          Jacobi sweep in LDS (the scalar solvers' rotation test), the
+//          product U of its rotations is logged;
+//   A <- A J (in place), A' = (A J)^H J into the other buffer, V <- V J (in
+//   place), J the block-diagonal unitary of the round's U (columns permuted).
+// After each sweep a state none of whose off-diagonal entries passes the
+// rotation test is converged; the diagonal and V go back to the solver's
+// own layout for select_kernel.
+constexpr int kBjBs = 16;
+constexpr int kBjSub = 2 * kBjBs;   // subproblem size (32)
+constexpr int kBjRows = 64;         // rows per block of the apply kernel
+// The update A' = (A J)^H J is Hermitian only up to rounding, and a unitary
+// conjugation carries an anti-Hermitian part forward unchanged (of order eps
+// times the largest eigenvalue).  The solve rotates the Hermitian part of
+// each submatrix, so the convergence test measures the Hermitian part of the
+// off-diagonal entries too (testing the raw entries, a sweep never comes out
+// clean once they are at that level); the rotation floor is the scalar
+// solvers' (kJacobiAbsTol times the trace).
+
+QMX_DEV int bj_col(int nb, int r, int m, int l) {
+  int P, Q;
+  circle_pair(nb, r, m, P, Q);
+  return l < kBjBs ? P * kBjBs + l : Q * kBjBs + (l - kBjBs);
+}
+
+// padded copies: ap = A (n x n of ld) with zeros up to npad, vp = I; scale =
+// trace, conv = (n < 2), rot = 0
+__global__ __launch_bounds__(256) void bj_init_kernel(const double2* a_all, size_t stride, const int* ncols_dev,
+                                                      int ld, double2* ap_all, double2* vp_all, size_t pstride,
+                                                      int ldp, double* scale, int* conv, int* rot) {
+  const int b = blockIdx.y;
+  const int n = ncols_dev[b];
+  const double2* a = a_all + (size_t)b * stride;
+  double2* ap = ap_all + (size_t)b * pstride;
+  double2* vp = vp_all + (size_t)b * pstride;
+  for (size_t e = blockIdx.x * (size_t)blockDim.x + threadIdx.x; e < (size_t)ldp * ldp;
+       e += (size_t)gridDim.x * blockDim.x) {
+    const int i = (int)(e / ldp), j = (int)(e % ldp);
+    ap[e] = (i < n && j < n) ? a[(size_t)i * ld + j] : cz();
+    vp[e] = make_double2(i == j ? 1.0 : 0.0, 0.0);
+  }
+  if (blockIdx.x == 0 && threadIdx.x == 0) {
+    double tr = 0.0;
+    for (int i = 0; i < n; ++i) tr += fabs(a[(size_t)i * ld + i].x);
+    scale[b] = tr;
+    conv[b] = n < 2 ? 1 : 0;
+    rot[b] = 0;
+  }
+}
+
+// one block per (block pair m, state b): one Jacobi sweep of the pair's
+// symmetrised submatrix S in LDS; U (product of the rotations) -> log
+__global__ __launch_bounds__(256) void bj_solve_kernel(const double2* ap_all, size_t pstride, int ldp, int nb,
+                                                       int r, double2* log_all, const double* scale,
+                                                       const int* conv) {
+  __shared__ double2 S[kBjSub][kBjSub + 1];
+  __shared__ double2 U[kBjSub][kBjSub + 1];
+  __shared__ double4 R[kBjSub / 2];
+  __shared__ int col[kBjSub];
+  const int b = blockIdx.y;
+  if (conv[b]) return;
+  const int m = blockIdx.x;
+  const double2* ap = ap_all + (size_t)b * pstride;
+  if (threadIdx.x < kBjSub) col[threadIdx.x] = bj_col(nb, r, m, threadIdx.x);
+  __syncthreads();
+  for (int e = threadIdx.x; e < kBjSub * kBjSub; e += blockDim.x) {
+    const int i = e / kBjSub, j = e % kBjSub;
+    const double2 x = ap[(size_t)col[i] * ldp + col[j]], y = ap[(size_t)col[j] * ldp + col[i]];
+    S[i][j] = i == j ? make_double2(x.x, 0.0) : make_double2(0.5 * (x.x + y.x), 0.5 * (x.y - y.y));
+    U[i][j] = make_double2(i == j ? 1.0 : 0.0, 0.0);
+  }
+  __syncthreads();
+  const double tol = kJacobiAbsTol * scale[b] + 1e-300;
+  constexpr int np = kBjSub / 2;
+  for (int ir = 0; ir < kBjSub - 1; ++ir) {
+    if (threadIdx.x < np) {
+      int p, q;
+      circle_pair(kBjSub, ir, threadIdx.x, p, q);
+      bool rotated;
+      R[threadIdx.x] = jgrid_rotation(S[p][p].x, S[q][q].x, S[p][q], tol, rotated);
+    }
+    __syncthreads();
+    {
+      // 2x2 block (P', Q') of S: K^H S J (one block per thread, in place)
+      const int Pb = threadIdx.x / np, Qb = threadIdx.x % np;
+      int p0, p1, q0, q1;
+      circle_pair(kBjSub, ir, Pb, p0, p1);
+      circle_pair(kBjSub, ir, Qb, q0, q1);
+      const double4 RP = R[Pb], RQ = R[Qb];
+      const double2 K00 = make_double2(RP.x, 0.0), K01 = make_double2(RP.y, 0.0);
+      const double2 K10 = cscale(make_double2(RP.z, RP.w), -RP.y);
+      const double2 K11 = cscale(make_double2(RP.z, RP.w), RP.x);
+      const double2 J00 = make_double2(RQ.x, 0.0), J01 = make_double2(RQ.y, 0.0);
+      const double2 J10 = cscale(make_double2(RQ.z, RQ.w), -RQ.y);
+      const double2 J11 = cscale(make_double2(RQ.z, RQ.w), RQ.x);
+      const double2 a00 = S[p0][q0], a01 = S[p0][q1], a10 = S[p1][q0], a11 = S[p1][q1];
+      const double2 b00 = cadd(cmul(a00, J00), cmul(a01, J10));
+      const double2 b01 = cadd(cmul(a00, J01), cmul(a01, J11));
+      const double2 b10 = cadd(cmul(a10, J00), cmul(a11, J10));
+      const double2 b11 = cadd(cmul(a10, J01), cmul(a11, J11));
+      double2 c00 = cadd(cmulc(K00, b00), cmulc(K10, b10));
+      double2 c01 = cadd(cmulc(K00, b01), cmulc(K10, b11));
+      double2 c10 = cadd(cmulc(K01, b00), cmulc(K11, b10));
+      double2 c11 = cadd(cmulc(K01, b01), cmulc(K11, b11));
+      if (Pb == Qb) {
+        c00.y = 0.0;
+        c11.y = 0.0;
+        c01 = cz();
+        c10 = cz();
+      }
+      S[p0][q0] = c00;
+      S[p0][q1] = c01;
+      S[p1][q0] = c10;
+      S[p1][q1] = c11;
+      // U <- U J: items (row i, pair k), two per thread
+      for (int it = threadIdx.x; it < kBjSub * np; it += blockDim.x) {
+        const int i = it / np, k = it % np;
+        int p, q;
+        circle_pair(kBjSub, ir, k, p, q);
+        const double4 Rk = R[k];
+        const double2 em = make_double2(Rk.z, Rk.w);
+        const double2 x = U[i][p], y = U[i][q];
+        U[i][p] = cadd(cmul(x, make_double2(Rk.x, 0.0)), cmul(y, cscale(em, -Rk.y)));
+        U[i][q] = cadd(cmul(x, make_double2(Rk.y, 0.0)), cmul(y, cscale(em, Rk.x)));
+      }
+    }
+    __syncthreads();
+  }
+  double2* lg = log_all + ((size_t)b * (nb / 2) + m) * kBjSub * kBjSub;
+  for (int e = threadIdx.x; e < kBjSub * kBjSub; e += blockDim.x) lg[e] = U[e / kBjSub][e % kBjSub];
+}
+
+// out[i][col_m[l]] = sum_k in'[i][col_m[k]] U_m[k][l] for rows [64 blockIdx.x, +64)
+// and the round's pair m = blockIdx.y; in' = in (conjt = 0) or in^H (conjt = 1).
+// With conjt = 0, in == out is allowed (a block reads its whole tile first).
+__global__ __launch_bounds__(256) void bj_apply_kernel(const double2* in_all, double2* out_all, size_t pstride,
+                                                       int ldp, int nb, int r, const double2* log_all, int conjt,
+                                                       const int* conv) {
+  __shared__ double2 X[kBjRows][kBjSub + 1];
+  __shared__ double2 Um[kBjSub][kBjSub + 1];
+  __shared__ int col[kBjSub];
+  const int b = blockIdx.z;
+  if (conv[b]) return;
+  const int m = blockIdx.y, i0 = blockIdx.x * kBjRows;
+  const double2* in = in_all + (size_t)b * pstride;
+  double2* out = out_all + (size_t)b * pstride;
+  const double2* lg = log_all + ((size_t)b * (nb / 2) + m) * kBjSub * kBjSub;
+  if (threadIdx.x < kBjSub) col[threadIdx.x] = bj_col(nb, r, m, threadIdx.x);
+  for (int e = threadIdx.x; e < kBjSub * kBjSub; e += blockDim.x) Um[e / kBjSub][e % kBjSub] = lg[e];
+  __syncthreads();
+  if (!conjt) {
+    for (int e = threadIdx.x; e < kBjRows * kBjSub; e += blockDim.x) {
+      const int ii = e / kBjSub, k = e % kBjSub;
+      X[ii][k] = in[(size_t)(i0 + ii) * ldp + col[k]];
+    }
+  } else {
+    for (int e = threadIdx.x; e < kBjRows * kBjSub; e += blockDim.x) {
+      const int ii = e % kBjRows, k = e / kBjRows;
+      X[ii][k] = cconj(in[(size_t)col[k] * ldp + i0 + ii]);
+    }
+  }
+  __syncthreads();
+  // thread: column l = tid % 32, rows ii = tid / 32 + 8 t (t < 8)
+  const int l = threadIdx.x % kBjSub, ib = threadIdx.x / kBjSub;
+  double2 acc[kBjRows / 8];
+#pragma unroll
+  for (int t = 0; t < kBjRows / 8; ++t) acc[t] = cz();
+  for (int k = 0; k < kBjSub; ++k) {
+    const double2 u = Um[k][l];
+#pragma unroll
+    for (int t = 0; t < kBjRows / 8; ++t) acc[t] = cfma(X[ib + 8 * t][k], u, acc[t]);
+  }
+#pragma unroll
+  for (int t = 0; t < kBjRows / 8; ++t) out[(size_t)(i0 + ib + 8 * t) * ldp + col[l]] = acc[t];
+}
+
+// rot[b] = 1 if the Hermitian part of an off-diagonal entry of the current A
+// passes the rotation test (rot is clear on entry)
+__global__ __launch_bounds__(256) void bj_check_kernel(const double2* acur_all, size_t pstride, const int* ncols_dev,
+                                                       int ldp, const double* scale, const int* conv, int* rot) {
+  const int b = blockIdx.y;
+  if (conv[b]) return;
+  const int n = ncols_dev[b];
+  const double2* a = acur_all + (size_t)b * pstride;
+  const double tol = kJacobiAbsTol * scale[b] + 1e-300;
+  bool pass = false;
+  for (size_t e = blockIdx.x * (size_t)blockDim.x + threadIdx.x; e < (size_t)n * n;
+       e += (size_t)gridDim.x * blockDim.x) {
+    const int i = (int)(e / n), j = (int)(e % n);
+    if (j <= i) continue;
+    const double2 x = a[(size_t)i * ldp + j], y = a[(size_t)j * ldp + i];
+    const double ag = 0.5 * sqrt((x.x + y.x) * (x.x + y.x) + (x.y - y.y) * (x.y - y.y));
+    if (ag > tol && ag > 1e-15 * sqrt(fabs(a[(size_t)i * ldp + i].x * a[(size_t)j * ldp + j].x))) pass = true;
+  }
+  if (pass) rot[b] = 1;
+}
+
+// end of a sweep: a running state whose A passed no rotation test converges
+__global__ void bj_sweep_end_kernel(int* conv, int* rot, int batch) {
+  const int b = blockIdx.x * blockDim.x + threadIdx.x;
+  if (b >= batch) return;
+  if (!conv[b] && !rot[b]) conv[b] = 1;
+  rot[b] = 0;
+}
+
+// eigenvalues (diagonal of the final A) and V back into the solver's layout;
+// a state still running after the last sweep did not converge
+__global__ __launch_bounds__(256) void bj_store_kernel(const double2* cur_all, const double2* vp_all, size_t pstride,
+                                                       int ldp, double2* a_all, double2* v_all, size_t stride,
+                                                       int ld, const int* ncols_dev, const int* conv, int* error) {
+  const int b = blockIdx.y;
+  const int n = ncols_dev[b];
+  const double2* cur = cur_all + (size_t)b * pstride;
+  const double2* vp = vp_all + (size_t)b * pstride;
+  double2* a = a_all + (size_t)b * stride;
+  double2* v = v_all + (size_t)b * stride;
+  for (size_t e = blockIdx.x * (size_t)blockDim.x + threadIdx.x; e < (size_t)n * n;
+       e += (size_t)gridDim.x * blockDim.x) {
+    const int i = (int)(e / n), j = (int)(e % n);
+    v[(size_t)i * ld + j] = vp[(size_t)i * ldp + j];
+    if (i == j) a[(size_t)i * ld + i] = make_double2(cur[(size_t)i * ldp + i].x, 0.0);
+  }
+  if (blockIdx.x == 0 && threadIdx.x == 0 && !conv[b]) atomicOr(error, kErrEigNoConv);
+}
+
 // ---------------------------------------------------------------- select ----
 // Sort eigenvalues (diag of A) in decreasing order, pick k (rsum2 cutoff and
 // max bond chi; k = n when n <= chi), and write the sorted eigenvectors into
@@ -930,6 +1163,62 @@ int launch_jacobi_grid(double2* a, double2* v, size_t stride, const int* ncols_d
   hipLaunchKernelGGL(jgrid_sweep_end_kernel, dim3(batch), dim3(256), 0, st, conv, rot, cur, a, stride, ncols_dev,
                      ncols_max, 1);
   hipLaunchKernelGGL(jacobi_lds_check_kernel, dim3((batch + 255) / 256), dim3(256), 0, st, conv, batch, error);
+  return 0;
+}
+
+// block Jacobi workspace: padded A (two buffers) and V, the round's U log,
+// scale, conv, rot
+int bj_ldp(int ncols_max) { return (ncols_max + kBjRows - 1) / kBjRows * kBjRows; }
+
+size_t jacobi_block_bytes(int ncols_max, int batch) {
+  const size_t ldp = (size_t)bj_ldp(ncols_max), nb = ldp / kBjBs;
+  return 3 * (size_t)batch * ldp * ldp * sizeof(double2) + 256 +
+         (size_t)batch * (nb / 2) * kBjSub * kBjSub * sizeof(double2) + 256 +
+         (size_t)batch * (sizeof(double) + 2 * sizeof(int)) + 256;
+}
+
+int launch_jacobi_block(double2* a, double2* v, size_t stride, const int* ncols_dev, int ncols_max, int batch,
+                        void* ws, int* conv_host_buf, int max_sweeps, int* error, hipStream_t st) {
+  if (stride != (size_t)ncols_max * ncols_max) return -1;
+  const int ldp = bj_ldp(ncols_max), nb = ldp / kBjBs;
+  const size_t pstride = (size_t)ldp * ldp;
+  auto up = [](size_t x) { return (x + 255) & ~static_cast<size_t>(255); };
+  char* p = static_cast<char*>(ws);
+  double2* ap = reinterpret_cast<double2*>(p);
+  double2* a2 = ap + (size_t)batch * pstride;
+  double2* vp = a2 + (size_t)batch * pstride;
+  p += up(3 * (size_t)batch * pstride * sizeof(double2));
+  double2* lg = reinterpret_cast<double2*>(p);
+  p += up((size_t)batch * (nb / 2) * kBjSub * kBjSub * sizeof(double2));
+  double* scale = reinterpret_cast<double*>(p);
+  int* conv = reinterpret_cast<int*>(scale + batch);
+  int* rot = conv + batch;
+  hipLaunchKernelGGL(bj_init_kernel, dim3(std::max(1, ldp * ldp / 256 / 16), batch), dim3(256), 0, st, a, stride,
+                     ncols_dev, ncols_max, ap, vp, pstride, ldp, scale, conv, rot);
+  double2* cur = ap;
+  double2* oth = a2;
+  const dim3 sgrid(nb / 2, batch), agrid(ldp / kBjRows, nb / 2, batch);
+  for (int s = 0; s < max_sweeps; ++s) {
+    for (int r = 0; r < nb - 1; ++r) {
+      hipLaunchKernelGGL(bj_solve_kernel, sgrid, dim3(256), 0, st, cur, pstride, ldp, nb, r, lg, scale, conv);
+      hipLaunchKernelGGL(bj_apply_kernel, agrid, dim3(256), 0, st, cur, cur, pstride, ldp, nb, r, lg, 0, conv);
+      hipLaunchKernelGGL(bj_apply_kernel, agrid, dim3(256), 0, st, cur, oth, pstride, ldp, nb, r, lg, 1, conv);
+      std::swap(cur, oth);
+      hipLaunchKernelGGL(bj_apply_kernel, agrid, dim3(256), 0, st, vp, vp, pstride, ldp, nb, r, lg, 0, conv);
+    }
+    hipLaunchKernelGGL(bj_check_kernel, dim3(256, batch), dim3(256), 0, st, cur, pstride, ncols_dev, ldp, scale,
+                       conv, rot);
+    hipLaunchKernelGGL(bj_sweep_end_kernel, dim3((batch + 255) / 256), dim3(256), 0, st, conv, rot, batch);
+    // stop once every state has converged (one host read per sweep)
+    if (hipMemcpyAsync(conv_host_buf, conv, sizeof(int) * batch, hipMemcpyDeviceToHost, st) != hipSuccess ||
+        hipStreamSynchronize(st) != hipSuccess)
+      return -1;
+    bool all = true;
+    for (int b = 0; b < batch; ++b) all = all && conv_host_buf[b];
+    if (all) break;
+  }
+  hipLaunchKernelGGL(bj_store_kernel, dim3(std::max(1, ncols_max * ncols_max / 256 / 16), batch), dim3(256), 0, st,
+                     cur, vp, pstride, ldp, a, v, stride, ncols_max, ncols_dev, conv, error);
   return 0;
 }
 
