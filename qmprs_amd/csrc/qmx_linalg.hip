@@ -731,7 +731,7 @@ __global__ void jgrid_sweep_end_kernel(int* conv, int* rot, const double2* acur_
 // own layout for select_kernel.
 constexpr int kBjBs = 16;
 constexpr int kBjSub = 2 * kBjBs;   // subproblem size (32)
-constexpr int kBjRows = 64;         // rows per block of the apply kernel
+constexpr int kBjRows = 32;         // rows per block of the apply kernel (ldp is a multiple of 64)
 // The update A' = (A J)^H J is Hermitian only up to rounding, and a unitary
 // conjugation carries an anti-Hermitian part forward unchanged (of order eps
 // times the largest eigenvalue).  The solve rotates the Hermitian part of
@@ -739,6 +739,22 @@ constexpr int kBjRows = 64;         // rows per block of the apply kernel
 // off-diagonal entries too (testing the raw entries, a sweep never comes out
 // clean once they are at that level); the rotation floor is the scalar
 // solvers' (kJacobiAbsTol times the trace).
+
+// The rotation of jgrid_rotation (same test, same formulas) with a shorter
+// chain of dependent FP64 operations, for the latency-bound block solve: the
+// test compares squares, and one reciprocal square root of |g|^2 gives both
+// e^{-i phi} and tau; t = sign(tau) / (|tau| + sqrt(1 + tau^2)), c =
+// rsqrt(1 + t^2), s = t c.  Equal to jgrid_rotation up to rounding.
+QMX_DEV double4 bj_rotation(double al, double be, double2 g, double tol) {
+  const double ag2 = g.x * g.x + g.y * g.y;
+  const double thr = fmax(tol * tol, 1e-30 * fabs(al * be));
+  if (!(ag2 > thr)) return make_double4(1.0, 0.0, 1.0, 0.0);
+  const double ri = rsqrt(ag2);   // 1 / |g|
+  const double tau = 0.5 * (be - al) * ri;
+  const double t = (tau >= 0 ? 1.0 : -1.0) / (fabs(tau) + sqrt(1.0 + tau * tau));
+  const double c = rsqrt(1.0 + t * t);
+  return make_double4(c, t * c, g.x * ri, -g.y * ri);
+}
 
 QMX_DEV int bj_col(int nb, int r, int m, int l) {
   int P, Q;
@@ -799,8 +815,7 @@ __global__ __launch_bounds__(256) void bj_solve_kernel(const double2* ap_all, si
     if (threadIdx.x < np) {
       int p, q;
       circle_pair(kBjSub, ir, threadIdx.x, p, q);
-      bool rotated;
-      R[threadIdx.x] = jgrid_rotation(S[p][p].x, S[q][q].x, S[p][q], tol, rotated);
+      R[threadIdx.x] = bj_rotation(S[p][p].x, S[q][q].x, S[p][q], tol);
     }
     __syncthreads();
     {
@@ -853,20 +868,23 @@ __global__ __launch_bounds__(256) void bj_solve_kernel(const double2* ap_all, si
   for (int e = threadIdx.x; e < kBjSub * kBjSub; e += blockDim.x) lg[e] = U[e / kBjSub][e % kBjSub];
 }
 
-// out[i][col_m[l]] = sum_k in'[i][col_m[k]] U_m[k][l] for rows [64 blockIdx.x, +64)
+// out[i][col_m[l]] = sum_k in'[i][col_m[k]] U_m[k][l] for rows [32 blockIdx.x, +32)
 // and the round's pair m = blockIdx.y; in' = in (conjt = 0) or in^H (conjt = 1).
-// With conjt = 0, in == out is allowed (a block reads its whole tile first).
-__global__ __launch_bounds__(256) void bj_apply_kernel(const double2* in_all, double2* out_all, size_t pstride,
-                                                       int ldp, int nb, int r, const double2* log_all, int conjt,
-                                                       const int* conv) {
+// blockIdx.z = mat * batch + b: matrix mat (0: in0 -> out0, 1: in1 -> out1)
+// of state b, so that two independent updates share one launch.  With
+// conjt = 0, in == out is allowed (a block reads its whole tile first).
+__global__ __launch_bounds__(256) void bj_apply_kernel(const double2* in0, double2* out0, const double2* in1,
+                                                       double2* out1, size_t pstride, int ldp, int nb, int r,
+                                                       const double2* log_all, int conjt, const int* conv,
+                                                       int batch) {
   __shared__ double2 X[kBjRows][kBjSub + 1];
   __shared__ double2 Um[kBjSub][kBjSub + 1];
   __shared__ int col[kBjSub];
-  const int b = blockIdx.z;
+  const int mat = blockIdx.z / batch, b = blockIdx.z % batch;
   if (conv[b]) return;
   const int m = blockIdx.y, i0 = blockIdx.x * kBjRows;
-  const double2* in = in_all + (size_t)b * pstride;
-  double2* out = out_all + (size_t)b * pstride;
+  const double2* in = (mat ? in1 : in0) + (size_t)b * pstride;
+  double2* out = (mat ? out1 : out0) + (size_t)b * pstride;
   const double2* lg = log_all + ((size_t)b * (nb / 2) + m) * kBjSub * kBjSub;
   if (threadIdx.x < kBjSub) col[threadIdx.x] = bj_col(nb, r, m, threadIdx.x);
   for (int e = threadIdx.x; e < kBjSub * kBjSub; e += blockDim.x) Um[e / kBjSub][e % kBjSub] = lg[e];
@@ -883,18 +901,19 @@ __global__ __launch_bounds__(256) void bj_apply_kernel(const double2* in_all, do
     }
   }
   __syncthreads();
-  // thread: column l = tid % 32, rows ii = tid / 32 + 8 t (t < 8)
+  // thread: column l = tid % 32, rows ii = tid / 32 + 8 t (t < kBjRows / 8)
+  constexpr int kPer = kBjRows / 8;
   const int l = threadIdx.x % kBjSub, ib = threadIdx.x / kBjSub;
-  double2 acc[kBjRows / 8];
+  double2 acc[kPer];
 #pragma unroll
-  for (int t = 0; t < kBjRows / 8; ++t) acc[t] = cz();
+  for (int t = 0; t < kPer; ++t) acc[t] = cz();
   for (int k = 0; k < kBjSub; ++k) {
     const double2 u = Um[k][l];
 #pragma unroll
-    for (int t = 0; t < kBjRows / 8; ++t) acc[t] = cfma(X[ib + 8 * t][k], u, acc[t]);
+    for (int t = 0; t < kPer; ++t) acc[t] = cfma(X[ib + 8 * t][k], u, acc[t]);
   }
 #pragma unroll
-  for (int t = 0; t < kBjRows / 8; ++t) out[(size_t)(i0 + ib + 8 * t) * ldp + col[l]] = acc[t];
+  for (int t = 0; t < kPer; ++t) out[(size_t)(i0 + ib + 8 * t) * ldp + col[l]] = acc[t];
 }
 
 // rot[b] = 1 if the Hermitian part of an off-diagonal entry of the current A
@@ -1168,7 +1187,7 @@ int launch_jacobi_grid(double2* a, double2* v, size_t stride, const int* ncols_d
 
 // block Jacobi workspace: padded A (two buffers) and V, the round's U log,
 // scale, conv, rot
-int bj_ldp(int ncols_max) { return (ncols_max + kBjRows - 1) / kBjRows * kBjRows; }
+int bj_ldp(int ncols_max) { return (ncols_max + 63) / 64 * 64; }
 
 size_t jacobi_block_bytes(int ncols_max, int batch) {
   const size_t ldp = (size_t)bj_ldp(ncols_max), nb = ldp / kBjBs;
@@ -1197,14 +1216,16 @@ int launch_jacobi_block(double2* a, double2* v, size_t stride, const int* ncols_
                      ncols_dev, ncols_max, ap, vp, pstride, ldp, scale, conv, rot);
   double2* cur = ap;
   double2* oth = a2;
-  const dim3 sgrid(nb / 2, batch), agrid(ldp / kBjRows, nb / 2, batch);
+  const dim3 sgrid(nb / 2, batch), agrid2(ldp / kBjRows, nb / 2, 2 * batch), agrid(ldp / kBjRows, nb / 2, batch);
   for (int s = 0; s < max_sweeps; ++s) {
     for (int r = 0; r < nb - 1; ++r) {
       hipLaunchKernelGGL(bj_solve_kernel, sgrid, dim3(256), 0, st, cur, pstride, ldp, nb, r, lg, scale, conv);
-      hipLaunchKernelGGL(bj_apply_kernel, agrid, dim3(256), 0, st, cur, cur, pstride, ldp, nb, r, lg, 0, conv);
-      hipLaunchKernelGGL(bj_apply_kernel, agrid, dim3(256), 0, st, cur, oth, pstride, ldp, nb, r, lg, 1, conv);
+      // A <- A J and V <- V J (in place) in one launch, then A' = (A J)^H J
+      hipLaunchKernelGGL(bj_apply_kernel, agrid2, dim3(256), 0, st, cur, cur, vp, vp, pstride, ldp, nb, r, lg, 0,
+                         conv, batch);
+      hipLaunchKernelGGL(bj_apply_kernel, agrid, dim3(256), 0, st, cur, oth, cur, oth, pstride, ldp, nb, r, lg, 1,
+                         conv, batch);
       std::swap(cur, oth);
-      hipLaunchKernelGGL(bj_apply_kernel, agrid, dim3(256), 0, st, vp, vp, pstride, ldp, nb, r, lg, 0, conv);
     }
     hipLaunchKernelGGL(bj_check_kernel, dim3(256, batch), dim3(256), 0, st, cur, pstride, ncols_dev, ldp, scale,
                        conv, rot);
