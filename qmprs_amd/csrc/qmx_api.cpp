@@ -205,7 +205,7 @@ struct ChiLayout {
   int* error;
   double* jlog;      // LDS Jacobi rotation log (nmax <= 128)
   int* jstate;
-  char* jgrid;       // grid Jacobi state (nmax > 512)
+  char* jblock;      // block Jacobi state (nmax > 512)
   int i_first, nmax, nchunks_max, nblk_norm;
 };
 
@@ -236,7 +236,7 @@ ChiLayout chi_layout(Carver& c, int n, int batch, int chi, bool need_psi) {
       L.jlog = c.take<double>(jacobi_lds_log_doubles(L.nmax) * batch);
       L.jstate = c.take<int>(2 * static_cast<size_t>(batch));
     }
-    if (L.nmax > kJacobiBlockMax) L.jgrid = c.take<char>(jacobi_block_bytes(L.nmax, batch));
+    if (L.nmax > kJacobiBlockMax) L.jblock = c.take<char>(jacobi_block_bytes(L.nmax, batch));
   }
   L.nn = c.take<int>(static_cast<size_t>(batch) * (n + 1));
   L.kk = c.take<int>(static_cast<size_t>(batch) * (n + 1));
@@ -257,9 +257,9 @@ struct LayerLayout {
 };
 constexpr int kChain2MaxBlocks = 256;
 constexpr int kJacobiSweeps = 30;   // sweep launches of the LDS Jacobi per bond (16-17 run at 2r = 128)
-// sweeps of the grid Jacobi (2r > 512): up to 34 were needed on the Gram
-// matrices of N=26 random-MPS states of bond 1024 at chi = 512
-constexpr int kJacobiGridSweeps = 80;
+// sweeps of the block Jacobi (2r > 512): 1 to 34 per bond (16 on average)
+// on the Gram matrices of an N=26 random-MPS state of bond 1024 at chi = 512
+constexpr int kJacobiBlockSweeps = 80;
 constexpr int kJacobiGlobalSweeps = 40;   // one-block Jacobi on global memory (2r <= 512)
 constexpr int kWinMaxBlocks = 256;
 // the finalize sums up to this many block partials per state itself; above
@@ -505,9 +505,9 @@ int run_chi(qmx_handle* h, int n, int batch, int chi, double cutoff, const void*
     launch_gram_reduce(L.part, nchunks, L.ncur, nmax, L.a, nm2, batch, st);
     // sweep limits of the three eigensolvers; QMX_OPT_JACOBI_SWEEP_CAP lowers them
     auto cap = [&](int limit) { return h->jacobi_sweep_cap > 0 ? std::min(limit, h->jacobi_sweep_cap) : limit; };
-    if (L.jgrid) {
+    if (L.jblock) {
       std::vector<int> conv(batch);
-      if (launch_jacobi_block(L.a, L.v, nm2, L.ncur, nmax, batch, L.jgrid, conv.data(), cap(kJacobiGridSweeps),
+      if (launch_jacobi_block(L.a, L.v, nm2, L.ncur, nmax, batch, L.jblock, conv.data(), cap(kJacobiBlockSweeps),
                               L.error, st) != 0)
         return fail(h, QMX_ERR_HIP, "block Jacobi");
     } else if (L.jlog && jacobi_lds_setup(nmax) == hipSuccess) {

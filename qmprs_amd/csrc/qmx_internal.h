@@ -264,16 +264,12 @@ void launch_gram(const double2* y, size_t y_stride, const int* ncols_dev, int ro
                  int ncols_max, double2* part, int nchunks, int batch, hipStream_t);
 void launch_gram_reduce(const double2* part, int nchunks, const int* ncols_dev, int ncols_max,
                         double2* a, size_t a_stride, int batch, hipStream_t);
-// grid-wide Jacobi for matrices above 512 x 512 (qmx_linalg.hip); returns -1
-// on a HIP failure; synchronises the stream once per sweep
-size_t jacobi_grid_bytes(int ncols_max, int batch);
-// block Jacobi for n > 512 (subproblems of 32 in LDS, block rotations applied
-// as small GEMMs); same interface as launch_jacobi_grid
+// block Jacobi for Gram matrices above 512 x 512 (subproblems of 32 in LDS,
+// block rotations applied as small GEMMs); returns -1 on a HIP failure;
+// synchronises the stream once per sweep
 size_t jacobi_block_bytes(int ncols_max, int batch);
 int launch_jacobi_block(double2* a, double2* v, size_t stride, const int* ncols_dev, int ncols_max, int batch,
                         void* ws, int* conv_host_buf, int max_sweeps, int* error, hipStream_t st);
-int launch_jacobi_grid(double2* a, double2* v, size_t stride, const int* ncols_dev, int ncols_max, int batch,
-                       void* ws, int* conv_host_buf, int max_sweeps, int* error, hipStream_t st);
 // LDS-resident Jacobi for matrices up to 128 x 128 (qmx_linalg.hip)
 size_t jacobi_lds_log_doubles(int ncols_max);
 bool jacobi_lds_fits(int ncols_max);

@@ -421,296 +421,15 @@ __global__ void jacobi_lds_check_kernel(const int* state, int batch, int* error)
   if (b < batch && !state[b] && error) atomicOr(error, kErrEigNoConv);
 }
 
-// ------------------------------------------------------ grid Jacobi -------
-// For matrices too large for one workgroup (n > 512, chi = 512): the same
-// parallel ordering and rotations as jacobi_kernel, one launch per round.  A
-// round reads A from one buffer and writes the rotated A to the other, so every
-// thread can compute the rotations of its own two pairs from the unchanged
-// input (one kernel, no separate parameter pass); the thread of each diagonal
-// 2x2 block logs its pair's rotation.  After the n - 1 rounds of a sweep one
-// kernel applies the logged rotations to the rows of V held in LDS, in round
-// order.  After each sweep a state none of whose off-diagonal entries passes
-// the rotation test is converged (its next sweep would rotate nothing), its
-// eigenvalues (the diagonal) are copied into the first buffer if the last round
-// wrote the second, and later launches return at once for it.
-
+// ------------------------------------------------ circle ordering --------
+// Round r of the parallel (circle) ordering of n indices (n even): pair m of
+// the n / 2 disjoint pairs, p < q; the n - 1 rounds of a sweep pair every
+// two indices once.
 QMX_DEV void circle_pair(int n, int r, int m, int& p, int& q) {
   p = (m == 0) ? n - 1 : (r + m) % (n - 1);
   q = (r + n - 1 - m) % (n - 1);
   if (m == 0) q = r % (n - 1);
   if (p > q) { const int t = p; p = q; q = t; }
-}
-
-// The same with rr = r mod (n - 1) given (no division), the partner of index
-// i, and the pair index m of (i, j), i < j, as circle_pair numbers it
-QMX_DEV void circle_pair_rr(int n, int rr, int m, int& p, int& q) {
-  const int M = n - 1;
-  if (m == 0) {
-    p = rr;
-    q = M;
-    return;
-  }
-  int a = rr + m, c = rr - m;
-  if (a >= M) a -= M;
-  if (c < 0) c += M;
-  p = min(a, c);
-  q = max(a, c);
-}
-QMX_DEV int circle_partner_rr(int n, int rr, int i) {
-  const int M = n - 1;
-  if (i == M) return rr;
-  if (i == rr) return M;
-  int j = 2 * rr - i;   // in (-M, 2M)
-  if (j < 0) j += M;
-  if (j >= M) j -= M;
-  return j;
-}
-QMX_DEV int circle_index_rr(int n, int rr, int i, int j) {
-  const int M = n - 1;
-  if (j == M) return 0;
-  int a = i - rr, c = j - rr;   // in (-M, M)
-  if (a < 0) a += M;
-  if (c < 0) c += M;
-  return min(a, c);
-}
-
-// rotation (c, s, e^{-i phi}) of a pair from its diagonal entries and its
-// upper off-diagonal entry
-QMX_DEV double4 jgrid_rotation(double al, double be, double2 g, double tol, bool& rotated) {
-  const double ag = cabs_(g);
-  double c = 1.0, sn = 0.0;
-  double2 em = make_double2(1.0, 0.0);
-  rotated = false;
-  // one comparison against the larger threshold (the same test as
-  // ag > tol && ag > 1e-15 sqrt|al be|): al and be are used on every path,
-  // so their loads are not sunk behind the block loads that follow them
-  if (ag > fmax(tol, 1e-15 * sqrt(fabs(al * be)))) {
-    const double tau = (be - al) / (2.0 * ag);
-    const double t = (tau >= 0 ? 1.0 : -1.0) / (fabs(tau) + sqrt(1.0 + tau * tau));
-    c = 1.0 / sqrt(1.0 + t * t);
-    sn = t * c;
-    em = make_double2(g.x / ag, -g.y / ag);   // e^{-i phi}
-    rotated = true;
-  }
-  return make_double4(c, sn, em.x, em.y);
-}
-
-// V = I; the diagonal and the round-0 pair entries of A into diag / off
-__global__ void jgrid_init_kernel(const double2* a_all, double2* v_all, size_t stride, const int* ncols_dev,
-                                  int ld, double* diag, double2* off, int max_pairs, double* scale, int* conv,
-                                  int* rot) {
-  const int b = blockIdx.y;
-  const int n = ncols_dev[b];
-  const double2* a = a_all + (size_t)b * stride;
-  double2* v = v_all + (size_t)b * stride;
-  for (size_t e = blockIdx.x * (size_t)blockDim.x + threadIdx.x; e < (size_t)n * n;
-       e += (size_t)gridDim.x * blockDim.x) {
-    const int i = (int)(e / n), j = (int)(e % n);
-    v[(size_t)i * ld + j] = make_double2(i == j ? 1.0 : 0.0, 0.0);
-  }
-  if (blockIdx.x == 0 && n >= 2) {
-    for (int i = threadIdx.x; i < n; i += blockDim.x) diag[(size_t)b * ld + i] = a[(size_t)i * ld + i].x;
-    for (int m = threadIdx.x; m < n / 2; m += blockDim.x) {
-      int p, q;
-      circle_pair(n, 0, m, p, q);
-      off[(size_t)b * max_pairs + m] = a[(size_t)p * ld + q];
-    }
-  }
-  if (blockIdx.x == 0 && threadIdx.x == 0) {
-    double tr = 0.0;
-    for (int i = 0; i < n; ++i) tr += fabs(a[(size_t)i * ld + i].x);
-    scale[b] = tr;
-    conv[b] = n < 2 ? 1 : 0;
-    rot[b] = 0;
-  }
-}
-
-// one round: block (P, Q) of aout = J_P^H (block (P, Q) of ain) J_Q.  The
-// rotations come from diag_in / off_in (the diagonal and this round's pair
-// entries of ain, contiguous); the entries of aout that the next round (rn)
-// pairs, and the diagonal, also go to diag_out / off_out.  A thread takes row
-// pair P and kJgridQPerThread column pairs Q = Qh + h qs (qs = np / kJgridQPerThread
-// rounded up), so consecutive lanes stay on consecutive Q.
-constexpr int kJgridQPerThread = 1;   // 2 and 4 measured slower (fewer waves in flight)
-
-__global__ __launch_bounds__(256) void jgrid_round_kernel(const double2* ain_all, double2* aout_all, size_t stride,
-                                                          const int* ncols_dev, int ld, int r, int rn,
-                                                          const double* diag_in, const double2* off_in,
-                                                          double* diag_out, double2* off_out,
-                                                          double4* log_all, size_t log_stride, int max_pairs,
-                                                          const double* scale, const int* conv) {
-  const int b = blockIdx.y;
-  if (conv[b]) return;
-  const int n = ncols_dev[b], np = n / 2;
-  const int qs = (np + kJgridQPerThread - 1) / kJgridQPerThread;
-  const size_t e = blockIdx.x * (size_t)blockDim.x + threadIdx.x;
-  if (e >= (size_t)np * qs) return;
-  const double2* a = ain_all + (size_t)b * stride;
-  double2* ao = aout_all + (size_t)b * stride;
-  const int P = (int)(e / qs), Qh = (int)(e % qs);
-  const int rr = r % (n - 1), rrn = rn % (n - 1);
-  const double tol = kJacobiAbsTol * scale[b] + 1e-300;
-  const double* dg = diag_in + (size_t)b * ld;
-  const double2* of = off_in + (size_t)b * max_pairs;
-  int p0, p1;
-  circle_pair_rr(n, rr, P, p0, p1);
-  static_assert(kJgridQPerThread == 1, "one column pair per thread");
-  const int Q = Qh;
-  int q0, q1;
-  circle_pair_rr(n, rr, Q, q0, q1);
-  // load order: the rotation inputs (small, cached) first, then the block of
-  // A; the rotations only wait for the first loads and are computed while
-  // the block is in flight (loads complete in issue order)
-  const double dp0 = dg[p0], dp1 = dg[p1], dq0 = dg[q0], dq1 = dg[q1];
-  const double2 oP = of[P], oQ = of[Q];
-  const double2 a00 = a[(size_t)p0 * ld + q0], a01 = a[(size_t)p0 * ld + q1];
-  const double2 a10 = a[(size_t)p1 * ld + q0], a11 = a[(size_t)p1 * ld + q1];
-  bool rotated;
-  const double4 RP = jgrid_rotation(dp0, dp1, oP, tol, rotated);
-  const double4 RQ = jgrid_rotation(dq0, dq1, oQ, tol, rotated);
-  if (Qh == 0) log_all[(size_t)b * log_stride + (size_t)r * max_pairs + P] = RP;
-  const double2 K00 = make_double2(RP.x, 0.0), K01 = make_double2(RP.y, 0.0);
-  const double2 K10 = cscale(make_double2(RP.z, RP.w), -RP.y);
-  const double2 K11 = cscale(make_double2(RP.z, RP.w), RP.x);
-  {
-    const double2 J00 = make_double2(RQ.x, 0.0), J01 = make_double2(RQ.y, 0.0);
-    const double2 J10 = cscale(make_double2(RQ.z, RQ.w), -RQ.y);
-    const double2 J11 = cscale(make_double2(RQ.z, RQ.w), RQ.x);
-    const double2 b00 = cadd(cmul(a00, J00), cmul(a01, J10));
-    const double2 b01 = cadd(cmul(a00, J01), cmul(a01, J11));
-    const double2 b10 = cadd(cmul(a10, J00), cmul(a11, J10));
-    const double2 b11 = cadd(cmul(a10, J01), cmul(a11, J11));
-    double2 c00 = cadd(cmulc(K00, b00), cmulc(K10, b10));
-    double2 c01 = cadd(cmulc(K00, b01), cmulc(K10, b11));
-    double2 c10 = cadd(cmulc(K01, b00), cmulc(K11, b10));
-    double2 c11 = cadd(cmulc(K01, b01), cmulc(K11, b11));
-    if (P == Q) {
-      c00.y = 0.0; c11.y = 0.0;
-      c01 = cz(); c10 = cz();
-    }
-    ao[(size_t)p0 * ld + q0] = c00; ao[(size_t)p0 * ld + q1] = c01;
-    ao[(size_t)p1 * ld + q0] = c10; ao[(size_t)p1 * ld + q1] = c11;
-    const int ei[4] = {p0, p0, p1, p1}, ej[4] = {q0, q1, q0, q1};
-    const double2 ev[4] = {c00, c01, c10, c11};
-#pragma unroll
-    for (int k = 0; k < 4; ++k) {
-      const int i = ei[k], j = ej[k];
-      if (i == j) {
-        diag_out[(size_t)b * ld + i] = ev[k].x;
-      } else if (i < j && circle_partner_rr(n, rrn, i) == j) {
-        off_out[(size_t)b * max_pairs + circle_index_rr(n, rrn, i, j)] = ev[k];
-      }
-    }
-  }
-}
-
-// V <- V J over one sweep's logged rounds; rows [rpb blockIdx.x, +rpb) of
-// state blockIdx.y in LDS (rpb * n complex128)
-__global__ __launch_bounds__(1024) void jgrid_rows_kernel(double2* v_all, size_t stride, const int* ncols_dev,
-                                                          int ld, const double4* log_all, size_t log_stride,
-                                                          int max_pairs, int rounds, int rpb, const int* conv) {
-  extern __shared__ double2 vrows[];
-  const int b = blockIdx.y;
-  if (conv[b]) return;
-  const int n = ncols_dev[b];
-  const int r0 = blockIdx.x * rpb;
-  if (n < 2 || r0 >= n) return;
-  double2* v = v_all + (size_t)b * stride;
-  const double4* lg = log_all + (size_t)b * log_stride;
-  const int nr = min(rpb, n - r0);
-  const int np = n / 2;
-  for (int e = threadIdx.x; e < nr * n; e += blockDim.x) vrows[e] = v[(size_t)(r0 + e / n) * ld + e % n];
-  // this thread's items (row i, pair m) are the same in every round; the
-  // unsorted pair (a, c) of pair m advances by one (mod n - 1) per round, and
-  // the next round's log entries are loaded before this round's rotations
-  constexpr int kItems = 4;   // nr * np <= kItems * blockDim.x (checked by the host)
-  const int M = n - 1;
-  int it_i[kItems], it_m[kItems], it_a[kItems], it_c[kItems];
-  double4 nxt[kItems];
-#pragma unroll
-  for (int t = 0; t < kItems; ++t) {
-    const int e = threadIdx.x + t * blockDim.x;
-    it_i[t] = -1;
-    if (e < nr * np) {
-      it_i[t] = e / np;
-      it_m[t] = e % np;
-      it_a[t] = it_m[t] == 0 ? M : it_m[t] % M;            // round 0
-      it_c[t] = it_m[t] == 0 ? 0 : (M - it_m[t]) % M;
-      nxt[t] = lg[it_m[t]];
-    }
-  }
-  __syncthreads();
-  for (int r = 0; r < rounds; ++r) {
-#pragma unroll
-    for (int t = 0; t < kItems; ++t) {
-      if (it_i[t] < 0) continue;
-      const double4 R = nxt[t];
-      if (r + 1 < rounds) nxt[t] = lg[(size_t)(r + 1) * max_pairs + it_m[t]];
-      const int p = min(it_a[t], it_c[t]), q = max(it_a[t], it_c[t]);
-      const int i = it_i[t];
-      const double2 em = make_double2(R.z, R.w);
-      const double2 J00 = make_double2(R.x, 0.0), J01 = make_double2(R.y, 0.0);
-      const double2 J10 = cscale(em, -R.y), J11 = cscale(em, R.x);
-      const double2 x = vrows[i * n + p], y = vrows[i * n + q];
-      vrows[i * n + p] = cadd(cmul(x, J00), cmul(y, J10));
-      vrows[i * n + q] = cadd(cmul(x, J01), cmul(y, J11));
-      // next round: pair 0 is (n - 1, r + 1 mod M); pair m >= 1 is (r + 1 + m, r + 1 - m) mod M
-      if (it_m[t] == 0) {
-        it_c[t] = it_c[t] + 1 == M ? 0 : it_c[t] + 1;
-      } else {
-        it_a[t] = it_a[t] + 1 == M ? 0 : it_a[t] + 1;
-        it_c[t] = it_c[t] + 1 == M ? 0 : it_c[t] + 1;
-      }
-    }
-    __syncthreads();
-  }
-  for (int e = threadIdx.x; e < nr * n; e += blockDim.x) v[(size_t)(r0 + e / n) * ld + e % n] = vrows[e];
-}
-
-// rot[b] = 1 if an off-diagonal entry of the current A passes the rotation
-// test (the next sweep would rotate); rot is clear on entry
-__global__ __launch_bounds__(256) void jgrid_check_kernel(const double2* acur_all, size_t stride,
-                                                          const int* ncols_dev, int ld, const double* scale,
-                                                          const int* conv, int* rot) {
-  const int b = blockIdx.y;
-  if (conv[b]) return;
-  const int n = ncols_dev[b];
-  const double2* a = acur_all + (size_t)b * stride;
-  const double tol = kJacobiAbsTol * scale[b] + 1e-300;
-  bool pass = false;
-  for (size_t e = blockIdx.x * (size_t)blockDim.x + threadIdx.x; e < (size_t)n * n;
-       e += (size_t)gridDim.x * blockDim.x) {
-    const int i = (int)(e / n), j = (int)(e % n);
-    if (j <= i) continue;
-    const double ag = cabs_(a[(size_t)i * ld + j]);
-    if (ag > tol && ag > 1e-15 * sqrt(fabs(a[(size_t)i * ld + i].x * a[(size_t)j * ld + j].x))) pass = true;
-  }
-  if (pass) rot[b] = 1;
-}
-
-// End of a sweep (final = 0): a state whose A passed no rotation test is
-// converged (its next sweep would rotate nothing and is not run); its diagonal
-// moves to the first buffer when the last round wrote the second.  After the
-// last sweep (final = 1): the same copy for the states still running.
-__global__ void jgrid_sweep_end_kernel(int* conv, int* rot, const double2* acur_all, double2* a_all, size_t stride,
-                                       const int* ncols_dev, int ld, int final_pass) {
-  __shared__ int copy;
-  const int b = blockIdx.x;
-  if (threadIdx.x == 0) {
-    const bool running = !conv[b];
-    const bool done = final_pass ? running : (running && !rot[b]);
-    copy = done && acur_all != a_all;
-    if (!final_pass) {
-      if (done) conv[b] = 1;
-      rot[b] = 0;
-    }
-  }
-  __syncthreads();
-  if (!copy) return;
-  const int n = ncols_dev[b];
-  const double2* ac = acur_all + (size_t)b * stride;
-  double2* a = a_all + (size_t)b * stride;
-  for (int j = threadIdx.x; j < n; j += blockDim.x) a[(size_t)j * ld + j] = ac[(size_t)j * ld + j];
 }
 
 // ----------------------------------------------------- block Jacobi -------
@@ -740,11 +459,13 @@ constexpr int kBjRows = 32;         // rows per block of the apply kernel (ldp i
 // clean once they are at that level); the rotation floor is the scalar
 // solvers' (kJacobiAbsTol times the trace).
 
-// The rotation of jgrid_rotation (same test, same formulas) with a shorter
-// chain of dependent FP64 operations, for the latency-bound block solve: the
-// test compares squares, and one reciprocal square root of |g|^2 gives both
-// e^{-i phi} and tau; t = sign(tau) / (|tau| + sqrt(1 + tau^2)), c =
-// rsqrt(1 + t^2), s = t c.  Equal to jgrid_rotation up to rounding.
+// Rotation (c, s, e^{-i phi}) of a pair from its diagonal entries al, be and
+// its upper off-diagonal entry g: the scalar solvers' test (|g| above tol
+// and above 1e-15 sqrt|al be|) and formulas, with a short chain of dependent
+// FP64 operations for the latency-bound block solve: the test compares
+// squares, and one reciprocal square root of |g|^2 gives both e^{-i phi} and
+// tau = (be - al) / (2 |g|); t = sign(tau) / (|tau| + sqrt(1 + tau^2)),
+// c = rsqrt(1 + t^2), s = t c.
 QMX_DEV double4 bj_rotation(double al, double be, double2 g, double tol) {
   const double ag2 = g.x * g.x + g.y * g.y;
   const double thr = fmax(tol * tol, 1e-30 * fabs(al * be));
@@ -1107,82 +828,6 @@ void launch_jacobi(double2* a, double2* v, size_t stride, const int* ncols_dev, 
                    int batch, int max_sweeps, int* error, hipStream_t st) {
   hipLaunchKernelGGL(jacobi_kernel, dim3(batch), dim3(512), 0, st, a, v, stride, ncols_dev,
                      ncols_max, max_sweeps, error);
-}
-
-constexpr int kJgridRowsLds = 65536;   // LDS bytes of V rows per block in jgrid_rows_kernel (4 rows of 1024: all CUs busy; 8 or 2 rows measured slower)
-
-// second A buffer, the rotation log of one sweep, two sets of round inputs
-// (diagonal, pair entries), scale, conv, rot
-size_t jacobi_grid_bytes(int ncols_max, int batch) {
-  const size_t n = (size_t)ncols_max, np = n / 2;
-  return (size_t)batch * n * n * sizeof(double2) + (size_t)batch * (n - 1) * np * sizeof(double4) +
-         2 * (size_t)batch * (np * sizeof(double2) + n * sizeof(double)) +
-         (size_t)batch * (sizeof(double) + 2 * sizeof(int));
-}
-
-int launch_jacobi_grid(double2* a, double2* v, size_t stride, const int* ncols_dev, int ncols_max, int batch,
-                       void* ws, int* conv_host_buf, int max_sweeps, int* error, hipStream_t st) {
-  const int max_pairs = ncols_max / 2;
-  const int rounds = ncols_max - 1;
-  char* p = static_cast<char*>(ws);
-  double2* a2 = reinterpret_cast<double2*>(p);
-  p += (size_t)batch * stride * sizeof(double2);
-  double4* lg = reinterpret_cast<double4*>(p);
-  const size_t log_stride = (size_t)rounds * max_pairs;
-  p += (size_t)batch * log_stride * sizeof(double4);
-  double2* off[2];
-  double* diag[2];
-  for (int k = 0; k < 2; ++k) {
-    off[k] = reinterpret_cast<double2*>(p);
-    p += (size_t)batch * max_pairs * sizeof(double2);
-    diag[k] = reinterpret_cast<double*>(p);
-    p += (size_t)batch * ncols_max * sizeof(double);
-  }
-  double* scale = reinterpret_cast<double*>(p);
-  p += (size_t)batch * sizeof(double);
-  int* conv = reinterpret_cast<int*>(p);
-  int* rot = conv + batch;
-  if (stride != (size_t)ncols_max * ncols_max) return -1;
-  const int rpb = std::max(1, std::min(16, kJgridRowsLds / (ncols_max * (int)sizeof(double2))));
-  const size_t rows_lds = (size_t)rpb * ncols_max * sizeof(double2);
-  if (rpb * max_pairs > 4 * 1024) return -1;   // jgrid_rows_kernel: 4 items per thread
-  if (hipFuncSetAttribute(reinterpret_cast<const void*>(jgrid_rows_kernel),
-                          hipFuncAttributeMaxDynamicSharedMemorySize, (int)rows_lds) != hipSuccess)
-    return -1;
-  hipLaunchKernelGGL(jgrid_init_kernel, dim3(std::max(1, ncols_max * ncols_max / 256 / 64), batch), dim3(256), 0,
-                     st, a, v, stride, ncols_dev, ncols_max, diag[0], off[0], max_pairs, scale, conv, rot);
-  const size_t work = (size_t)max_pairs * ((max_pairs + kJgridQPerThread - 1) / kJgridQPerThread);
-  const dim3 rgrid((unsigned)((work + 255) / 256), batch);
-  const dim3 vgrid((unsigned)((ncols_max + rpb - 1) / rpb), batch);
-  double2* cur = a;   // buffer holding the current A of the running states
-  int k = 0;          // round-input set of the current round
-  for (int s = 0; s < max_sweeps; ++s) {
-    for (int r = 0; r < rounds; ++r) {
-      double2* nxt = (cur == a) ? a2 : a;
-      const int rn = (r + 1 == rounds) ? 0 : r + 1;
-      hipLaunchKernelGGL(jgrid_round_kernel, rgrid, dim3(256), 0, st, cur, nxt, stride, ncols_dev, ncols_max, r,
-                         rn, diag[k], off[k], diag[1 - k], off[1 - k], lg, log_stride, max_pairs, scale, conv);
-      cur = nxt;
-      k = 1 - k;
-    }
-    hipLaunchKernelGGL(jgrid_rows_kernel, vgrid, dim3(1024), rows_lds, st, v, stride, ncols_dev, ncols_max, lg,
-                       log_stride, max_pairs, rounds, rpb, conv);
-    hipLaunchKernelGGL(jgrid_check_kernel, dim3(256, batch), dim3(256), 0, st, cur, stride, ncols_dev, ncols_max,
-                       scale, conv, rot);
-    hipLaunchKernelGGL(jgrid_sweep_end_kernel, dim3(batch), dim3(256), 0, st, conv, rot, cur, a, stride,
-                       ncols_dev, ncols_max, 0);
-    // stop once every state has converged (one host read per sweep)
-    if (hipMemcpyAsync(conv_host_buf, conv, sizeof(int) * batch, hipMemcpyDeviceToHost, st) != hipSuccess ||
-        hipStreamSynchronize(st) != hipSuccess)
-      return -1;
-    bool all = true;
-    for (int b = 0; b < batch; ++b) all = all && conv_host_buf[b];
-    if (all) return 0;
-  }
-  hipLaunchKernelGGL(jgrid_sweep_end_kernel, dim3(batch), dim3(256), 0, st, conv, rot, cur, a, stride, ncols_dev,
-                     ncols_max, 1);
-  hipLaunchKernelGGL(jacobi_lds_check_kernel, dim3((batch + 255) / 256), dim3(256), 0, st, conv, batch, error);
-  return 0;
 }
 
 // block Jacobi workspace: padded A (two buffers) and V, the round's U log,

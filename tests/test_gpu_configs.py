@@ -264,7 +264,7 @@ def test_injected_non_unitary_gate_raises_value_error(gpu):
 
 
 @pytest.mark.parametrize("n,chi", [(12, 16), (11, 150), (18, 300)],
-                         ids=["lds-jacobi", "global-jacobi", "grid-jacobi"])
+                         ids=["lds-jacobi", "global-jacobi", "block-jacobi"])
 def test_eigensolver_sweep_cap_raises_runtime_error(gpu, n, chi):
     """QMX_OPT_JACOBI_SWEEP_CAP = 1 stops each of the three chi-truncation
     eigensolvers after one sweep; states that need more fail with

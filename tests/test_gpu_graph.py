@@ -100,7 +100,7 @@ def test_graph_device_error_reported_at_sync(gpu):
 
 
 def test_graph_refuses_large_chi_truncation(gpu):
-    """chi > 256 needs the grid Jacobi, which reads convergence on the host:
+    """chi > 256 needs the block Jacobi, which reads convergence on the host:
     capture is refused with ValueError (nothing is launched)."""
     from qmprs_amd import engine
     sv = O.random_mps_state(19, 4, 3).astype(np.complex64)

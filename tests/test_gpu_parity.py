@@ -62,7 +62,7 @@ def _gate_list_from_device(gates, widths, used):
     (4, 2, "uniform", 1), (6, 4, "uniform", 2), (8, 8, "mps", 3), (10, 16, "mps", 4),
     (10, 512, "uniform", 5), (12, 24, "mps", 6), (14, 32, "mps", 7),
     (11, 150, "uniform", 9),     # Gram matrices up to 256 x 256: one-block global Jacobi
-    (18, 300, "uniform", 8)])    # 600 x 600: grid Jacobi
+    (18, 300, "uniform", 8)])    # 600 x 600: block Jacobi
 def test_chi_truncate_matches_oracle(gpu, n, chi, kind, seed):
     eng = _engine()
     sv = _states(kind, n, seed, bond=2 * chi)
