@@ -211,6 +211,19 @@ int qmx_sweep(qmx_handle* h, int num_qubits, int batch, int num_layers, const in
               const double* goal, int* sweeps_done, void* circuit_out, void* workspace,
               size_t workspace_bytes, void* stream);
 
+/* qmx_sweep that also copies the gates after snap_first and snap_first + 1
+ * completed sweeps of this call (0: the gates it starts from; 0 <= snap_first
+ * < num_sweeps) into snapshots (device, 2 x B x L x N x 16 complex128: slot
+ * 0, then slot 1).  No counterpart in the reference: the learned warm start
+ * reads a gate's last two updates from them (one call per stretch of sweeps
+ * instead of three).  A snapshot whose sweep is not reached (every state at
+ * its goal earlier) is not written. */
+int qmx_sweep_snapshots(qmx_handle* h, int num_qubits, int batch, int num_layers, const int* layers_used,
+                        int num_sweeps, const void* target, int storage_dtype, void* gates,
+                        const int* widths, const void* psi_ref, void* fidelity, double* overlap,
+                        const double* goal, int* sweeps_done, void* circuit_out, int snap_first,
+                        void* snapshots, void* workspace, size_t workspace_bytes, void* stream);
+
 /* qmx_sweep without a workspace argument.  No counterpart in the reference:
  * for callers that cannot allocate device memory (code under HIP graph
  * capture, tight serving loops, harnesses that forbid hipMalloc).  The

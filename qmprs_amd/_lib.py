@@ -91,6 +91,8 @@ SIGNATURES = {
     "qmx_sweep_workspace": (_SZ, [_INT, _INT, _INT]),
     "qmx_sweep": (_INT, [_VP, _INT, _INT, _INT, _VP, _INT, _VP, _INT, _VP, _VP, _VP, _VP, _VP, _VP,
                          _VP, _VP, _VP, _SZ, _VP]),
+    "qmx_sweep_snapshots": (_INT, [_VP, _INT, _INT, _INT, _VP, _INT, _VP, _INT, _VP, _VP, _VP, _VP, _VP, _VP,
+                                   _VP, _VP, _INT, _VP, _VP, _SZ, _VP]),
     "qmx_sweep_noalloc": (_INT, [_VP, _INT, _INT, _INT, _VP, _INT, _VP, _INT, _VP, _VP, _VP, _VP, _VP, _VP,
                                  _VP, _VP, _VP]),
     "qmx_prepare_input_workspace": (_SZ, [_INT, _INT]),

@@ -744,6 +744,12 @@ struct SweepJob {
   // the batch.
   const int* list = nullptr;
   int live = 0;
+  // gate snapshots (qmx_sweep_snapshots): this job's gates after snap_first
+  // and snap_first + 1 completed sweeps of the call go to snaps and
+  // snaps + snap_stride (null: none)
+  int snap_first = -1;
+  double2* snaps = nullptr;
+  size_t snap_stride = 0;
   std::vector<FwdPass> fwd_plan;   // forward passes of the circuit rebuild
 };
 
@@ -907,6 +913,16 @@ void job_step_sweep(qmx_handle* h, SweepJob<S>& J, int s) {
   }
 }
 
+// the job's gates after `done` completed sweeps of the call, when requested
+template <typename S>
+int job_snapshot(qmx_handle* h, SweepJob<S>& J, int done) {
+  if (!J.snaps || done < J.snap_first || done > J.snap_first + 1) return QMX_OK;
+  const size_t bytes = (size_t)J.batch * J.lmax * J.n * 16 * sizeof(double2);
+  QMX_HIP(h, hipMemcpyAsync(J.snaps + (size_t)(done - J.snap_first) * J.snap_stride, J.gates, bytes,
+                            hipMemcpyDeviceToDevice, J.st));
+  return QMX_OK;
+}
+
 // record `ev` on `from` and make `to` wait for it
 int stream_join(qmx_handle* h, hipEvent_t ev, hipStream_t from, hipStream_t to) {
   QMX_HIP(h, hipEventRecord(ev, from));
@@ -924,6 +940,7 @@ int run_sweep_jobs(qmx_handle* h, std::vector<SweepJob<S>>& jobs) {
   const size_t nj = jobs.size();
   const hipStream_t s0 = jobs[0].st;
   for (auto& J : jobs) QMX_CALL(job_begin(h, J));
+  for (auto& J : jobs) QMX_CALL(job_snapshot(h, J, 0));
   const long long rebuild_launches =
       (n >= 5 && h->sweep_mode != QMX_SWEEP_STEP) ? (long long)plan_fwd(n, kmax).size()
                                                    : (long long)forward_circuit_passes(n, lmax).size();
@@ -958,6 +975,7 @@ int run_sweep_jobs(qmx_handle* h, std::vector<SweepJob<S>>& jobs) {
         if (J.active) job_step_sweep(h, J, s);
       QMX_LAUNCHED(h, "sweep");
     }
+    for (auto& J : jobs) QMX_CALL(job_snapshot(h, J, s + 1));
     for (size_t j = 1; j < nj; ++j) QMX_CALL(stream_join(h, h->join_ev[j], jobs[j].st, s0));
     swt.stop(windowed ? (long long)wins.size() : (long long)kmax);
   }
@@ -978,7 +996,8 @@ template <typename S>
 int sweep_dispatch(qmx_handle* h, int n, int batch, int lmax, const int* layers_used, int nsweeps,
                    const S* target, double2* gates, const int* widths, const double2* psi_ref,
                    double2* fidelity, double* overlap, const double* goal, int* sweeps_done,
-                   S* circuit_out, Carver& c, hipStream_t st) {
+                   S* circuit_out, Carver& c, hipStream_t st, int snap_first = -1,
+                   double2* snapshots = nullptr) {
   const bool split = h->split_sweeps && batch >= 2 && sweep_windowed(h, n);
   const size_t len = state_len(n);
   auto job = [&](int b0, int nb, hipStream_t js) {
@@ -997,6 +1016,11 @@ int sweep_dispatch(qmx_handle* h, int n, int batch, int lmax, const int* layers_
     J.goal = goal ? goal + b0 : nullptr;
     J.sweeps_done = sweeps_done ? sweeps_done + b0 : nullptr;
     J.circuit_out = circuit_out ? circuit_out + (size_t)b0 * len : nullptr;
+    if (snapshots && snap_first >= 0) {
+      J.snap_first = snap_first;
+      J.snaps = snapshots + (size_t)b0 * lmax * n * 16;
+      J.snap_stride = (size_t)batch * lmax * n * 16;
+    }
     J.L = sweep_layout<S>(c, n, nb);
     J.st = js;
     return J;
@@ -1377,25 +1401,40 @@ int qmx_sweep(qmx_handle* h, int num_qubits, int batch, int num_layers, const in
               const int* widths, const void* psi_ref, void* fidelity, double* overlap,
               const double* goal, int* sweeps_done, void* circuit_out, void* workspace,
               size_t workspace_bytes, void* stream) {
+  return qmx_sweep_snapshots(h, num_qubits, batch, num_layers, layers_used, num_sweeps, target, storage_dtype,
+                             gates, widths, psi_ref, fidelity, overlap, goal, sweeps_done, circuit_out, -1,
+                             nullptr, workspace, workspace_bytes, stream);
+}
+
+int qmx_sweep_snapshots(qmx_handle* h, int num_qubits, int batch, int num_layers, const int* layers_used,
+                        int num_sweeps, const void* target, int storage_dtype, void* gates,
+                        const int* widths, const void* psi_ref, void* fidelity, double* overlap,
+                        const double* goal, int* sweeps_done, void* circuit_out, int snap_first,
+                        void* snapshots, void* workspace, size_t workspace_bytes, void* stream) {
   QMX_CALL(check_common(h, num_qubits, batch));
   if (num_layers < 1 || num_sweeps < 0) return fail(h, QMX_ERR_INVALID, "num_layers / num_sweeps");
   if (!layers_used || !target || !gates || !widths || !workspace)
     return fail(h, QMX_ERR_INVALID, "null pointer");
   if (storage_dtype != QMX_C64 && storage_dtype != QMX_C128) return fail(h, QMX_ERR_INVALID, "storage_dtype");
+  if (snapshots && (snap_first < 0 || snap_first + 1 > num_sweeps))
+    return fail(h, QMX_ERR_INVALID, "snapshots: 0 <= snap_first < num_sweeps");
   if (workspace_bytes < qmx_sweep_workspace(num_qubits, batch, storage_dtype))
     return fail(h, QMX_ERR_WORKSPACE, "workspace too small");
   QMX_HIP(h, hipSetDevice(h->device));
   hipStream_t st = static_cast<hipStream_t>(stream);
   Carver c(workspace);
+  double2* snaps = static_cast<double2*>(snapshots);
   if (storage_dtype == QMX_C64)
     return sweep_dispatch<float2>(h, num_qubits, batch, num_layers, layers_used, num_sweeps,
                                   static_cast<const float2*>(target), static_cast<double2*>(gates), widths,
                                   static_cast<const double2*>(psi_ref), static_cast<double2*>(fidelity),
-                                  overlap, goal, sweeps_done, static_cast<float2*>(circuit_out), c, st);
+                                  overlap, goal, sweeps_done, static_cast<float2*>(circuit_out), c, st,
+                                  snap_first, snaps);
   return sweep_dispatch<double2>(h, num_qubits, batch, num_layers, layers_used, num_sweeps,
                                  static_cast<const double2*>(target), static_cast<double2*>(gates), widths,
                                  static_cast<const double2*>(psi_ref), static_cast<double2*>(fidelity),
-                                 overlap, goal, sweeps_done, static_cast<double2*>(circuit_out), c, st);
+                                 overlap, goal, sweeps_done, static_cast<double2*>(circuit_out), c, st,
+                                 snap_first, snaps);
 }
 
 size_t qmx_prepare_input_workspace(int num_qubits, int batch) {

@@ -180,8 +180,11 @@ class SweepResult:
 
 def sweep(target: torch.Tensor, gates: torch.Tensor, widths: torch.Tensor, layers_used: torch.Tensor,
           num_sweeps: int, psi_ref: torch.Tensor | None = None, goal: torch.Tensor | None = None,
-          circuit_out: bool = False) -> SweepResult:
-    """(e): environment sweeps, gates updated in place."""
+          circuit_out: bool = False, snapshots: tuple[int, torch.Tensor] | None = None) -> SweepResult:
+    """(e): environment sweeps, gates updated in place.  ``snapshots`` =
+    (first, buffer of shape (2,) + gates.shape): the gates after ``first``
+    and ``first + 1`` completed sweeps of this call are copied into buffer[0]
+    and buffer[1] (qmx_sweep_snapshots)."""
     b, length = target.shape
     n = num_qubits_of(length)
     num_layers = gates.shape[1]
@@ -193,10 +196,20 @@ def sweep(target: torch.Tensor, gates: torch.Tensor, widths: torch.Tensor, layer
     done = torch.zeros(b, dtype=torch.int32, device=dev)
     circ = torch.empty_like(target) if circuit_out else None
     ws = _lib.workspace(lib.qmx_sweep_workspace(n, b, dt), dev)
-    st = lib.qmx_sweep(h, n, b, num_layers, _lib.ptr(layers_used), num_sweeps, _lib.ptr(target), dt,
-                       _lib.ptr(gates), _lib.ptr(widths), _lib.ptr(psi_ref), _lib.ptr(fid),
-                       _lib.ptr(ov if num_sweeps > 0 else None), _lib.ptr(goal), _lib.ptr(done),
-                       _lib.ptr(circ), _lib.ptr(ws), ws.numel(), _lib.stream_ptr())
+    if snapshots is None:
+        st = lib.qmx_sweep(h, n, b, num_layers, _lib.ptr(layers_used), num_sweeps, _lib.ptr(target), dt,
+                           _lib.ptr(gates), _lib.ptr(widths), _lib.ptr(psi_ref), _lib.ptr(fid),
+                           _lib.ptr(ov if num_sweeps > 0 else None), _lib.ptr(goal), _lib.ptr(done),
+                           _lib.ptr(circ), _lib.ptr(ws), ws.numel(), _lib.stream_ptr())
+    else:
+        first, buf = snapshots
+        if tuple(buf.shape) != (2,) + tuple(gates.shape) or buf.dtype != gates.dtype or not buf.is_contiguous():
+            raise ValueError("snapshots: a contiguous buffer of shape (2,) + gates.shape")
+        st = lib.qmx_sweep_snapshots(h, n, b, num_layers, _lib.ptr(layers_used), num_sweeps, _lib.ptr(target),
+                                     dt, _lib.ptr(gates), _lib.ptr(widths), _lib.ptr(psi_ref), _lib.ptr(fid),
+                                     _lib.ptr(ov if num_sweeps > 0 else None), _lib.ptr(goal), _lib.ptr(done),
+                                     _lib.ptr(circ), int(first), _lib.ptr(buf), _lib.ptr(ws), ws.numel(),
+                                     _lib.stream_ptr())
     _lib.check(st, h)
     return SweepResult(fid, ov[:, :num_sweeps], done, circ)
 

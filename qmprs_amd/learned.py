@@ -320,21 +320,22 @@ def encode_warm(statevectors, bond_dimension: int, num_layers: int, num_sweeps: 
     done = torch.zeros(b, dtype=torch.int32, device=gates.device)
     accepted = torch.zeros(b, dtype=torch.int32, device=gates.device)
 
-    def run(k):
+    def run(k, snapshots=None):
         nonlocal done
-        r = engine.sweep(tgt, gates, widths, used, k, psi_ref=psi, goal=goal)
+        r = engine.sweep(tgt, gates, widths, used, k, psi_ref=psi, goal=goal, snapshots=snapshots)
         done = done + (r.sweeps_done if goal is not None else k)
         return r.fidelity[:, -1].abs()
 
     s, tried, fid = 0, 0, None
     j = first
+    # the gates after sweeps j - 2 and j - 1 come from the same call's snapshots
+    snaps = torch.empty((2,) + tuple(gates.shape), dtype=gates.dtype, device=gates.device)
     while j <= num_sweeps and j >= 2:
-        if j - 2 > s:
-            run(j - 2 - s)
-        ga = gates.clone()
-        run(1)
-        gb = gates.clone()
-        fid = run(1)
+        snaps[0].copy_(gates)
+        snaps[1].copy_(gates)
+        k = j - s
+        fid = run(k, snapshots=(k - 2, snaps))
+        ga, gb = snaps[0], snaps[1]
         s = j
         active = (fid < goal) if goal is not None else torch.ones_like(fid, dtype=torch.bool)
         if not bool(active.any()):

@@ -252,3 +252,29 @@ def test_warm_start_against_plain(gpu):
     assert int(res.accepted.sum()) > 0
     assert torch.all(res.fidelity >= goal - 1e-9)
     assert warm_steps.float().mean() < plain_steps.float().mean()
+
+
+@pytest.mark.parametrize("first", [0, 1, 2])
+def test_sweep_snapshots_equal_split_calls(gpu, first):
+    """qmx_sweep_snapshots: the gates after `first` and `first + 1` sweeps of
+    one call equal the gates of the same sweeps run as separate calls, bit for
+    bit (no goal: every call runs the same launches), and the call's final
+    gates and fidelity equal the split run's."""
+    from qmprs_amd import engine
+    n, chi, L, k = 11, 16, 3, 4
+    svs = np.stack([O.random_mps_state(n, 24, 60 + s) for s in range(3)]).astype(np.complex64)
+    p = engine.encode_prepare(svs, chi, L, sweep_dtype=torch.complex64)
+    g1 = p.gates.clone()
+    buf = torch.empty((2,) + tuple(g1.shape), dtype=g1.dtype, device=g1.device)
+    r1 = engine.sweep(p.target, g1, p.widths, p.layers_used, k, psi_ref=p.psi, snapshots=(first, buf))
+    g2 = p.gates.clone()
+    snaps = []
+    for done in range(k + 1):
+        if done in (first, first + 1):
+            snaps.append(g2.clone())
+        if done < k:
+            r2 = engine.sweep(p.target, g2, p.widths, p.layers_used, 1, psi_ref=p.psi)
+    torch.cuda.synchronize()
+    assert torch.equal(buf[0], snaps[0]) and torch.equal(buf[1], snaps[1])
+    assert torch.equal(g1, g2)
+    assert torch.equal(r1.fidelity[:, -1], r2.fidelity[:, -1])
