@@ -220,6 +220,8 @@ QMX_OPT_STAGED_WINDOWS = 5
 QMX_OPT_SWEEP_RELAXATION = 6
 QMX_OPT_JACOBI_SWEEP_CAP = 7
 QMX_OPT_FAIL_UNITARITY = 8
+QMX_OPT_FWD_MFMA = 9
+FWD_MFMA_DEFAULT = 1  # the library's default for QMX_OPT_FWD_MFMA
 
 
 def set_option(option: int, value: int) -> None:

@@ -35,6 +35,7 @@ struct qmx_handle {
   bool split_sweeps = true;             // QMX_OPT_SPLIT_SWEEPS
   int win_blocks_per_cu = 16;           // QMX_OPT_WINDOW_BLOCKS_PER_CU
   int fwd_blocks_per_cu = 4;            // QMX_OPT_FWD_BLOCKS_PER_CU
+  int fwd_mfma = 1;                     // QMX_OPT_FWD_MFMA
   int staged_windows = 1;               // QMX_OPT_STAGED_WINDOWS (0, 1, 2)
   int relax_milli = 1000;               // QMX_OPT_SWEEP_RELAXATION (omega * 1000)
   int jacobi_sweep_cap = 0;             // QMX_OPT_JACOBI_SWEEP_CAP (0: the solvers' own limits)
@@ -307,6 +308,59 @@ GroupMap make_group(int n, uint64_t p, uint64_t u) {
       g.restl |= 1u << local(bit);
     }
   return g;
+}
+
+// Lane roles of one stage of the matrix-core forward pass (tiles of 2^12
+// amplitudes).  Within an LDS access, the 32 lanes of a phase vary the 4 jl
+// bits and kq's low bit; with the tile swizzle (low 5 bits ^= bits 5-9 ^
+// bits 10-11) they fall on distinct banks when the 5 tile-local positions
+// have distinct residues (b mod 5 below 10, b - 10 above).  A stage that
+// reads or writes HBM (the first and the last) keeps jl on the 4 lowest rest
+// bits, whose accesses coalesce; only kq is chosen there.
+MfmaMap make_mfma_map(int n, uint64_t p, const GroupMap& g, bool hbm) {
+  int restl[8], restg[8], m = 0;
+  for (int bit = 0; bit < n && m < 8; ++bit)
+    if (((p >> bit) & 1ull) && !((g.mask >> bit) & 1ull)) {
+      restg[m] = bit;
+      restl[m] = __builtin_popcountll(p & ((1ull << bit) - 1ull));
+      ++m;
+    }
+  auto residue = [](int b) { return b < 10 ? b % 5 : b - 10; };
+  int best = -1, bsel = 0, bk = 0;
+  for (int sel = 0; sel < 256; ++sel) {
+    if (__builtin_popcount(sel) != 4 || (hbm && sel != 15)) continue;
+    for (int k0 = 0; k0 < 4; ++k0) {
+      int seen = 1 << residue(g.ux[k0]);
+      for (int i = 0; i < 8; ++i)
+        if ((sel >> i) & 1) seen |= 1 << residue(restl[i]);
+      const int score = __builtin_popcount(seen);
+      if (score > best) best = score, bsel = sel, bk = k0;
+    }
+  }
+  auto put = [](uint32_t& w, int i, int v) { w |= (uint32_t)v << (8 * i); };
+  MfmaMap mm{};
+  int nj = 0, nh = 0;
+  for (int i = 0; i < 8; ++i) {
+    if ((bsel >> i) & 1) {
+      put(mm.jl, nj, restl[i]);
+      put(mm.jg, nj++, restg[i]);
+    } else {
+      put(mm.hl, nh, restl[i]);
+      put(mm.hg, nh++, restg[i]);
+    }
+  }
+  int kx[2] = {bk, bk == 0 ? 1 : 0}, sx[2], ns = 0;
+  for (int q = 0; q < 4; ++q)
+    if (q != kx[0] && q != kx[1]) sx[ns++] = q;
+  for (int i = 0; i < 2; ++i) {
+    put(mm.ksl, i, g.ux[kx[i]]);
+    put(mm.ksl, 2 + i, g.ux[sx[i]]);
+    put(mm.ksg, i, g.ubit[kx[i]]);
+    put(mm.ksg, 2 + i, g.ubit[sx[i]]);
+    put(mm.kxsx, i, kx[i]);
+    put(mm.kxsx, 2 + i, sx[i]);
+  }
+  return mm;
 }
 
 // Forward (circuit-order) passes over the light cone of a bit window.  Two
@@ -662,6 +716,8 @@ int run_fwd(qmx_handle* h, int n, int batch, int lmax, const std::vector<FwdPass
     const uint64_t qmask = all & ~p;
     a.tmask = a.full ? qmask : (qmask & live);
     for (int k = 0; k < fp.nstage; ++k) a.gs[k] = make_group(n, p, fp.mask[k]);
+    if (h->fwd_mfma && t == kFwdTileBits && n <= 28)
+      for (int k = 0; k < fp.nstage; ++k) a.mm[k] = make_mfma_map(n, p, a.gs[k], k == 0 || k + 1 == fp.nstage);
     for (int k = 0; k <= fp.nstage; ++k) a.gstart[k] = fp.gstart[k];
     for (int j = 0; j < fp.gstart[fp.nstage]; ++j) a.fpos[j] = fp.fpos[j];
     a.psi = psi;
@@ -670,6 +726,7 @@ int run_fwd(qmx_handle* h, int n, int batch, int lmax, const std::vector<FwdPass
     a.layers = layers;
     a.skip = skip;
     a.list = list;
+    a.mfma = h->fwd_mfma;
     const long ntiles = 1l << __builtin_popcountll(a.tmask);
     const int nblk =
         (int)std::max(1l, std::min(ntiles, ((long)h->fwd_blocks_per_cu * h->cus + nrow - 1) / nrow));
@@ -1131,6 +1188,7 @@ void qmx_copy_options(const qmx_handle* from, qmx_handle* to) {
   to->fwd_blocks_per_cu = from->fwd_blocks_per_cu;
   to->staged_windows = from->staged_windows;
   to->relax_milli = from->relax_milli;
+  to->fwd_mfma = from->fwd_mfma;
   to->jacobi_sweep_cap = from->jacobi_sweep_cap;
   to->fail_unitarity = from->fail_unitarity;
 }
@@ -1201,6 +1259,11 @@ int qmx_set_option(qmx_handle* h, int option, int value) {
   if (option == QMX_OPT_STAGED_WINDOWS) {
     if (value < 0 || value > 2) return fail(h, QMX_ERR_INVALID, "staged windows: 0, 1 or 2");
     h->staged_windows = value;
+    return QMX_OK;
+  }
+  if (option == QMX_OPT_FWD_MFMA) {
+    if (value < 0 || value > 1) return fail(h, QMX_ERR_INVALID, "forward passes on the matrix cores: 0 or 1");
+    h->fwd_mfma = value;
     return QMX_OK;
   }
   if (option == QMX_OPT_FWD_BLOCKS_PER_CU) {

@@ -83,6 +83,21 @@ struct GroupMap {
   int ux[4];           // their tile-local positions
 };
 
+// Lane roles of a stage in the matrix-core forward pass (2^12-amplitude
+// tiles: 8 rest bits, 4 gate bits).  Lane jl + 16 kq of wave w handles, for
+// column block cb, the group whose rest bits are jl at jl*, cb and w at h*
+// (cb in h*[0..1], w in h*[2..3]), and amplitudes whose gate bits are kq at
+// k* and the k-step s (inputs) or the item r (outputs) at s*.  *l: tile-local
+// positions, *g: global positions, kx / sx: indices into the stage's
+// ux / ubit.  The host picks them so that the 32 lanes of each LDS access
+// phase fall on distinct banks.  Each word holds four positions, one per byte
+// (32-bit words: the kernel reads them with scalar loads).
+struct MfmaMap {
+  uint32_t jl, hl, ksl;  // jl[0..3]; hl[0..3]; kl[0], kl[1], sl[0], sl[1]
+  uint32_t jg, hg, ksg;  // the same as global positions
+  uint32_t kxsx;         // kx[0], kx[1], sx[0], sx[1]
+};
+
 // Windowed sweep (qmx_window.hip): window = sweep steps [s0, s1) on the
 // 4-bit set umask; the previous window [p0, p1) (p1 == s0, empty for the
 // first) has its updated gates applied to T' in the same pass.
@@ -117,11 +132,13 @@ template <typename S>
 struct FwdArgs {
   int n, lmax, t, nstage;
   int from_basis0;     // the input is |0..0> (psi is not read)
+  int mfma;            // complex64 tiles of 2^kFwdTileBits: stage matrices on the matrix cores
   int full;            // write every amplitude (else only those inside live_out)
   uint64_t tmask;      // tile-index bits visited (state bits outside the tile)
   uint64_t live_in;    // amplitudes with a bit outside live_in are zero before the pass
   uint64_t live_out;   // ... and outside live_out after it
   GroupMap gs[kMaxStages];
+  MfmaMap mm[kMaxStages];  // lane roles of the matrix-core pass (mfma only)
   int gstart[kMaxStages + 1];
   int fpos[kMaxFwdGates];
   S* psi;

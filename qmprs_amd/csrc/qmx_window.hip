@@ -549,6 +549,210 @@ __global__ __launch_bounds__(256) void fwd_pass_kernel(FwdArgs<S> a) {
   }
 }
 
+// The forward pass on the matrix cores (complex64 states, tiles of 2^12
+// amplitudes, N <= 28): each stage's gates are fused into one 16 x 16 complex matrix M
+// (column c = the stage's gates applied in order to the unit vector e_c of the
+// stage's 4-bit group index), and each wave applies M to its 64 groups, 16 at
+// a time, as complex products on v_mfma_f32_16x16x4f32 (four real products
+// per k-step: Re M Re x - Im M Im x and Re M Im x + Im M Re x).  Operand
+// layouts of v_mfma_f32_16x16x4f32 (checked on gfx950): A[i][k] in lane
+// i + 16 k, B[k][j] in lane j + 16 k, D[i][j] in lane j + 16 (i / 4), item
+// i % 4.  Lane jl + 16 kq of wave w therefore holds, for column block cb,
+// amplitudes 4 s + kq (s < 4) of group 64 w + 16 cb + jl as B operands and
+// receives amplitudes 4 kq + r (r < 4) of the same group.  Loads, support
+// (live_in / live_out), the stage-0 prefetch of the next tile and the LDS
+// exchange between stages are those of fwd_pass_kernel.
+typedef float f4v __attribute__((ext_vector_type(4)));
+
+// index parts of the matrix-core pass (positions from MfmaMap words, one
+// per byte): the lane's (jl at j[0..3], kq at ks[0..1]), the wave's (w at
+// h[2..3]), and the uniform (cb at h[0..1], s at ks[2..3]); the three are
+// disjoint bit fields
+QMX_DEV uint32_t pos8(uint32_t w, int i) { return (w >> (8 * i)) & 255u; }
+QMX_DEV uint32_t lanepart(uint32_t j, uint32_t ks, int jl, int kq) {
+  return ((uint32_t)(jl & 1) << pos8(j, 0)) | ((uint32_t)((jl >> 1) & 1) << pos8(j, 1)) |
+         ((uint32_t)((jl >> 2) & 1) << pos8(j, 2)) | ((uint32_t)(jl >> 3) << pos8(j, 3)) |
+         ((uint32_t)(kq & 1) << pos8(ks, 0)) | ((uint32_t)(kq >> 1) << pos8(ks, 1));
+}
+QMX_DEV uint32_t wavepart(uint32_t h, int w) {
+  return ((uint32_t)(w & 1) << pos8(h, 2)) | ((uint32_t)(w >> 1) << pos8(h, 3));
+}
+QMX_DEV uint32_t unipart(uint32_t h, uint32_t ks, int cb, int s) {
+  return ((uint32_t)(cb & 1) << pos8(h, 0)) | ((uint32_t)(cb >> 1) << pos8(h, 1)) |
+         ((uint32_t)(s & 1) << pos8(ks, 2)) | ((uint32_t)(s >> 1) << pos8(ks, 3));
+}
+
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3, 3))) void fwd_mfma_kernel(FwdArgs<float2> a) {
+  extern __shared__ char wsm[];
+  __shared__ int gcode[kMaxFwdGates];
+  __shared__ uint32_t mmw[kMaxStages][3];  // per stage: jl, hl, ksl of MfmaMap
+  const int b = a.list ? a.list[blockIdx.y] : blockIdx.y;
+  const int n = a.n;
+  if (a.skip && a.skip[b]) return;
+  const int lb = a.layers[b];
+  const int ng = a.gstart[a.nstage];
+  const int t = a.t;   // kFwdTileBits: 256 groups of 16, one wave per 64
+  float2* lds = reinterpret_cast<float2*>(wsm);
+  // the stage matrices after the tile, in A-operand order: with P(q, s) the
+  // amplitude index whose bits kx are q and bits sx are s (lane roles),
+  // mst[st][s][jl + 16 kq] = M[P(jl / 4, jl % 4)][P(kq, s)], so that a
+  // wave's A load is contiguous
+  float2* mst = lds + ((size_t)1 << t);
+  // gates (as G^dagger, which grp_apply applies as G) in the tile area, which
+  // stage 0 only writes after the matrices are built
+  float2(*gmat)[16] = reinterpret_cast<float2(*)[16]>(wsm);
+  for (int e2 = threadIdx.x; e2 < ng * 16; e2 += blockDim.x) {
+    const int e = e2 & 15, j = e2 >> 4;
+    const int f = a.fpos[j];
+    if (f < lb * n) {
+      int st = 0;
+      while (a.gstart[st + 1] <= j) ++st;
+      const int layer = lb - 1 - f / n, site = f % n;
+      const size_t gi = ((size_t)b * a.lmax + layer) * n + site;
+      const double2 gv = a.gates[gi * 16 + (e & 3) * 4 + (e >> 2)];
+      gmat[j][e] = make_float2((float)gv.x, (float)-gv.y);
+      if (e == 0) gcode[j] = gate_code(a.gs[st].mask, n, site, wsane(n, site, a.widths[gi]));
+    } else if (e == 0) {
+      gcode[j] = -1;
+    }
+  }
+  __syncthreads();
+  int total = 0;
+  for (int j = 0; j < ng; ++j) total += a.fpos[j] < lb * n ? 1 : 0;
+  const uint64_t all = (1ull << n) - 1ull;
+  if (total == 0 && !a.from_basis0 && a.live_in == all) return;
+  if ((int)threadIdx.x < a.nstage * 16) {
+    const int st = threadIdx.x >> 4, c = threadIdx.x & 15;
+    const MfmaMap& mm = a.mm[st];
+    float2 v[16];
+#pragma unroll
+    for (int x = 0; x < 16; ++x) v[x] = make_float2(x == c ? 1.f : 0.f, 0.f);
+    for (int j = a.gstart[st]; j < a.gstart[st + 1]; ++j) grp_apply<float>(gcode[j], v, gmat[j]);
+    // amplitude index x <-> (q, s): q = its bits kx, s = its bits sx
+    const uint32_t kx0 = pos8(mm.kxsx, 0), kx1 = pos8(mm.kxsx, 1), sx0 = pos8(mm.kxsx, 2), sx1 = pos8(mm.kxsx, 3);
+    auto qof = [&](int x) { return (int)(((x >> kx0) & 1) | (((x >> kx1) & 1) << 1)); };
+    auto sof = [&](int x) { return (int)(((x >> sx0) & 1) | (((x >> sx1) & 1) << 1)); };
+    const int kc = qof(c), sc = sof(c);
+#pragma unroll
+    for (int x = 0; x < 16; ++x) mst[(st * 4 + sc) * 64 + 16 * kc + 4 * qof(x) + sof(x)] = v[x];
+    if (c == 0) {
+      mmw[st][0] = mm.jl;
+      mmw[st][1] = mm.hl;
+      mmw[st][2] = mm.ksl;
+    }
+  }
+  __syncthreads();
+  const size_t len = (size_t)1 << n;
+  // 32-bit byte offsets from a scalar base (N <= 28: offsets below 2^31)
+  char* pbytes = reinterpret_cast<char*>(a.psi + (size_t)b * len);
+  const long ntiles = 1l << __popcll(a.tmask);
+  const uint32_t dead_in = (uint32_t)(~a.live_in & all), dead_out = (uint32_t)(~a.live_out & all);
+  const int lane = threadIdx.x & 63;
+  const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+  const int jl = lane & 15, kq = lane >> 4;
+  // Every index below is an OR of disjoint bit fields, split into a per-lane
+  // part (jl: the 4 low bits of the group index, kq: 2 bits of the amplitude
+  // index) and a part that is uniform over the wave (wave, cb, s4 / r).  The
+  // deposits and xoffs are linear over disjoint fields and the LDS swizzle is
+  // linear over XOR, so an access costs one OR or XOR of the two parts.
+  const MfmaMap& m0 = a.mm[0];
+  const MfmaMap& mL = a.mm[a.nstage - 1];
+  const uint32_t h0g = __builtin_amdgcn_readfirstlane(m0.hg), k0g = __builtin_amdgcn_readfirstlane(m0.ksg);
+  const uint32_t hLg = __builtin_amdgcn_readfirstlane(mL.hg), kLg = __builtin_amdgcn_readfirstlane(mL.ksg);
+  const uint32_t g0lane = lanepart(m0.jg, k0g, jl, kq);
+  const uint32_t gLlane = lanepart(mL.jg, kLg, jl, kq);
+  const uint32_t g0w = wavepart(h0g, wave), gLw = wavepart(hLg, wave);
+  // B operands of stage 0 for one tile: [cb * 4 + s] = amplitude P(kq, s)
+  float2 pre[16];
+  auto load0 = [&](uint32_t bse) {
+#pragma unroll
+    for (int cb = 0; cb < 4; ++cb)
+#pragma unroll
+      for (int s4 = 0; s4 < 4; ++s4) {
+        const uint32_t ad = bse + (g0w | unipart(h0g, k0g, cb, s4)) + g0lane;
+        float2 v;
+        if (ad & dead_in)
+          v = make_float2(0.f, 0.f);
+        else if (a.from_basis0)
+          v = make_float2(ad == 0u ? 1.f : 0.f, 0.f);
+        else
+          v = *reinterpret_cast<const float2*>(pbytes + (ad << 3));
+        pre[cb * 4 + s4] = v;
+      }
+  };
+  if ((long)blockIdx.x < ntiles) load0((uint32_t)wdeposit((uint64_t)blockIdx.x, a.tmask));
+  for (long tile = blockIdx.x; tile < ntiles; tile += gridDim.x) {
+    const uint32_t base = (uint32_t)wdeposit((uint64_t)tile, a.tmask);
+    for (int st = 0; st < a.nstage; ++st) {
+      // lane roles of the stage (uniform: LDS broadcast, then scalar)
+      const uint32_t mjl = __builtin_amdgcn_readfirstlane(mmw[st][0]);
+      const uint32_t mhl = __builtin_amdgcn_readfirstlane(mmw[st][1]);
+      const uint32_t mksl = __builtin_amdgcn_readfirstlane(mmw[st][2]);
+      const bool last = (st + 1 == a.nstage);
+      // tile-local offsets: per lane (jl, kq, swizzled once per stage) and
+      // per (wave, cb, s or r)
+      const int lsw = swz((int)lanepart(mjl, mksl, jl, kq));
+      const uint32_t lw = wavepart(mhl, wave);
+      float2 xv[16];
+      if (st == 0) {
+#pragma unroll
+        for (int q = 0; q < 16; ++q) xv[q] = pre[q];
+        const long nxt = tile + gridDim.x;
+        if (nxt < ntiles) load0((uint32_t)wdeposit((uint64_t)nxt, a.tmask));
+      } else {
+#pragma unroll
+        for (int cb = 0; cb < 4; ++cb)
+#pragma unroll
+          for (int s4 = 0; s4 < 4; ++s4) xv[cb * 4 + s4] = lds[lsw ^ swz((int)(lw | unipart(mhl, mksl, cb, s4)))];
+      }
+      // A operands: M[jl][4 s + kq]
+      float2 am[4];
+#pragma unroll
+      for (int s4 = 0; s4 < 4; ++s4) am[s4] = mst[(st * 4 + s4) * 64 + lane];
+      // two column blocks at a time: k-steps outside, the two blocks inside,
+      // so that a product's accumulator was last written 4 products earlier
+#pragma unroll
+      for (int cp = 0; cp < 4; cp += 2) {
+        f4v yr[2] = {f4v{0.f, 0.f, 0.f, 0.f}, f4v{0.f, 0.f, 0.f, 0.f}};
+        f4v yi[2] = {f4v{0.f, 0.f, 0.f, 0.f}, f4v{0.f, 0.f, 0.f, 0.f}};
+#pragma unroll
+        for (int s4 = 0; s4 < 4; ++s4) {
+#pragma unroll
+          for (int c = 0; c < 2; ++c) {
+            const float2 x = xv[(cp + c) * 4 + s4];
+            yr[c] = __builtin_amdgcn_mfma_f32_16x16x4f32(am[s4].x, x.x, yr[c], 0, 0, 0);
+            yi[c] = __builtin_amdgcn_mfma_f32_16x16x4f32(am[s4].x, x.y, yi[c], 0, 0, 0);
+          }
+#pragma unroll
+          for (int c = 0; c < 2; ++c) {
+            const float2 x = xv[(cp + c) * 4 + s4];
+            yr[c] = __builtin_amdgcn_mfma_f32_16x16x4f32(-am[s4].y, x.y, yr[c], 0, 0, 0);
+            yi[c] = __builtin_amdgcn_mfma_f32_16x16x4f32(am[s4].y, x.x, yi[c], 0, 0, 0);
+          }
+        }
+#pragma unroll
+        for (int c = 0; c < 2; ++c) {
+          const int cb = cp + c;
+          if (last) {
+#pragma unroll
+            for (int r = 0; r < 4; ++r) {
+              const uint32_t ad = base + (gLw | unipart(hLg, kLg, cb, r)) + gLlane;
+              if (a.full || !(ad & dead_out))
+                *reinterpret_cast<float2*>(pbytes + (ad << 3)) = make_float2(yr[c][r], yi[c][r]);
+            }
+          } else {
+#pragma unroll
+            for (int r = 0; r < 4; ++r)
+              lds[lsw ^ swz((int)(lw | unipart(mhl, mksl, cb, r)))] = make_float2(yr[c][r], yi[c][r]);
+          }
+        }
+      }
+      // stage exchange, and the next tile's stage 0 must not overwrite LDS early
+      if (a.nstage > 1) __syncthreads();
+    }
+  }
+}
+
 // Replay the window on the 16-dim space of U (one block per state).
 // It runs beside the other half-batch's window pass, whose waves fill the
 // SIMDs; its own waves take the highest issue priority so that this short,
@@ -687,6 +891,14 @@ __global__ __launch_bounds__(256) void window_finalize_kernel(WinFinArgs a) {
 template <typename S>
 void launch_fwd_pass(const FwdArgs<S>& a, int nblk, int batch, hipStream_t st) {
   size_t shm = sizeof(S) << a.t;
+  if constexpr (sizeof(S) == 8) {
+    if (a.mfma && a.t == kFwdTileBits && a.n <= 28) {
+      // the tile, then one 16 x 16 complex64 matrix per stage
+      shm += (size_t)a.nstage * 256 * sizeof(float2);
+      hipLaunchKernelGGL(fwd_mfma_kernel, dim3(nblk, batch), dim3(256), shm, st, a);
+      return;
+    }
+  }
   hipLaunchKernelGGL(fwd_pass_kernel<S>, dim3(nblk, batch), dim3(256), shm, st, a);
 }
 template void launch_fwd_pass<float2>(const FwdArgs<float2>&, int, int, hipStream_t);
