@@ -157,6 +157,37 @@ def traffic_probe(args) -> int:
     return 0
 
 
+def rebuild_probe(args) -> int:
+    """Profiling body for the circuit rebuild: the bench batch's gates (chi
+    truncation and layer extraction of the seeded states, no sweeps), then
+    qmx_circuit_state in complex64 three times with the forward passes on the
+    matrix cores (QMX_OPT_FWD_MFMA=1) and three times on the vector units
+    (QMX_OPT_FWD_MFMA=0).  Prints the mean wall time per call of each."""
+    import time
+    import torch
+    from qmprs_amd import _lib, engine
+    wl = WORKLOAD
+    _lib.require_gpu()
+    svs = _states(wl["num_qubits"], wl["generator_bond"], args.batch, seed0=1000)
+    r = engine.encode(engine.as_device_states(svs), wl["bond_dimension"], wl["num_layers"], 0,
+                      sweep_dtype=torch.complex64)
+    torch.cuda.synchronize()
+    try:
+        for mode in (1, 0):
+            _lib.set_option(_lib.QMX_OPT_FWD_MFMA, mode)
+            engine.circuit_state(r.gates, r.widths, r.layers_used, dtype=torch.complex64)
+            torch.cuda.synchronize()
+            t0 = time.perf_counter()
+            for _ in range(3):
+                engine.circuit_state(r.gates, r.widths, r.layers_used, dtype=torch.complex64)
+            torch.cuda.synchronize()
+            print(f"QMX_OPT_FWD_MFMA={mode}: {(time.perf_counter() - t0) / 3 * 1e3:.2f} ms per "
+                  f"circuit_state of {args.batch} states", flush=True)
+    finally:
+        _lib.set_option(_lib.QMX_OPT_FWD_MFMA, _lib.FWD_MFMA_DEFAULT)
+    return 0
+
+
 def _counter_means(csv_dir: str) -> dict:
     """{kernel-name key: (mean counter value, dispatches)} from rocprofv3's csv."""
     import csv
@@ -472,6 +503,9 @@ def main() -> int:
     ap.add_argument("--no-traffic", action="store_true",
                     help="skip the two rocprofv3 counter runs that measure roofline.traffic")
     ap.add_argument("--traffic-probe", action="store_true", help=argparse.SUPPRESS)
+    ap.add_argument("--rebuild-probe", action="store_true",
+                    help="profiling body: complex64 circuit rebuilds of the bench batch, matrix-core and "
+                         "vector-unit forward passes (no JSON line)")
     ap.add_argument("--no-warm-start", action="store_true",
                     help="skip the learned warm-start leg (training on N=16 plain-method "
                          "pairs, then the N=20 warm-started encode against the FP64 goal)")
@@ -495,6 +529,8 @@ def main() -> int:
     from qmprs_amd import _lib, engine
     if args.traffic_probe:
         return traffic_probe(args)
+    if args.rebuild_probe:
+        return rebuild_probe(args)
     if args.workload == "ragged":
         return ragged_main(args)
     if args.workload == "large":
