@@ -709,42 +709,35 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3, 3))) voi
       float2 am[4];
 #pragma unroll
       for (int s4 = 0; s4 < 4; ++s4) am[s4] = mst[(st * 4 + s4) * 64 + lane];
-      // two column blocks at a time: k-steps outside, the two blocks inside,
-      // so that a product's accumulator was last written 4 products earlier
+      // one column block at a time, each complex product as three real ones
+      // (Gauss): k1 = Re M Re x, k2 = Im M Im x, k3 = (Re M + Im M)(Re x +
+      // Im x); Re y = k1 - k2, Im y = k3 - k1 - k2.  The three accumulators
+      // alternate, so consecutive products are independent.
+      float asum[4];
 #pragma unroll
-      for (int cp = 0; cp < 4; cp += 2) {
-        f4v yr[2] = {f4v{0.f, 0.f, 0.f, 0.f}, f4v{0.f, 0.f, 0.f, 0.f}};
-        f4v yi[2] = {f4v{0.f, 0.f, 0.f, 0.f}, f4v{0.f, 0.f, 0.f, 0.f}};
+      for (int s4 = 0; s4 < 4; ++s4) asum[s4] = am[s4].x + am[s4].y;
+#pragma unroll
+      for (int cb = 0; cb < 4; ++cb) {
+        f4v k1 = {0.f, 0.f, 0.f, 0.f}, k2 = {0.f, 0.f, 0.f, 0.f}, k3 = {0.f, 0.f, 0.f, 0.f};
 #pragma unroll
         for (int s4 = 0; s4 < 4; ++s4) {
-#pragma unroll
-          for (int c = 0; c < 2; ++c) {
-            const float2 x = xv[(cp + c) * 4 + s4];
-            yr[c] = __builtin_amdgcn_mfma_f32_16x16x4f32(am[s4].x, x.x, yr[c], 0, 0, 0);
-            yi[c] = __builtin_amdgcn_mfma_f32_16x16x4f32(am[s4].x, x.y, yi[c], 0, 0, 0);
-          }
-#pragma unroll
-          for (int c = 0; c < 2; ++c) {
-            const float2 x = xv[(cp + c) * 4 + s4];
-            yr[c] = __builtin_amdgcn_mfma_f32_16x16x4f32(-am[s4].y, x.y, yr[c], 0, 0, 0);
-            yi[c] = __builtin_amdgcn_mfma_f32_16x16x4f32(am[s4].y, x.x, yi[c], 0, 0, 0);
-          }
+          const float2 x = xv[cb * 4 + s4];
+          k1 = __builtin_amdgcn_mfma_f32_16x16x4f32(am[s4].x, x.x, k1, 0, 0, 0);
+          k2 = __builtin_amdgcn_mfma_f32_16x16x4f32(am[s4].y, x.y, k2, 0, 0, 0);
+          k3 = __builtin_amdgcn_mfma_f32_16x16x4f32(asum[s4], x.x + x.y, k3, 0, 0, 0);
         }
+        const f4v yr = k1 - k2, yi = k3 - k1 - k2;
+        if (last) {
 #pragma unroll
-        for (int c = 0; c < 2; ++c) {
-          const int cb = cp + c;
-          if (last) {
-#pragma unroll
-            for (int r = 0; r < 4; ++r) {
-              const uint32_t ad = base + (gLw | unipart(hLg, kLg, cb, r)) + gLlane;
-              if (a.full || !(ad & dead_out))
-                *reinterpret_cast<float2*>(pbytes + (ad << 3)) = make_float2(yr[c][r], yi[c][r]);
-            }
-          } else {
-#pragma unroll
-            for (int r = 0; r < 4; ++r)
-              lds[lsw ^ swz((int)(lw | unipart(mhl, mksl, cb, r)))] = make_float2(yr[c][r], yi[c][r]);
+          for (int r = 0; r < 4; ++r) {
+            const uint32_t ad = base + (gLw | unipart(hLg, kLg, cb, r)) + gLlane;
+            if (a.full || !(ad & dead_out))
+              *reinterpret_cast<float2*>(pbytes + (ad << 3)) = make_float2(yr[r], yi[r]);
           }
+        } else {
+#pragma unroll
+          for (int r = 0; r < 4; ++r)
+            lds[lsw ^ swz((int)(lw | unipart(mhl, mksl, cb, r)))] = make_float2(yr[r], yi[r]);
         }
       }
       // stage exchange, and the next tile's stage 0 must not overwrite LDS early

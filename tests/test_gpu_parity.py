@@ -553,13 +553,16 @@ def test_circuit_state_large_n(gpu, n, layers, seed):
         assert err < 1e-12
 
 
-@pytest.mark.parametrize("n,layers,seed", [(12, 3, 0), (13, 4, 1), (16, 3, 2), (20, 2, 3)])
+@pytest.mark.parametrize("n,layers,seed", [(12, 3, 0), (12, 10, 4), (13, 4, 1), (16, 3, 2), (20, 2, 3)])
 def test_circuit_state_mfma_matches_oracle(gpu, n, layers, seed):
     """The complex64 forward passes on the matrix cores (each stage's gates
     fused into a 16 x 16 matrix, FP32 MFMA products) against the oracle's
     dense forward application and against the vector-unit passes that apply
     the gates one by one: FP32 rounding apart, the same state (max abs error
-    below 2e-6 on states of norm 1) and the same overlap with a reference."""
+    below 2e-6 on states of norm 1) and the same overlap with a reference.
+    The plans include one-stage passes (N=13, 4 layers: HBM to HBM, no LDS
+    exchange) and twelve-stage passes (N=12, 10 layers: the most stage
+    matrices a block holds)."""
     from qmprs_amd import _lib
     eng = _engine()
     rng = np.random.default_rng(100 + seed)
