@@ -410,6 +410,38 @@ def test_split_sweeps_equal_single_stream(gpu, batch):
     assert (one.fidelity - two.fidelity).abs().max().item() < 1e-12
 
 
+def test_encode_mfma_rebuild_matches_vector_rebuild(gpu):
+    """A complex64 encode (N=14, 3 sweeps) with the circuit rebuilds on the
+    matrix cores and with the vector-unit rebuilds: the same layers, and
+    fidelities within the FP32 rounding that separates the two (the sweeps
+    start from rebuilt states that differ by ~1e-7); both encodes' final
+    gates re-evaluated in FP64 give the same fidelity.  Gate entries are not
+    compared: where a gate's environment is rank-deficient its polar factor
+    is not unique, and rounding picks a different completion of the same
+    circuit fidelity (entries differ by O(1) there)."""
+    from qmprs_amd import _lib
+    eng = _engine()
+    n, chi, L, S = 14, 16, 3, 3
+    svs = np.stack([O.random_mps_state(n, 24, 500 + s) for s in range(3)])
+    try:
+        _lib.set_option(_lib.QMX_OPT_FWD_MFMA, 0)
+        vec = eng.encode(svs, chi, L, S, sweep_dtype=torch.complex64)
+        _lib.set_option(_lib.QMX_OPT_FWD_MFMA, 1)
+        mat = eng.encode(svs, chi, L, S, sweep_dtype=torch.complex64)
+    finally:
+        _lib.set_option(_lib.QMX_OPT_FWD_MFMA, _lib.FWD_MFMA_DEFAULT)
+    torch.cuda.synchronize()
+    assert torch.equal(vec.layers_used, mat.layers_used)
+    dg = (vec.gates - mat.gates).abs().max().item()
+    df = (vec.fidelity.abs() - mat.fidelity.abs()).abs().max().item()
+    psi = torch.from_numpy(svs).to(gpu)
+    _, fv = eng.circuit_state(vec.gates, vec.widths, vec.layers_used, psi_ref=psi)
+    _, fm = eng.circuit_state(mat.gates, mat.widths, mat.layers_used, psi_ref=psi)
+    d64 = (fv.abs() - fm.abs()).abs().max().item()
+    print(f"max |dgate| {dg:.2e}, max |dF| {df:.2e}, FP64 re-evaluation |dF| {d64:.2e}")
+    assert df < 1e-5 and d64 < 1e-5
+
+
 @pytest.mark.parametrize("omega", [1.0, 1.7])
 def test_encode_ragged_equals_bucket_encodes(gpu, omega):
     """qmx_encode_ragged: states of different (N, chi), shuffled, give what
