@@ -310,6 +310,83 @@ QMX_DEV bool polar4_newton(double2* u) {
   return false;
 }
 
+// ---- the same Newton iteration on 16 lanes -----------------------------------
+// Lanes 0..15 of one wave (all active, the rest of the wave idle) hold one
+// entry each: lane l = 4 i + j holds X[i][j].  Cross-lane reads are __shfl
+// within the 16 lanes; sums are xor butterflies, in which both partners add
+// the same two values at every level, so every lane holds the bitwise same
+// sum and every branch below is uniform over the 16 lanes.
+QMX_DEV double2 shfl16(double2 v, int src) {
+  return make_double2(__shfl(v.x, src, 16), __shfl(v.y, src, 16));
+}
+QMX_DEV double sum16(double v) {
+#pragma unroll
+  for (int m = 1; m < 16; m <<= 1) v += __shfl_xor(v, m, 16);
+  return v;
+}
+
+// In-place inverse by Gauss-Jordan elimination with row pivoting (largest
+// |entry| in the pivot column, first on ties), the operations of
+// inv4_inplace entry by entry.  Returns false (on every lane) for a zero
+// pivot or a non-finite result.
+QMX_DEV bool inv4_lanes(double2& m, int l) {
+  const int i = l >> 2, j = l & 3;
+  int perm[4];
+#pragma unroll
+  for (int k = 0; k < 4; ++k) {
+    const double c = cabs2(m);
+    int p = k;
+    double best = __shfl(c, k * 4 + k, 16);
+#pragma unroll
+    for (int r = k + 1; r < 4; ++r) {
+      const double v = __shfl(c, r * 4 + k, 16);
+      if (v > best) {
+        best = v;
+        p = r;
+      }
+    }
+    if (!(best > 0.0)) return false;
+    perm[k] = p;
+    if (p != k) m = shfl16(m, i == k ? p * 4 + j : (i == p ? k * 4 + j : l));
+    const double2 piv = shfl16(m, k * 4 + k);
+    const double ib = 1.0 / best;
+    const double2 ip = make_double2(piv.x * ib, -piv.y * ib);   // 1 / pivot
+    const double2 rk = shfl16(m, k * 4 + j);                      // row k, column j
+    const double2 f = shfl16(m, i * 4 + k);                       // row i, column k
+    const double2 rkj = cmul(j == k ? make_double2(1.0, 0.0) : rk, ip);
+    m = (i == k) ? rkj : csub(j == k ? cz() : m, cmul(f, rkj));
+  }
+#pragma unroll
+  for (int k = 3; k >= 0; --k) {
+    const int p = perm[k];
+    if (p != k) m = shfl16(m, j == k ? i * 4 + p : (j == p ? i * 4 + k : l));
+  }
+  return sum16(cabs2(m)) < 1e300;   // false for inf / nan
+}
+
+// polar4_newton on 16 lanes (lane l holds X[l / 4][l % 4] on entry and the
+// unitary factor on a true return).
+QMX_DEV bool polar4_newton_lanes(double2& x, int l) {
+  const int i = l >> 2, j = l & 3;
+  bool scale = true;
+  for (int it = 0; it < 40; ++it) {
+    double2 xi = x;
+    if (!inv4_lanes(xi, l)) return false;
+    const double nx = sum16(cabs2(x)), ni = sum16(cabs2(xi));
+    if (it == 0 && nx * ni > 1e12) return false;
+    const double z = scale ? sqrt(sqrt(ni / nx)) : 1.0;
+    const double zi = 1.0 / z;
+    // X_{k+1}[i][j] = (z X[i][j] + conj(X^{-1}[j][i]) / z) / 2
+    const double2 b = shfl16(xi, j * 4 + i);
+    const double2 v = make_double2(0.5 * (z * x.x + zi * b.x), 0.5 * (z * x.y - zi * b.y));
+    const double dl = sum16(cabs2(csub(v, x)));
+    x = v;
+    if (dl < 1e-4) scale = false;
+    if (dl <= 1e-16) return true;
+  }
+  return false;
+}
+
 // Unitary polar factor U V^H of the d x d (d = 2 or 4) matrix e (row-major,
 // stride 4).  Well-conditioned inputs (see polar4_newton) take the Newton
 // iteration; the others (and any input it rejects) a one-sided (Hestenes) Jacobi SVD on a 4 x 4 working matrix (a

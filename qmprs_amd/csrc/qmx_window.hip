@@ -760,6 +760,8 @@ __global__ __launch_bounds__(256) void window_finalize_kernel(WinFinArgs a) {
   __shared__ double2 es[16];
   __shared__ double2 rm[16];
   __shared__ double part_sum[512];
+  __shared__ double s_nuc;
+  __shared__ int s_ok;
   const int b = a.list ? a.list[blockIdx.x] : blockIdx.x;
   const int n = a.n;
   if (a.skip && a.skip[b]) return;
@@ -827,20 +829,50 @@ __global__ __launch_bounds__(256) void window_finalize_kernel(WinFinArgs a) {
       es[threadIdx.x] = acc;
     }
     __syncthreads();
-    if (threadIdx.x == 0) {
-      // E stays in LDS (es): the polar solves read it from there
-      double2 u[16];
-      const int d = w == 2 ? 4 : 2;
-      double nuc = 0.0;
-      polar4(d, es, u, &nuc);
-#pragma unroll
-      for (int k = 0; k < 16; ++k) u[k] = cconj(u[k]);
-      if (a.omega_m1 != 0.0) nuc = relax_update(d, a.omega_m1, gm[j], u, es, rm);
-#pragma unroll
-      for (int k = 0; k < 16; ++k) gnew[k] = u[k];
-      if (a.s0 + j == kb - 1 && a.overlap) a.overlap[(size_t)b * a.overlap_stride] = nuc;
+    const int d = w == 2 ? 4 : 2;
+    const bool last_gate = a.s0 + j == kb - 1 && a.overlap;
+    if (threadIdx.x < 16) {
+      // polar(E) by the scaled Newton iteration on 16 lanes (one entry
+      // each); the iterate is E, or diag(E, |E|_F I) for a 2 x 2 problem
+      const int l = threadIdx.x, r = l >> 2, c = l & 3;
+      const bool in = r < d && c < d;
+      double2 xv = in ? es[l] : cz();
+      const double fro = sum16(cabs2(xv));
+      if (d == 2 && (l == 10 || l == 15)) xv = make_double2(fro > 0.0 ? sqrt(fro) : 1.0, 0.0);
+      const bool ok = fro > 0.0 && polar4_newton_lanes(xv, l);
+      if (ok) {
+        const double2 uv = in ? xv : cz();
+        // Re tr(U^H E) = sum of the singular values
+        const double tr = sum16(in ? uv.x * es[l].x + uv.y * es[l].y : 0.0);
+        gnew[l] = cconj(uv);
+        if (l == 0) {
+          s_nuc = tr;
+          if (last_gate && a.omega_m1 == 0.0) a.overlap[(size_t)b * a.overlap_stride] = tr;
+        }
+      }
+      if (l == 0) s_ok = ok ? 1 : 0;
     }
     __syncthreads();
+    if (!s_ok || a.omega_m1 != 0.0) {
+      // ill-conditioned E (Jacobi SVD, one lane), and the over-relaxed step
+      if (threadIdx.x == 0) {
+        double2 u[16];
+        double nuc = s_nuc;
+        if (!s_ok) {
+          polar4_jacobi(d, es, u, &nuc);
+#pragma unroll
+          for (int k = 0; k < 16; ++k) u[k] = cconj(u[k]);
+        } else {
+#pragma unroll
+          for (int k = 0; k < 16; ++k) u[k] = gnew[k];
+        }
+        if (a.omega_m1 != 0.0) nuc = relax_update(d, a.omega_m1, gm[j], u, es, rm);
+#pragma unroll
+        for (int k = 0; k < 16; ++k) gnew[k] = u[k];
+        if (last_gate) a.overlap[(size_t)b * a.overlap_stride] = nuc;
+      }
+      __syncthreads();
+    }
     if (threadIdx.x < 16) {
       int layer, site;
       step_gate(n, a.s0 + j, layer, site);
