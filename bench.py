@@ -89,9 +89,13 @@ def _host_cores() -> tuple[int, int]:
     return aff, share
 
 
-def cpu_baseline(svs: np.ndarray, wl: dict, gpu_fid_trace=None, seeds: int = 3) -> dict:
+def cpu_baseline(svs: np.ndarray, wl: dict, gpu_fid_trace=None, seeds: int = 3, gpu_chi=None) -> dict:
     """FP64 oracle timing on ``seeds`` states of the workload (bounded sample,
-    each scaled to a full prep), median state-preps/s."""
+    each scaled to a full prep), median state-preps/s.  On the first state
+    the oracle's results also check the GPU's: fidelity after 0 and 1 FP64
+    sweeps (``gpu_fid_trace``) and the chi-truncation (``gpu_chi`` = (target,
+    spectra) of qmx_chi_truncate): the truncated state's relative error and
+    the retained squared singular values per bond."""
     sys.path.insert(0, os.path.dirname(os.path.abspath(__file__)))
     from threadpoolctl import threadpool_limits
     from oracle import qmprs_oracle as O
@@ -103,7 +107,7 @@ def cpu_baseline(svs: np.ndarray, wl: dict, gpu_fid_trace=None, seeds: int = 3) 
         for k in range(seeds):
             t0 = time.perf_counter()
             psi = O.ket(svs[k].astype(np.complex128))
-            target, _ = O.chi_truncate(psi, n, chi)
+            target, spectra = O.chi_truncate(psi, n, chi)
             t1 = time.perf_counter()
             layers = O.get_unitary_layers(target, n, L)
             t2 = time.perf_counter()
@@ -120,6 +124,20 @@ def cpu_baseline(svs: np.ndarray, wl: dict, gpu_fid_trace=None, seeds: int = 3) 
                 parity = {"oracle_fid_sweep0": f0, "oracle_fid_sweep1": f1,
                           "gpu_fp64_fid_sweep0": float(gpu_fid_trace[0]),
                           "gpu_fp64_fid_sweep1": float(gpu_fid_trace[1])}
+            if k == 0 and gpu_chi is not None:
+                g_target, g_spec = gpu_chi
+                parity = parity or {}
+                parity["chi_target_rel_err"] = float(np.linalg.norm(g_target - target) / np.linalg.norm(target))
+                # retained squared singular values (eigenvalues of the bond Gram
+                # matrices), largest error relative to each bond's largest
+                errs = []
+                for i, lam in enumerate(spectra):
+                    if lam is None:
+                        continue
+                    kk = min(len(lam), chi)
+                    errs.append(float(np.max(np.abs(g_spec[i, :kk] - lam[:kk])) / lam[0]))
+                parity["chi_spectra_max_rel_err"] = max(errs) if errs else 0.0
+                parity["chi_bonds_checked"] = len(errs)
     med = float(np.median(rates))
     c, l, w = (float(np.median([p[i] for p in parts])) for i in range(3))
     out = {"value": med, "unit": "state-preps/s", "cores": cores, "kind": "port",
@@ -668,8 +686,12 @@ def main() -> int:
     if args.relax_omega != 1.0:
         out["relaxed"] = relaxed_leg(sv_dev, psi128, goal, wl, args)
     if not args.no_cpu_baseline:
+        # the GPU chi-truncation (FP64) of the first state, checked against the
+        # oracle's inside the baseline leg
+        _, tgt0, spec0 = engine.chi_truncate(sv_dev[:1].contiguous(), wl["bond_dimension"], spectra=True)
         out["cpu_baseline"] = cpu_baseline(svs, wl, ref.fidelity[0].abs().cpu().numpy()
-                                           if ref.fidelity.shape[1] > 1 else None)
+                                           if ref.fidelity.shape[1] > 1 else None,
+                                           gpu_chi=(tgt0[0].cpu().numpy(), spec0[0].cpu().numpy()))
     print(json.dumps(out))
     return 0
 
