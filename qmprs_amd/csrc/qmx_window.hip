@@ -333,6 +333,17 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(WPE))) void
   const int rlane = xl_i + group_roffl(a.gu, wave * kper + (lane >> 4));
   d4v acc_r = {0, 0, 0, 0}, acc_i = {0, 0, 0, 0};
   const uint32_t lomask = (1u << c0) - 1u;
+  // Direct loads: a group is dead once stored, so the next tile's group is
+  // loaded into the same registers right then and its loads stay in flight
+  // through the barriers and the R phase.  The next tile is this block's
+  // alone (tiles go round-robin over the blocks), so no other block writes it.
+  c2 v[16];
+  auto load_group = [&](uint32_t g) {
+#pragma unroll
+    for (int x = 0; x < 16; ++x) v[x] = T::mk(to_c(src[g + xg[x]]));
+  };
+  if (!STAGED && active && (long)blockIdx.x < ntiles)
+    load_group((uint32_t)wdeposit((uint64_t)blockIdx.x, a.qmask) + roffg);
   for (long tile = blockIdx.x; tile < ntiles; tile += gridDim.x) {
     const uint32_t base = (uint32_t)wdeposit((uint64_t)tile, a.qmask);
     const uint32_t gb = base + roffg;
@@ -367,11 +378,11 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(WPE))) void
       }
       __syncthreads();
     }
-    c2 v[16];
     if (active) {
+      if (STAGED) {
 #pragma unroll
-      for (int x = 0; x < 16; ++x)
-        v[x] = STAGED ? T::mk(to_c(lds[swz(roffl + xl[x])])) : T::mk(to_c(src[gb + xg[x]]));
+        for (int x = 0; x < 16; ++x) v[x] = T::mk(to_c(lds[swz(roffl + xl[x])]));
+      }
       for (int j = 0; j <= ngate; ++j) {
         if (j == lds_after) {
 #pragma unroll
@@ -383,6 +394,10 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(WPE))) void
       if (!STAGED && store_dst) {
 #pragma unroll
         for (int x = 0; x < 16; ++x) dst[gb + xg[x]] = from_c<S>(make_double2(v[x].x, v[x].y));
+      }
+      if (!STAGED) {
+        const long nxt = tile + gridDim.x;
+        if (nxt < ntiles) load_group((uint32_t)wdeposit((uint64_t)nxt, a.qmask) + roffg);
       }
     }
     __syncthreads();
