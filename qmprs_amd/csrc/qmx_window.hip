@@ -565,18 +565,19 @@ __global__ __launch_bounds__(256) void fwd_pass_kernel(FwdArgs<S> a) {
 }
 
 // The forward pass on the matrix cores (complex64 states, tiles of 2^12
-// amplitudes, N <= 28): each stage's gates are fused into one 16 x 16 complex matrix M
-// (column c = the stage's gates applied in order to the unit vector e_c of the
-// stage's 4-bit group index), and each wave applies M to its 64 groups, 16 at
-// a time, as complex products on v_mfma_f32_16x16x4f32 (four real products
-// per k-step: Re M Re x - Im M Im x and Re M Im x + Im M Re x).  Operand
-// layouts of v_mfma_f32_16x16x4f32 (checked on gfx950): A[i][k] in lane
-// i + 16 k, B[k][j] in lane j + 16 k, D[i][j] in lane j + 16 (i / 4), item
-// i % 4.  Lane jl + 16 kq of wave w therefore holds, for column block cb,
-// amplitudes 4 s + kq (s < 4) of group 64 w + 16 cb + jl as B operands and
-// receives amplitudes 4 kq + r (r < 4) of the same group.  Loads, support
-// (live_in / live_out), the stage-0 prefetch of the next tile and the LDS
-// exchange between stages are those of fwd_pass_kernel.
+// amplitudes, N <= 28): each stage's gates are fused into one 16 x 16
+// complex matrix M (column c = the stage's gates applied in order to the unit
+// vector e_c of the stage's 4-bit group index), and each wave applies M to
+// its 64 groups, 16 at a time, as complex products on v_mfma_f32_16x16x4f32
+// (three real products per k-step, below).  Operand layouts of
+// v_mfma_f32_16x16x4f32 (checked on gfx950): A[i][k] in lane i + 16 k,
+// B[k][j] in lane j + 16 k, D[i][j] in lane j + 16 (i / 4), item i % 4.
+// Lane jl + 16 kq of wave w therefore holds, for column block cb, the
+// amplitudes P(kq, s) (s < 4) of one group as B operands and receives the
+// amplitudes P(kq, r) (r < 4) of the same group, where P and the group's rest
+// bits follow the stage's lane roles (MfmaMap, chosen on the host).  Loads,
+// support (live_in / live_out), the stage-0 prefetch of the next tile and the
+// LDS exchange between stages are those of fwd_pass_kernel.
 typedef float f4v __attribute__((ext_vector_type(4)));
 
 // index parts of the matrix-core pass (positions from MfmaMap words, one
