@@ -585,7 +585,7 @@ def test_circuit_state_large_n(gpu, n, layers, seed):
         assert err < 1e-12
 
 
-@pytest.mark.parametrize("n,layers,seed", [(12, 3, 0), (12, 10, 4), (13, 4, 1), (16, 3, 2), (20, 2, 3)])
+@pytest.mark.parametrize("n,layers,seed", [(12, 3, 0), (12, 10, 4), (13, 4, 1), (16, 3, 2), (20, 2, 3), (22, 2, 5)])
 def test_circuit_state_mfma_matches_oracle(gpu, n, layers, seed):
     """The complex64 forward passes on the matrix cores (each stage's gates
     fused into a 16 x 16 matrix, FP32 MFMA products) against the oracle's
