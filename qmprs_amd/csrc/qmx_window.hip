@@ -576,8 +576,8 @@ __global__ __launch_bounds__(256) void fwd_pass_kernel(FwdArgs<S> a) {
 // amplitudes P(kq, s) (s < 4) of one group as B operands and receives the
 // amplitudes P(kq, r) (r < 4) of the same group, where P and the group's rest
 // bits follow the stage's lane roles (MfmaMap, chosen on the host).  Loads,
-// support (live_in / live_out), the stage-0 prefetch of the next tile and the
-// LDS exchange between stages are those of fwd_pass_kernel.
+// support (live_in / live_out) and the LDS exchange between stages are those
+// of fwd_pass_kernel.
 typedef float f4v __attribute__((ext_vector_type(4)));
 
 // index parts of the matrix-core pass (positions from MfmaMap words, one
@@ -598,7 +598,7 @@ QMX_DEV uint32_t unipart(uint32_t h, uint32_t ks, int cb, int s) {
          ((uint32_t)(s & 1) << pos8(ks, 2)) | ((uint32_t)(s >> 1) << pos8(ks, 3));
 }
 
-__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3, 3))) void fwd_mfma_kernel(FwdArgs<float2> a) {
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4, 4))) void fwd_mfma_kernel(FwdArgs<float2> a) {
   extern __shared__ char wsm[];
   __shared__ int gcode[kMaxFwdGates];
   __shared__ uint32_t mmw[kMaxStages][3];  // per stage: jl, hl, ksl of MfmaMap
@@ -696,9 +696,13 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3, 3))) voi
         pre[cb * 4 + s4] = v;
       }
   };
-  if ((long)blockIdx.x < ntiles) load0((uint32_t)wdeposit((uint64_t)blockIdx.x, a.tmask));
+
   for (long tile = blockIdx.x; tile < ntiles; tile += gridDim.x) {
     const uint32_t base = (uint32_t)wdeposit((uint64_t)tile, a.tmask);
+    // stage 0's operands come from HBM at the tile's start: without a
+    // prefetch of the next tile's (32 registers) the kernel runs at 4 waves
+    // per SIMD instead of 3, which hides the load latency better
+    load0(base);
     for (int st = 0; st < a.nstage; ++st) {
       // lane roles of the stage (uniform: LDS broadcast, then scalar)
       const uint32_t mjl = __builtin_amdgcn_readfirstlane(mmw[st][0]);
@@ -713,8 +717,6 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3, 3))) voi
       if (st == 0) {
 #pragma unroll
         for (int q = 0; q < 16; ++q) xv[q] = pre[q];
-        const long nxt = tile + gridDim.x;
-        if (nxt < ntiles) load0((uint32_t)wdeposit((uint64_t)nxt, a.tmask));
       } else {
 #pragma unroll
         for (int cb = 0; cb < 4; ++cb)
