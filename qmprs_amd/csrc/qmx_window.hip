@@ -344,31 +344,34 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(WPE))) void
   };
   if (!STAGED && active && (long)blockIdx.x < ntiles)
     load_group((uint32_t)wdeposit((uint64_t)blockIdx.x, a.qmask) + roffg);
+  // Staged tiles of 2^kWinTileBits amplitudes over 256 threads: the 128
+  // T'-side threads copy the T' tile (16 amplitudes each, tile order), the
+  // 128 phi-side threads the phi tile, through the group registers v.  The
+  // next tile's loads are issued into v once v is dead (T' side: after the
+  // group phase; phi side: once its final values are back in LDS), and v is
+  // written to LDS at the next tile's start.
+  // With a contiguous low run of c0 >= 7 bits, tile index sl0 + 128 i
+  // (sl0 < 128) sits at state offset sl0 + ((128 i) & lomask) + hoff[(128 i)
+  // >> c0]: a per-lane part and a block-uniform part.
+  const bool fast_stage = STAGED && t == kWinTileBits && blockDim.x == 256 && c0 >= 7;
+  const int sl0 = threadIdx.x & 127;
+  uint32_t ust[16];
+#pragma unroll
+  for (int i = 0; i < 16; ++i)
+    ust[i] = fast_stage ? __builtin_amdgcn_readfirstlane(((128u * i) & lomask) + hoff[(128 * i) >> c0]) : 0u;
+  auto load_stage = [&](uint32_t tb) {
+#pragma unroll
+    for (int i = 0; i < 16; ++i) v[i] = T::mk(to_c(src[tb + ust[i] + sl0]));
+  };
+  if (fast_stage && (long)blockIdx.x < ntiles) load_stage((uint32_t)wdeposit((uint64_t)blockIdx.x, a.qmask));
   for (long tile = blockIdx.x; tile < ntiles; tile += gridDim.x) {
     const uint32_t base = (uint32_t)wdeposit((uint64_t)tile, a.qmask);
     const uint32_t gb = base + roffg;
+    const long nxt = tile + gridDim.x;
     if (STAGED) {
-      constexpr int kPer = (1 << kWinTileBits) / 256;
-      if (t == kWinTileBits && blockDim.x == 256) {
-        // half of the loads in flight before the first LDS store (all of
-        // them would not fit the register budget of 4 waves per SIMD)
-#pragma unroll 1
-        for (int h = 0; h < 2; ++h) {
-          S tv[kPer / 2], pv[kPer / 2];
+      if (fast_stage) {
 #pragma unroll
-          for (int i = 0; i < kPer / 2; ++i) {
-            const int l = threadIdx.x + 256 * (h * (kPer / 2) + i);
-            const uint32_t off = base + ((uint32_t)l & lomask) + hoff[l >> c0];
-            tv[i] = tsrc[off];
-            pv[i] = phi[off];
-          }
-#pragma unroll
-          for (int i = 0; i < kPer / 2; ++i) {
-            const int l = threadIdx.x + 256 * (h * (kPer / 2) + i);
-            sT[swz(l)] = tv[i];
-            sP[swz(l)] = pv[i];
-          }
-        }
+        for (int i = 0; i < 16; ++i) lds[swz(sl0 + 128 * i)] = from_c<S>(make_double2(v[i].x, v[i].y));
       } else {
         for (int l = threadIdx.x; l < (1 << t); l += blockDim.x) {
           const uint32_t off = base + ((uint32_t)l & lomask) + hoff[l >> c0];
@@ -396,9 +399,9 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(WPE))) void
         for (int x = 0; x < 16; ++x) dst[gb + xg[x]] = from_c<S>(make_double2(v[x].x, v[x].y));
       }
       if (!STAGED) {
-        const long nxt = tile + gridDim.x;
         if (nxt < ntiles) load_group((uint32_t)wdeposit((uint64_t)nxt, a.qmask) + roffg);
       }
+      if (fast_stage && tside && nxt < ntiles) load_stage((uint32_t)wdeposit((uint64_t)nxt, a.qmask));
     }
     __syncthreads();
     // R += conj(T') phi^T over this tile: A[x][k] = conj(T'[x, r]), B[k][y] = phi[y, r]
@@ -415,19 +418,29 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(WPE))) void
       acc_i = __builtin_amdgcn_mfma_f64_16x16x4f64(-tv.y, pv.x, acc_i, 0, 0, 0);
     }
     __syncthreads();
-    if (STAGED && a.store) {
+    if (STAGED) {
       // phi's final values (all window gates) replace its R operand in LDS,
       // then both tiles go back to HBM in tile order (T' holds its final
       // values since the group phase)
-      if (active && !tside) {
+      if (a.store && active && !tside) {
 #pragma unroll
         for (int x = 0; x < 16; ++x) sP[swz(roffl + xl[x])] = from_c<S>(make_double2(v[x].x, v[x].y));
       }
+      if (fast_stage && !tside && nxt < ntiles) load_stage((uint32_t)wdeposit((uint64_t)nxt, a.qmask));
+    }
+    if (STAGED && a.store) {
       __syncthreads();
-      for (int l = threadIdx.x; l < (1 << t); l += blockDim.x) {
-        const uint32_t off = base + ((uint32_t)l & lomask) + hoff[l >> c0];
-        if (!first) tdst[off] = sT[swz(l)];
-        phi[off] = sP[swz(l)];
+      if (fast_stage) {
+        if (store_dst) {
+#pragma unroll
+          for (int i = 0; i < 16; ++i) dst[base + ust[i] + sl0] = lds[swz(sl0 + 128 * i)];
+        }
+      } else {
+        for (int l = threadIdx.x; l < (1 << t); l += blockDim.x) {
+          const uint32_t off = base + ((uint32_t)l & lomask) + hoff[l >> c0];
+          if (!first) tdst[off] = sT[swz(l)];
+          phi[off] = sP[swz(l)];
+        }
       }
       __syncthreads();
     }

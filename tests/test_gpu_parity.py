@@ -523,17 +523,20 @@ def test_encode_ragged_rejects_bad_input(gpu):
         eng.encode_ragged([np.ones(8, np.complex64)], [0], 2, 1)
 
 
-@pytest.mark.parametrize("dtype,mode", [(torch.complex64, 1), (torch.complex128, 1), (torch.complex64, 2),
-                                        (torch.complex128, 2)])
-def test_staged_windows_equal_direct(gpu, dtype, mode):
+@pytest.mark.parametrize("dtype,mode,n,count", [(torch.complex64, 1, 13, 3), (torch.complex128, 1, 13, 3),
+                                                (torch.complex64, 2, 13, 3), (torch.complex128, 2, 13, 3),
+                                                (torch.complex64, 1, 20, 8), (torch.complex128, 1, 20, 8)])
+def test_staged_windows_equal_direct(gpu, dtype, mode, n, count):
     """Windows whose gate bits are the lowest state bits move their tiles
     through LDS in tile order (mode 1: tiles with a contiguous low run of
     2^7 amplitudes, mode 2: from 2^3); the arithmetic is the same as the
-    direct group loads, so results are identical bit for bit."""
+    direct group loads, so results are identical bit for bit.  At N=20 with
+    8 states each block takes two of a state's 512 tiles, so the next tile's
+    amplitudes are loaded while the current one is processed."""
     from qmprs_amd import _lib
     eng = _engine()
-    n, chi, L, S = 13, 16, 3, 2
-    svs = np.stack([O.random_mps_state(n, 24, 80 + s) for s in range(3)])
+    chi, L, S = 16, (3 if n < 20 else 2), (2 if n < 20 else 1)
+    svs = np.stack([O.random_mps_state(n, 24, 80 + s) for s in range(count)])
     _lib.set_option(_lib.QMX_OPT_STAGED_WINDOWS, 0)
     try:
         direct = eng.encode(svs, chi, L, S, sweep_dtype=dtype)
