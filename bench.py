@@ -181,9 +181,10 @@ def traffic_probe(args) -> int:
 def rebuild_probe(args) -> int:
     """Profiling body for the circuit rebuild: the bench batch's gates (chi
     truncation and layer extraction of the seeded states, no sweeps), then
-    qmx_circuit_state in complex64 three times with the forward passes on the
-    matrix cores (QMX_OPT_FWD_MFMA=1) and three times on the vector units
-    (QMX_OPT_FWD_MFMA=0).  Prints the mean wall time per call of each."""
+    qmx_circuit_state in ``--rebuild-dtype`` (complex64 or complex128) three
+    times with the forward passes on the matrix cores (QMX_OPT_FWD_MFMA=1)
+    and three times on the vector units (QMX_OPT_FWD_MFMA=0), each after one
+    untimed call.  Prints the mean wall time per call of each."""
     import time
     import torch
     from qmprs_amd import _lib, engine
@@ -193,17 +194,18 @@ def rebuild_probe(args) -> int:
     r = engine.encode(engine.as_device_states(svs), wl["bond_dimension"], wl["num_layers"], 0,
                       sweep_dtype=torch.complex64)
     torch.cuda.synchronize()
+    dtype = getattr(torch, args.rebuild_dtype)
     try:
         for mode in (1, 0):
             _lib.set_option(_lib.QMX_OPT_FWD_MFMA, mode)
-            engine.circuit_state(r.gates, r.widths, r.layers_used, dtype=torch.complex64)
+            engine.circuit_state(r.gates, r.widths, r.layers_used, dtype=dtype)
             torch.cuda.synchronize()
             t0 = time.perf_counter()
             for _ in range(3):
-                engine.circuit_state(r.gates, r.widths, r.layers_used, dtype=torch.complex64)
+                engine.circuit_state(r.gates, r.widths, r.layers_used, dtype=dtype)
             torch.cuda.synchronize()
             print(f"QMX_OPT_FWD_MFMA={mode}: {(time.perf_counter() - t0) / 3 * 1e3:.2f} ms per "
-                  f"circuit_state of {args.batch} states", flush=True)
+                  f"{args.rebuild_dtype} circuit_state of {args.batch} states", flush=True)
     finally:
         _lib.set_option(_lib.QMX_OPT_FWD_MFMA, _lib.FWD_MFMA_DEFAULT)
     return 0
@@ -529,8 +531,10 @@ def main() -> int:
                     help="skip the two rocprofv3 counter runs that measure roofline.traffic")
     ap.add_argument("--traffic-probe", action="store_true", help=argparse.SUPPRESS)
     ap.add_argument("--rebuild-probe", action="store_true",
-                    help="profiling body: complex64 circuit rebuilds of the bench batch, matrix-core and "
+                    help="profiling body: circuit rebuilds of the bench batch, matrix-core and "
                          "vector-unit forward passes (no JSON line)")
+    ap.add_argument("--rebuild-dtype", choices=("complex64", "complex128"), default="complex64",
+                    help="state type of --rebuild-probe")
     ap.add_argument("--no-warm-start", action="store_true",
                     help="skip the learned warm-start leg (training on N=16 plain-method "
                          "pairs, then the N=20 warm-started encode against the FP64 goal)")

@@ -716,7 +716,7 @@ int run_fwd(qmx_handle* h, int n, int batch, int lmax, const std::vector<FwdPass
     const uint64_t qmask = all & ~p;
     a.tmask = a.full ? qmask : (qmask & live);
     for (int k = 0; k < fp.nstage; ++k) a.gs[k] = make_group(n, p, fp.mask[k]);
-    if (h->fwd_mfma && t == kFwdTileBits && n <= 28)
+    if (fwd_mfma_applies(h->fwd_mfma, t, n, sizeof(S)))
       for (int k = 0; k < fp.nstage; ++k) a.mm[k] = make_mfma_map(n, p, a.gs[k], k == 0 || k + 1 == fp.nstage);
     for (int k = 0; k <= fp.nstage; ++k) a.gstart[k] = fp.gstart[k];
     for (int j = 0; j < fp.gstart[fp.nstage]; ++j) a.fpos[j] = fp.fpos[j];

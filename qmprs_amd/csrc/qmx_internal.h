@@ -127,12 +127,18 @@ struct WinPassArgs {
 constexpr int kMaxStages = 12;
 constexpr int kMaxFwdGates = 64;
 constexpr int kFwdTileBits = 12;
+// The forward pass runs on the matrix cores (fwd_mfma_kernel for complex64,
+// fwd_mfma64_kernel for complex128) when enabled, on tiles of 2^kFwdTileBits
+// amplitudes, while 32-bit byte offsets reach every amplitude.
+inline bool fwd_mfma_applies(int enabled, int t, int n, size_t amp_bytes) {
+  return enabled && t == kFwdTileBits && n <= (amp_bytes == 8 ? 28 : 27);
+}
 constexpr int kFwdMaxUnion = 8;      // gate bits per pass (>= 4 coalescing bits)
 template <typename S>
 struct FwdArgs {
   int n, lmax, t, nstage;
   int from_basis0;     // the input is |0..0> (psi is not read)
-  int mfma;            // complex64 tiles of 2^kFwdTileBits: stage matrices on the matrix cores
+  int mfma;            // tiles of 2^kFwdTileBits: stage matrices on the matrix cores (fwd_mfma_applies)
   int full;            // write every amplitude (else only those inside live_out)
   uint64_t tmask;      // tile-index bits visited (state bits outside the tile)
   uint64_t live_in;    // amplitudes with a bit outside live_in are zero before the pass

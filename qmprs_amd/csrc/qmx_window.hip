@@ -577,41 +577,76 @@ __global__ __launch_bounds__(256) void fwd_pass_kernel(FwdArgs<S> a) {
   }
 }
 
-// The forward pass on the matrix cores (complex64 states, tiles of 2^12
-// amplitudes, N <= 28): each stage's gates are fused into one 16 x 16
-// complex matrix M (column c = the stage's gates applied in order to the unit
-// vector e_c of the stage's 4-bit group index), and each wave applies M to
-// its 64 groups, 16 at a time, as complex products on v_mfma_f32_16x16x4f32
-// (three real products per k-step, below).  Operand layouts of
-// v_mfma_f32_16x16x4f32 (checked on gfx950): A[i][k] in lane i + 16 k,
-// B[k][j] in lane j + 16 k, D[i][j] in lane j + 16 (i / 4), item i % 4.
-// Lane jl + 16 kq of wave w therefore holds, for column block cb, the
-// amplitudes P(kq, s) (s < 4) of one group as B operands and receives the
-// amplitudes P(kq, r) (r < 4) of the same group, where P and the group's rest
-// bits follow the stage's lane roles (MfmaMap, chosen on the host).  Loads,
-// support (live_in / live_out) and the LDS exchange between stages are those
-// of fwd_pass_kernel.
+// The forward pass on the matrix cores (tiles of 2^12 amplitudes; complex64
+// states with N <= 28, complex128 with N <= 27): each stage's gates are fused
+// into one 16 x 16 complex matrix M (column c = the stage's gates applied in
+// order to the unit vector e_c of the stage's 4-bit group index), and each
+// wave applies M to its groups, 16 at a time, as complex products on
+// v_mfma_f32_16x16x4f32 or v_mfma_f64_16x16x4f64 (three real products per
+// k-step, below).  Operand layouts (checked on gfx950): A[i][k] in lane
+// i + 16 k and B[k][j] in lane j + 16 k for both; D[i][j] in lane
+// j + 16 (i / 4), item i % 4 for f32 and in lane j + 16 (i % 4), item i / 4
+// for f64.  Lane jl + 16 kq of a wave therefore holds, for column block cb,
+// the amplitudes P(kq, s) (s < 4) of one group as B operands and receives the
+// amplitudes P(kq, r) (r < 4) of the same group in item r, when the rows of M
+// are ordered by MfmaT::row; P and the group's rest bits follow the stage's
+// lane roles (MfmaMap, chosen on the host).  Loads, support (live_in /
+// live_out) and the LDS exchange between stages are those of fwd_pass_kernel.
 typedef float f4v __attribute__((ext_vector_type(4)));
+
+// Per-type parts of the matrix-core pass.  A tile holds 256 groups: complex64
+// runs 4 waves of 4 column blocks (16 groups each), complex128 8 waves of 2
+// (its 64 KiB tile allows one block per CU; 8 waves make 2 per SIMD).
+template <typename S> struct MfmaT;
+template <> struct MfmaT<float2> {
+  using R = float;
+  using V4 = f4v;
+  static constexpr int kWaves = 4;
+  static constexpr int kShift = 3;   // log2 bytes per amplitude
+  static QMX_DEV V4 mfma(R x, R y, V4 c) { return __builtin_amdgcn_mfma_f32_16x16x4f32(x, y, c, 0, 0, 0); }
+  static QMX_DEV float2 mk(double x, double y) { return make_float2((float)x, (float)y); }
+  static QMX_DEV int row(int q, int r) { return 4 * q + r; }
+};
+template <> struct MfmaT<double2> {
+  using R = double;
+  using V4 = d4v;
+  static constexpr int kWaves = 8;
+  static constexpr int kShift = 4;
+  static QMX_DEV V4 mfma(R x, R y, V4 c) { return __builtin_amdgcn_mfma_f64_16x16x4f64(x, y, c, 0, 0, 0); }
+  static QMX_DEV double2 mk(double x, double y) { return make_double2(x, y); }
+  static QMX_DEV int row(int q, int r) { return q + 4 * r; }
+};
 
 // index parts of the matrix-core pass (positions from MfmaMap words, one
 // per byte): the lane's (jl at j[0..3], kq at ks[0..1]), the wave's (w at
-// h[2..3]), and the uniform (cb at h[0..1], s at ks[2..3]); the three are
-// disjoint bit fields
+// h[NB..3]), and the uniform (cb at h[0..NB-1], s at ks[2..3]), NB = 2 with 4
+// column blocks per wave and 1 with 2; the three are disjoint bit fields
 QMX_DEV uint32_t pos8(uint32_t w, int i) { return (w >> (8 * i)) & 255u; }
 QMX_DEV uint32_t lanepart(uint32_t j, uint32_t ks, int jl, int kq) {
   return ((uint32_t)(jl & 1) << pos8(j, 0)) | ((uint32_t)((jl >> 1) & 1) << pos8(j, 1)) |
          ((uint32_t)((jl >> 2) & 1) << pos8(j, 2)) | ((uint32_t)(jl >> 3) << pos8(j, 3)) |
          ((uint32_t)(kq & 1) << pos8(ks, 0)) | ((uint32_t)(kq >> 1) << pos8(ks, 1));
 }
-QMX_DEV uint32_t wavepart(uint32_t h, int w) {
-  return ((uint32_t)(w & 1) << pos8(h, 2)) | ((uint32_t)(w >> 1) << pos8(h, 3));
+template <int NB> QMX_DEV uint32_t wavepart(uint32_t h, int w) {
+  uint32_t r = 0;
+#pragma unroll
+  for (int i = NB; i < 4; ++i) r |= (uint32_t)((w >> (i - NB)) & 1) << pos8(h, i);
+  return r;
 }
-QMX_DEV uint32_t unipart(uint32_t h, uint32_t ks, int cb, int s) {
-  return ((uint32_t)(cb & 1) << pos8(h, 0)) | ((uint32_t)(cb >> 1) << pos8(h, 1)) |
-         ((uint32_t)(s & 1) << pos8(ks, 2)) | ((uint32_t)(s >> 1) << pos8(ks, 3));
+template <int NB> QMX_DEV uint32_t unipart(uint32_t h, uint32_t ks, int cb, int s) {
+  uint32_t r = ((uint32_t)(s & 1) << pos8(ks, 2)) | ((uint32_t)(s >> 1) << pos8(ks, 3));
+#pragma unroll
+  for (int i = 0; i < NB; ++i) r |= (uint32_t)((cb >> i) & 1) << pos8(h, i);
+  return r;
 }
 
-__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4, 4))) void fwd_mfma_kernel(FwdArgs<float2> a) {
+template <typename S>
+__attribute__((always_inline)) QMX_DEV void fwd_mfma_body(const FwdArgs<S>& a) {
+  using M = MfmaT<S>;
+  using R = typename M::R;
+  using V4 = typename M::V4;
+  constexpr int kCB = 16 / M::kWaves;       // column blocks of 16 groups per wave
+  constexpr int kNB = kCB == 4 ? 2 : 1;     // their index bits
   extern __shared__ char wsm[];
   __shared__ int gcode[kMaxFwdGates];
   __shared__ uint32_t mmw[kMaxStages][3];  // per stage: jl, hl, ksl of MfmaMap
@@ -620,16 +655,16 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4, 4))) voi
   if (a.skip && a.skip[b]) return;
   const int lb = a.layers[b];
   const int ng = a.gstart[a.nstage];
-  const int t = a.t;   // kFwdTileBits: 256 groups of 16, one wave per 64
-  float2* lds = reinterpret_cast<float2*>(wsm);
+  const int t = a.t;   // kFwdTileBits: 256 groups of 16, 16 kCB per wave
+  S* lds = reinterpret_cast<S*>(wsm);
   // the stage matrices after the tile, in A-operand order: with P(q, s) the
   // amplitude index whose bits kx are q and bits sx are s (lane roles),
   // mst[st][s][jl + 16 kq] = M[P(jl / 4, jl % 4)][P(kq, s)], so that a
   // wave's A load is contiguous
-  float2* mst = lds + ((size_t)1 << t);
+  S* mst = lds + ((size_t)1 << t);
   // gates (as G^dagger, which grp_apply applies as G) in the tile area, which
   // stage 0 only writes after the matrices are built
-  float2(*gmat)[16] = reinterpret_cast<float2(*)[16]>(wsm);
+  S(*gmat)[16] = reinterpret_cast<S(*)[16]>(wsm);
   for (int e2 = threadIdx.x; e2 < ng * 16; e2 += blockDim.x) {
     const int e = e2 & 15, j = e2 >> 4;
     const int f = a.fpos[j];
@@ -639,7 +674,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4, 4))) voi
       const int layer = lb - 1 - f / n, site = f % n;
       const size_t gi = ((size_t)b * a.lmax + layer) * n + site;
       const double2 gv = a.gates[gi * 16 + (e & 3) * 4 + (e >> 2)];
-      gmat[j][e] = make_float2((float)gv.x, (float)-gv.y);
+      gmat[j][e] = M::mk(gv.x, -gv.y);
       if (e == 0) gcode[j] = gate_code(a.gs[st].mask, n, site, wsane(n, site, a.widths[gi]));
     } else if (e == 0) {
       gcode[j] = -1;
@@ -653,17 +688,17 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4, 4))) voi
   if ((int)threadIdx.x < a.nstage * 16) {
     const int st = threadIdx.x >> 4, c = threadIdx.x & 15;
     const MfmaMap& mm = a.mm[st];
-    float2 v[16];
+    S v[16];
 #pragma unroll
-    for (int x = 0; x < 16; ++x) v[x] = make_float2(x == c ? 1.f : 0.f, 0.f);
-    for (int j = a.gstart[st]; j < a.gstart[st + 1]; ++j) grp_apply<float>(gcode[j], v, gmat[j]);
+    for (int x = 0; x < 16; ++x) v[x] = M::mk(x == c ? 1.0 : 0.0, 0.0);
+    for (int j = a.gstart[st]; j < a.gstart[st + 1]; ++j) grp_apply<R>(gcode[j], v, gmat[j]);
     // amplitude index x <-> (q, s): q = its bits kx, s = its bits sx
     const uint32_t kx0 = pos8(mm.kxsx, 0), kx1 = pos8(mm.kxsx, 1), sx0 = pos8(mm.kxsx, 2), sx1 = pos8(mm.kxsx, 3);
     auto qof = [&](int x) { return (int)(((x >> kx0) & 1) | (((x >> kx1) & 1) << 1)); };
     auto sof = [&](int x) { return (int)(((x >> sx0) & 1) | (((x >> sx1) & 1) << 1)); };
     const int kc = qof(c), sc = sof(c);
 #pragma unroll
-    for (int x = 0; x < 16; ++x) mst[(st * 4 + sc) * 64 + 16 * kc + 4 * qof(x) + sof(x)] = v[x];
+    for (int x = 0; x < 16; ++x) mst[(st * 4 + sc) * 64 + 16 * kc + M::row(qof(x), sof(x))] = v[x];
     if (c == 0) {
       mmw[st][0] = mm.jl;
       mmw[st][1] = mm.hl;
@@ -672,7 +707,8 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4, 4))) voi
   }
   __syncthreads();
   const size_t len = (size_t)1 << n;
-  // 32-bit byte offsets from a scalar base (N <= 28: offsets below 2^31)
+  // 32-bit byte offsets from a scalar base (below 2^32 for N <= 28 in
+  // complex64 and N <= 27 in complex128)
   char* pbytes = reinterpret_cast<char*>(a.psi + (size_t)b * len);
   const long ntiles = 1l << __popcll(a.tmask);
   const uint32_t dead_in = (uint32_t)(~a.live_in & all), dead_out = (uint32_t)(~a.live_out & all);
@@ -690,22 +726,22 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4, 4))) voi
   const uint32_t hLg = __builtin_amdgcn_readfirstlane(mL.hg), kLg = __builtin_amdgcn_readfirstlane(mL.ksg);
   const uint32_t g0lane = lanepart(m0.jg, k0g, jl, kq);
   const uint32_t gLlane = lanepart(mL.jg, kLg, jl, kq);
-  const uint32_t g0w = wavepart(h0g, wave), gLw = wavepart(hLg, wave);
+  const uint32_t g0w = wavepart<kNB>(h0g, wave), gLw = wavepart<kNB>(hLg, wave);
   // B operands of stage 0 for one tile: [cb * 4 + s] = amplitude P(kq, s)
-  float2 pre[16];
+  S pre[4 * kCB];
   auto load0 = [&](uint32_t bse) {
 #pragma unroll
-    for (int cb = 0; cb < 4; ++cb)
+    for (int cb = 0; cb < kCB; ++cb)
 #pragma unroll
       for (int s4 = 0; s4 < 4; ++s4) {
-        const uint32_t ad = bse + (g0w | unipart(h0g, k0g, cb, s4)) + g0lane;
-        float2 v;
+        const uint32_t ad = bse + (g0w | unipart<kNB>(h0g, k0g, cb, s4)) + g0lane;
+        S v;
         if (ad & dead_in)
-          v = make_float2(0.f, 0.f);
+          v = M::mk(0.0, 0.0);
         else if (a.from_basis0)
-          v = make_float2(ad == 0u ? 1.f : 0.f, 0.f);
+          v = M::mk(ad == 0u ? 1.0 : 0.0, 0.0);
         else
-          v = *reinterpret_cast<const float2*>(pbytes + (ad << 3));
+          v = *reinterpret_cast<const S*>(pbytes + (ad << M::kShift));
         pre[cb * 4 + s4] = v;
       }
   };
@@ -725,56 +761,63 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4, 4))) voi
       // tile-local offsets: per lane (jl, kq, swizzled once per stage) and
       // per (wave, cb, s or r)
       const int lsw = swz((int)lanepart(mjl, mksl, jl, kq));
-      const uint32_t lw = wavepart(mhl, wave);
-      float2 xv[16];
+      const uint32_t lw = wavepart<kNB>(mhl, wave);
+      S xv[4 * kCB];
       if (st == 0) {
 #pragma unroll
-        for (int q = 0; q < 16; ++q) xv[q] = pre[q];
+        for (int q = 0; q < 4 * kCB; ++q) xv[q] = pre[q];
       } else {
 #pragma unroll
-        for (int cb = 0; cb < 4; ++cb)
+        for (int cb = 0; cb < kCB; ++cb)
 #pragma unroll
-          for (int s4 = 0; s4 < 4; ++s4) xv[cb * 4 + s4] = lds[lsw ^ swz((int)(lw | unipart(mhl, mksl, cb, s4)))];
+          for (int s4 = 0; s4 < 4; ++s4) xv[cb * 4 + s4] = lds[lsw ^ swz((int)(lw | unipart<kNB>(mhl, mksl, cb, s4)))];
       }
       // A operands: M[jl][4 s + kq]
-      float2 am[4];
+      S am[4];
 #pragma unroll
       for (int s4 = 0; s4 < 4; ++s4) am[s4] = mst[(st * 4 + s4) * 64 + lane];
       // one column block at a time, each complex product as three real ones
       // (Gauss): k1 = Re M Re x, k2 = Im M Im x, k3 = (Re M + Im M)(Re x +
       // Im x); Re y = k1 - k2, Im y = k3 - k1 - k2.  The three accumulators
       // alternate, so consecutive products are independent.
-      float asum[4];
+      R asum[4];
 #pragma unroll
       for (int s4 = 0; s4 < 4; ++s4) asum[s4] = am[s4].x + am[s4].y;
 #pragma unroll
-      for (int cb = 0; cb < 4; ++cb) {
-        f4v k1 = {0.f, 0.f, 0.f, 0.f}, k2 = {0.f, 0.f, 0.f, 0.f}, k3 = {0.f, 0.f, 0.f, 0.f};
+      for (int cb = 0; cb < kCB; ++cb) {
+        V4 k1 = {0, 0, 0, 0}, k2 = {0, 0, 0, 0}, k3 = {0, 0, 0, 0};
 #pragma unroll
         for (int s4 = 0; s4 < 4; ++s4) {
-          const float2 x = xv[cb * 4 + s4];
-          k1 = __builtin_amdgcn_mfma_f32_16x16x4f32(am[s4].x, x.x, k1, 0, 0, 0);
-          k2 = __builtin_amdgcn_mfma_f32_16x16x4f32(am[s4].y, x.y, k2, 0, 0, 0);
-          k3 = __builtin_amdgcn_mfma_f32_16x16x4f32(asum[s4], x.x + x.y, k3, 0, 0, 0);
+          const S x = xv[cb * 4 + s4];
+          k1 = M::mfma(am[s4].x, x.x, k1);
+          k2 = M::mfma(am[s4].y, x.y, k2);
+          k3 = M::mfma(asum[s4], x.x + x.y, k3);
         }
-        const f4v yr = k1 - k2, yi = k3 - k1 - k2;
+        const V4 yr = k1 - k2, yi = k3 - k1 - k2;
         if (last) {
 #pragma unroll
           for (int r = 0; r < 4; ++r) {
-            const uint32_t ad = base + (gLw | unipart(hLg, kLg, cb, r)) + gLlane;
+            const uint32_t ad = base + (gLw | unipart<kNB>(hLg, kLg, cb, r)) + gLlane;
             if (a.full || !(ad & dead_out))
-              *reinterpret_cast<float2*>(pbytes + (ad << 3)) = make_float2(yr[r], yi[r]);
+              *reinterpret_cast<S*>(pbytes + (ad << M::kShift)) = M::mk(yr[r], yi[r]);
           }
         } else {
 #pragma unroll
           for (int r = 0; r < 4; ++r)
-            lds[lsw ^ swz((int)(lw | unipart(mhl, mksl, cb, r)))] = make_float2(yr[r], yi[r]);
+            lds[lsw ^ swz((int)(lw | unipart<kNB>(mhl, mksl, cb, r)))] = M::mk(yr[r], yi[r]);
         }
       }
       // stage exchange, and the next tile's stage 0 must not overwrite LDS early
       if (a.nstage > 1) __syncthreads();
     }
   }
+}
+
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4, 4))) void fwd_mfma_kernel(FwdArgs<float2> a) {
+  fwd_mfma_body(a);
+}
+__global__ __launch_bounds__(512) __attribute__((amdgpu_waves_per_eu(2, 2))) void fwd_mfma64_kernel(FwdArgs<double2> a) {
+  fwd_mfma_body(a);
 }
 
 // Replay the window on the 16-dim space of U (one block per state).
@@ -947,12 +990,21 @@ __global__ __launch_bounds__(256) void window_finalize_kernel(WinFinArgs a) {
 template <typename S>
 void launch_fwd_pass(const FwdArgs<S>& a, int nblk, int batch, hipStream_t st) {
   size_t shm = sizeof(S) << a.t;
-  if constexpr (sizeof(S) == 8) {
-    if (a.mfma && a.t == kFwdTileBits && a.n <= 28) {
-      // the tile, then one 16 x 16 complex64 matrix per stage
-      shm += (size_t)a.nstage * 256 * sizeof(float2);
+  // the host fills a.mm under the same condition (fwd_mfma_applies)
+  if (fwd_mfma_applies(a.mfma, a.t, a.n, sizeof(S))) {
+    // the tile, then one 16 x 16 complex matrix per stage
+    shm += (size_t)a.nstage * 256 * sizeof(S);
+    if constexpr (sizeof(S) == 8) {
       hipLaunchKernelGGL(fwd_mfma_kernel, dim3(nblk, batch), dim3(256), shm, st, a);
       return;
+    } else {
+      // above the default 64 KiB of dynamic LDS
+      if (hipFuncSetAttribute(reinterpret_cast<const void*>(fwd_mfma64_kernel),
+                              hipFuncAttributeMaxDynamicSharedMemorySize, (int)shm) == hipSuccess) {
+        hipLaunchKernelGGL(fwd_mfma64_kernel, dim3(nblk, batch), dim3(512), shm, st, a);
+        return;
+      }
+      shm = sizeof(S) << a.t;
     }
   }
   hipLaunchKernelGGL(fwd_pass_kernel<S>, dim3(nblk, batch), dim3(256), shm, st, a);

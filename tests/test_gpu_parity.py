@@ -410,24 +410,25 @@ def test_split_sweeps_equal_single_stream(gpu, batch):
     assert (one.fidelity - two.fidelity).abs().max().item() < 1e-12
 
 
-def test_encode_mfma_rebuild_matches_vector_rebuild(gpu):
-    """A complex64 encode (N=14, 3 sweeps) with the circuit rebuilds on the
-    matrix cores and with the vector-unit rebuilds: the same layers, and
-    fidelities within the FP32 rounding that separates the two (the sweeps
-    start from rebuilt states that differ by ~1e-7); both encodes' final
-    gates re-evaluated in FP64 give the same fidelity.  Gate entries are not
-    compared: where a gate's environment is rank-deficient its polar factor
-    is not unique, and rounding picks a different completion of the same
-    circuit fidelity (entries differ by O(1) there)."""
+@pytest.mark.parametrize("dtype,tol", [(torch.complex64, 1e-5), (torch.complex128, 1e-10)])
+def test_encode_mfma_rebuild_matches_vector_rebuild(gpu, dtype, tol):
+    """An encode (N=14, 3 sweeps) with the circuit rebuilds on the matrix
+    cores and with the vector-unit rebuilds: the same layers, and fidelities
+    within the rounding that separates the two (complex64: the sweeps start
+    from rebuilt states that differ by ~1e-7; complex128: by ~1e-16); both
+    encodes' final gates re-evaluated in FP64 give the same fidelity.  Gate
+    entries are not compared: where a gate's environment is rank-deficient
+    its polar factor is not unique, and rounding picks a different completion
+    of the same circuit fidelity (entries differ by O(1) there)."""
     from qmprs_amd import _lib
     eng = _engine()
     n, chi, L, S = 14, 16, 3, 3
     svs = np.stack([O.random_mps_state(n, 24, 500 + s) for s in range(3)])
     try:
         _lib.set_option(_lib.QMX_OPT_FWD_MFMA, 0)
-        vec = eng.encode(svs, chi, L, S, sweep_dtype=torch.complex64)
+        vec = eng.encode(svs, chi, L, S, sweep_dtype=dtype)
         _lib.set_option(_lib.QMX_OPT_FWD_MFMA, 1)
-        mat = eng.encode(svs, chi, L, S, sweep_dtype=torch.complex64)
+        mat = eng.encode(svs, chi, L, S, sweep_dtype=dtype)
     finally:
         _lib.set_option(_lib.QMX_OPT_FWD_MFMA, _lib.FWD_MFMA_DEFAULT)
     torch.cuda.synchronize()
@@ -439,7 +440,7 @@ def test_encode_mfma_rebuild_matches_vector_rebuild(gpu):
     _, fm = eng.circuit_state(mat.gates, mat.widths, mat.layers_used, psi_ref=psi)
     d64 = (fv.abs() - fm.abs()).abs().max().item()
     print(f"max |dgate| {dg:.2e}, max |dF| {df:.2e}, FP64 re-evaluation |dF| {d64:.2e}")
-    assert df < 1e-5 and d64 < 1e-5
+    assert df < tol and d64 < tol
 
 
 @pytest.mark.parametrize("omega", [1.0, 1.7])
@@ -588,13 +589,15 @@ def test_circuit_state_large_n(gpu, n, layers, seed):
         assert err < 1e-12
 
 
+@pytest.mark.parametrize("dtype", [torch.complex64, torch.complex128])
 @pytest.mark.parametrize("n,layers,seed", [(12, 3, 0), (12, 10, 4), (13, 4, 1), (16, 3, 2), (20, 2, 3), (22, 2, 5)])
-def test_circuit_state_mfma_matches_oracle(gpu, n, layers, seed):
-    """The complex64 forward passes on the matrix cores (each stage's gates
-    fused into a 16 x 16 matrix, FP32 MFMA products) against the oracle's
-    dense forward application and against the vector-unit passes that apply
-    the gates one by one: FP32 rounding apart, the same state (max abs error
-    below 2e-6 on states of norm 1) and the same overlap with a reference.
+def test_circuit_state_mfma_matches_oracle(gpu, dtype, n, layers, seed):
+    """The forward passes on the matrix cores (each stage's gates fused into
+    a 16 x 16 matrix; FP32 MFMA products for complex64 states, FP64 for
+    complex128) against the oracle's dense forward application and against
+    the vector-unit passes that apply the gates one by one: rounding apart,
+    the same state (max abs error below 2e-6 in complex64, 1e-12 in
+    complex128, on states of norm 1) and the same overlap with a reference.
     The plans include one-stage passes (N=13, 4 layers: HBM to HBM, no LDS
     exchange) and twelve-stage passes (N=12, 10 layers: the most stage
     matrices a block holds)."""
@@ -620,12 +623,13 @@ def test_circuit_state_mfma_matches_oracle(gpu, n, layers, seed):
     psi_ref = torch.from_numpy(psi).to(gpu).contiguous()
     try:
         _lib.set_option(_lib.QMX_OPT_FWD_MFMA, 1)
-        s_mfma, f_mfma = eng.circuit_state(gates, widths, lu, dtype=torch.complex64, psi_ref=psi_ref)
+        s_mfma, f_mfma = eng.circuit_state(gates, widths, lu, dtype=dtype, psi_ref=psi_ref)
         _lib.set_option(_lib.QMX_OPT_FWD_MFMA, 0)
-        s_valu, f_valu = eng.circuit_state(gates, widths, lu, dtype=torch.complex64, psi_ref=psi_ref)
+        s_valu, f_valu = eng.circuit_state(gates, widths, lu, dtype=dtype, psi_ref=psi_ref)
     finally:
         _lib.set_option(_lib.QMX_OPT_FWD_MFMA, _lib.FWD_MFMA_DEFAULT)
     torch.cuda.synchronize()
+    tol, ftol = (2e-6, 1e-5) if dtype == torch.complex64 else (1e-12, 1e-12)
     for b in range(B):
         gl = _gate_list_from_device(g[b], w[b], int(used[b]))
         ref = O.circuit_state(n, gl)
@@ -634,9 +638,9 @@ def test_circuit_state_mfma_matches_oracle(gpu, n, layers, seed):
         em, ev = np.abs(sm - ref).max(), np.abs(sv - ref).max()
         fr = np.vdot(psi[b], ref)
         print(f"n={n} state {b} layers {used[b]}: mfma err {em:.3e}, vector err {ev:.3e}")
-        assert em < 2e-6 and ev < 2e-6
-        assert np.abs(sm - sv).max() < 2e-6
-        assert abs(complex(f_mfma[b]) - fr) < 1e-5 and abs(complex(f_valu[b]) - fr) < 1e-5
+        assert em < tol and ev < tol
+        assert np.abs(sm - sv).max() < tol
+        assert abs(complex(f_mfma[b]) - fr) < ftol and abs(complex(f_valu[b]) - fr) < ftol
 
 
 @pytest.mark.parametrize("schedule", ["IterDiOi", "IterDiOall"])
