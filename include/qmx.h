@@ -70,11 +70,12 @@ const char* qmx_last_error(const qmx_handle* h);
 /* QMX_OPT_FWD_BLOCKS_PER_CU: grid of the forward-circuit rebuild passes, in
  * blocks per compute unit over the whole batch (1..16, default 4). */
 #define QMX_OPT_FWD_BLOCKS_PER_CU 4
-/* QMX_OPT_FWD_MFMA: 1 (default) runs the forward-circuit rebuild of
- * complex64 states with 2^12-amplitude tiles (12 <= N <= 28) on the matrix
- * cores: each stage's gates are fused into one 16 x 16 complex matrix and
- * applied with FP32 MFMA products; 0 applies the gates one by one on the
- * vector units.  Results agree to FP32 rounding. */
+/* QMX_OPT_FWD_MFMA: 1 (default) runs the forward-circuit rebuild on the
+ * matrix cores, with 2^12-amplitude tiles: complex64 states with
+ * 12 <= N <= 28 (FP32 MFMA products) and complex128 states with
+ * 12 <= N <= 27 (FP64 MFMA products); each stage's gates are fused into one
+ * 16 x 16 complex matrix.  0 applies the gates one by one on the vector
+ * units.  Results agree to the rounding of the state's type. */
 #define QMX_OPT_FWD_MFMA 9
 /* QMX_OPT_STAGED_WINDOWS: windows whose register groups stride the state
  * (gate bits at the lowest state bits) can copy their tiles between HBM and
