@@ -655,12 +655,12 @@ __attribute__((always_inline)) QMX_DEV void fwd_mfma_body(const FwdArgs<S>& a) {
   if (a.skip && a.skip[b]) return;
   const int lb = a.layers[b];
   const int ng = a.gstart[a.nstage];
-  const int t = a.t;   // kFwdTileBits: 256 groups of 16, 16 kCB per wave
+  const int t = a.t;   // kFwdTileBits: 256 groups of 16, 16 * kCB per wave
   S* lds = reinterpret_cast<S*>(wsm);
   // the stage matrices after the tile, in A-operand order: with P(q, s) the
   // amplitude index whose bits kx are q and bits sx are s (lane roles),
-  // mst[st][s][jl + 16 kq] = M[P(jl / 4, jl % 4)][P(kq, s)], so that a
-  // wave's A load is contiguous
+  // mst[st][s][row(q, r) + 16 kq] = M[P(q, r)][P(kq, s)], so that a wave's
+  // A load is contiguous
   S* mst = lds + ((size_t)1 << t);
   // gates (as G^dagger, which grp_apply applies as G) in the tile area, which
   // stage 0 only writes after the matrices are built
