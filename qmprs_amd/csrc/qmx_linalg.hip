@@ -912,8 +912,13 @@ void launch_jacobi_lds(double2* a, double2* v, size_t stride, const int* ncols_d
                      error);
 }
 
+// The dynamic-LDS limit is set to the largest triangle the kernel takes
+// (kJacobiLdsMax columns), whatever ncols_max this call needs: calls from
+// several host threads (one per ragged bucket) then never lower it under a
+// launch that needs more.
 hipError_t jacobi_lds_setup(int ncols_max) {
-  const size_t tri_bytes = sizeof(double2) * (size_t)ncols_max * (ncols_max + 1) / 2;
+  if (ncols_max > kJacobiLdsMax) return hipErrorInvalidValue;
+  const size_t tri_bytes = sizeof(double2) * (size_t)kJacobiLdsMax * (kJacobiLdsMax + 1) / 2;
   return hipFuncSetAttribute(reinterpret_cast<const void*>(jacobi_lds_sweep_kernel),
                              hipFuncAttributeMaxDynamicSharedMemorySize, (int)tri_bytes);
 }

@@ -998,9 +998,12 @@ void launch_fwd_pass(const FwdArgs<S>& a, int nblk, int batch, hipStream_t st) {
       hipLaunchKernelGGL(fwd_mfma_kernel, dim3(nblk, batch), dim3(256), shm, st, a);
       return;
     } else {
-      // above the default 64 KiB of dynamic LDS
+      // above the default 64 KiB of dynamic LDS; the limit is always set to
+      // the kernel's largest need (tile + kMaxStages matrices), so that
+      // launches from several host threads never lower it under each other
+      const size_t most = (sizeof(S) << kFwdTileBits) + (size_t)kMaxStages * 256 * sizeof(S);
       if (hipFuncSetAttribute(reinterpret_cast<const void*>(fwd_mfma64_kernel),
-                              hipFuncAttributeMaxDynamicSharedMemorySize, (int)shm) == hipSuccess) {
+                              hipFuncAttributeMaxDynamicSharedMemorySize, (int)most) == hipSuccess) {
         hipLaunchKernelGGL(fwd_mfma64_kernel, dim3(nblk, batch), dim3(512), shm, st, a);
         return;
       }
