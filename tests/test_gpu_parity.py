@@ -643,6 +643,49 @@ def test_circuit_state_mfma_matches_oracle(gpu, dtype, n, layers, seed):
         assert abs(complex(f_mfma[b]) - fr) < ftol and abs(complex(f_valu[b]) - fr) < ftol
 
 
+def test_circuit_state_mfma_fp64_largest_size(gpu):
+    """The complex128 matrix-core forward passes at their largest size, N=27
+    (32-bit byte offsets up to 2^31 - 16), against the vector-unit passes on
+    one state of 2^27 amplitudes: the same state to 1e-12 (max abs), norm 1
+    to 1e-12 (the circuit is unitary) and the same overlap with a random
+    reference state.  The dense oracle is out of reach at this size; the
+    vector passes are checked against it at N <= 22 above."""
+    from qmprs_amd import _lib
+    import torch
+    eng = _engine()
+    n, layers = 27, 2
+    rng = np.random.default_rng(2027)
+    g = np.zeros((1, layers, n, 4, 4), dtype=np.complex128)
+    w = np.full((1, layers, n), 2, dtype=np.int32)
+    for l in range(layers):
+        w[0, l, n - 1] = 1
+        for s in rng.choice(n - 1, size=3, replace=False):
+            w[0, l, s] = 1
+        for s in range(n):
+            d = 4 if w[0, l, s] == 2 else 2
+            g[0, l, s, :d, :d] = _random_unitary(rng, d)
+    gates = torch.from_numpy(g).to(gpu).contiguous()
+    widths = torch.from_numpy(w).to(gpu).contiguous()
+    lu = torch.tensor([layers], dtype=torch.int32, device=gpu)
+    gen = torch.Generator(device=gpu)
+    gen.manual_seed(27)
+    ref = torch.randn((1, 1 << n), dtype=torch.complex128, device=gpu, generator=gen)
+    ref /= torch.linalg.vector_norm(ref)
+    try:
+        _lib.set_option(_lib.QMX_OPT_FWD_MFMA, 1)
+        s_mfma, f_mfma = eng.circuit_state(gates, widths, lu, dtype=torch.complex128, psi_ref=ref)
+        _lib.set_option(_lib.QMX_OPT_FWD_MFMA, 0)
+        s_valu, f_valu = eng.circuit_state(gates, widths, lu, dtype=torch.complex128, psi_ref=ref)
+    finally:
+        _lib.set_option(_lib.QMX_OPT_FWD_MFMA, _lib.FWD_MFMA_DEFAULT)
+    torch.cuda.synchronize()
+    diff = (s_mfma - s_valu).abs().max().item()
+    norm = torch.linalg.vector_norm(s_mfma).item()
+    df = abs(complex(f_mfma[0]) - complex(f_valu[0]))
+    print(f"N=27 complex128: max |mfma - vector| {diff:.2e}, norm - 1 {norm - 1:.2e}, overlap diff {df:.2e}")
+    assert diff < 1e-12 and abs(norm - 1.0) < 1e-12 and df < 1e-12
+
+
 @pytest.mark.parametrize("schedule", ["IterDiOi", "IterDiOall"])
 @pytest.mark.parametrize("n,chi,layers,sweeps,seed", [(6, 8, 3, 3, 0), (9, 16, 3, 2, 1)])
 def test_schedule_matches_oracle(gpu, schedule, n, chi, layers, sweeps, seed):
