@@ -594,14 +594,16 @@ __global__ __launch_bounds__(256) void fwd_pass_kernel(FwdArgs<S> a) {
 // live_out) and the LDS exchange between stages are those of fwd_pass_kernel.
 typedef float f4v __attribute__((ext_vector_type(4)));
 
-// Per-type parts of the matrix-core pass.  A tile holds 256 groups: complex64
-// runs 4 waves of 4 column blocks (16 groups each), complex128 8 waves of 2
-// (its 64 KiB tile allows one block per CU; 8 waves make 2 per SIMD).
+// Per-type parts of the matrix-core pass.  A tile holds 256 groups, run by
+// 8 waves of 2 column blocks (16 groups each): complex64 at 6 waves per SIMD
+// (3 blocks per CU while their LDS fits; 4 waves of 4 column blocks ran 3%
+// slower at 4 per SIMD), complex128 at 2 (its 64 KiB tile leaves room for
+// one block per CU).
 template <typename S> struct MfmaT;
 template <> struct MfmaT<float2> {
   using R = float;
   using V4 = f4v;
-  static constexpr int kWaves = 4;
+  static constexpr int kWaves = 8;
   static constexpr int kShift = 3;   // log2 bytes per amplitude
   static QMX_DEV V4 mfma(R x, R y, V4 c) { return __builtin_amdgcn_mfma_f32_16x16x4f32(x, y, c, 0, 0, 0); }
   static QMX_DEV float2 mk(double x, double y) { return make_float2((float)x, (float)y); }
@@ -748,9 +750,10 @@ __attribute__((always_inline)) QMX_DEV void fwd_mfma_body(const FwdArgs<S>& a) {
 
   for (long tile = blockIdx.x; tile < ntiles; tile += gridDim.x) {
     const uint32_t base = (uint32_t)wdeposit((uint64_t)tile, a.tmask);
-    // stage 0's operands come from HBM at the tile's start: without a
-    // prefetch of the next tile's (32 registers) the kernel runs at 4 waves
-    // per SIMD instead of 3, which hides the load latency better
+    // stage 0's operands come from HBM at the tile's start: a prefetch of
+    // the next tile's would cost the occupancy that hides the load latency
+    // (measured with 4 column blocks per wave: 3 instead of 4 waves per
+    // SIMD, slower)
     load0(base);
     for (int st = 0; st < a.nstage; ++st) {
       // lane roles of the stage (uniform: LDS broadcast, then scalar)
@@ -813,7 +816,7 @@ __attribute__((always_inline)) QMX_DEV void fwd_mfma_body(const FwdArgs<S>& a) {
   }
 }
 
-__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4, 4))) void fwd_mfma_kernel(FwdArgs<float2> a) {
+__global__ __launch_bounds__(512) __attribute__((amdgpu_waves_per_eu(6, 6))) void fwd_mfma_kernel(FwdArgs<float2> a) {
   fwd_mfma_body(a);
 }
 __global__ __launch_bounds__(512) __attribute__((amdgpu_waves_per_eu(2, 2))) void fwd_mfma64_kernel(FwdArgs<double2> a) {
@@ -995,7 +998,7 @@ void launch_fwd_pass(const FwdArgs<S>& a, int nblk, int batch, hipStream_t st) {
     // the tile, then one 16 x 16 complex matrix per stage
     shm += (size_t)a.nstage * 256 * sizeof(S);
     if constexpr (sizeof(S) == 8) {
-      hipLaunchKernelGGL(fwd_mfma_kernel, dim3(nblk, batch), dim3(256), shm, st, a);
+      hipLaunchKernelGGL(fwd_mfma_kernel, dim3(nblk, batch), dim3(512), shm, st, a);
       return;
     } else {
       // above the default 64 KiB of dynamic LDS; the limit is always set to
